@@ -1,0 +1,746 @@
+// pt_api.hip -- host runtime of the path-tracing core and the C ABI of
+// include/pt_render.h.  It replaces the host half of the reference's
+// src/cuda/cuda_renderer.cu: validation, scene upload, the per-renderer
+// accumulation state and the add_samples bounce loop.
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "pt_device.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct status_error : std::runtime_error {
+    pt_status code;
+    status_error(pt_status c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+[[noreturn]] void fail(pt_status c, const std::string& m) { throw status_error(c, m); }
+
+void hip_check(hipError_t e, const char *what) {
+    if(e != hipSuccess) fail(PT_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+#define HIP_CHECK(call) hip_check((call), #call)
+
+template<class F> pt_status guarded(F&& f) {
+    try {
+        f();
+        g_last_error.clear();
+        return PT_OK;
+    } catch(const status_error& e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch(const std::bad_alloc&) {
+        g_last_error = "host allocation failed";
+        return PT_ENOMEM;
+    } catch(const std::exception& e) {
+        g_last_error = e.what();
+        return PT_EDEVICE;
+    }
+}
+
+template<class T> class dev_buf {
+public:
+    dev_buf() = default;
+    explicit dev_buf(size_t n) { alloc(n); }
+    ~dev_buf() { release(); }
+    dev_buf(const dev_buf&) = delete;
+    dev_buf& operator=(const dev_buf&) = delete;
+    void alloc(size_t n) {
+        release();
+        if(n > std::numeric_limits<size_t>::max() / sizeof(T)) fail(PT_EINVAL, "device allocation size overflow");
+        count = n;
+        if(n) HIP_CHECK(hipMalloc(reinterpret_cast<void **>(&ptr), n * sizeof(T)));
+    }
+    void upload(const T *src, size_t n) {
+        alloc(n);
+        if(n) HIP_CHECK(hipMemcpy(ptr, src, n * sizeof(T), hipMemcpyHostToDevice));
+    }
+    void release() { if(ptr) (void)hipFree(ptr); ptr = nullptr; count = 0; }
+    T *get() const { return ptr; }
+    size_t size() const { return count; }
+private:
+    T *ptr = nullptr;
+    size_t count = 0;
+};
+
+// ---------------- validation (src/flat/flat_validate.h:10-83) ----------------
+constexpr int stack_limit = 64; // flat_stack_capacity, src/flat/flat_data.h:52
+
+bool finite3(const pt_vec3& v) { return std::isfinite(v.x) && std::isfinite(v.y) && std::isfinite(v.z); }
+float axis(const pt_vec3& v, int a) { return a == 0 ? v.x : a == 1 ? v.y : v.z; }
+bool valid_id(int32_t id, int64_t n) { return id >= 0 && int64_t(id) < n; }
+
+void validate_scene(const pt_scene_desc& s) {
+    auto require = [](bool ok) { if(!ok) fail(PT_ESCENE, "invalid flat scene data"); };
+    require(s.triangle_count >= 0 && s.sphere_count >= 0 && s.node_count >= 0 && s.material_count >= 0 &&
+            s.texture_count >= 0 && s.pixel_bytes >= 0);
+    require((s.triangles || !s.triangle_count) && (s.spheres || !s.sphere_count) && (s.nodes || !s.node_count) &&
+            (s.materials || !s.material_count) && (s.textures || !s.texture_count) && (s.pixels || !s.pixel_bytes));
+    require(s.node_count < int64_t(INT32_MAX) && s.triangle_count < int64_t(INT32_MAX / 2) && s.sphere_count < int64_t(INT32_MAX / 2));
+    for(int64_t i = 0; i < s.texture_count; i++) {
+        const auto& t = s.textures[i];
+        require(finite3(t.color) && finite3(t.scale) && finite3(t.offset));
+        require(t.type >= PT_CONSTANT && t.type <= PT_MAP);
+        if(t.type == PT_IMAGE) {
+            require(t.width > 0 && t.height > 0 && t.byte_offset >= 0);
+            uint64_t end = uint64_t(t.byte_offset) + 3 * uint64_t(t.width) * uint64_t(t.height);
+            require(end <= uint64_t(s.pixel_bytes));
+        } else if(t.type == PT_MAP) {
+            require(valid_id(t.image_id, s.texture_count));
+            require(s.textures[t.image_id].type == PT_IMAGE && (t.clamp == 0 || t.clamp == 1));
+        }
+    }
+    for(int64_t i = 0; i < s.material_count; i++) {
+        const auto& m = s.materials[i];
+        require(m.type >= PT_DIFFUSE && m.type <= PT_LIGHT && finite3(m.albedo));
+        require(std::isfinite(m.fuzz) && std::isfinite(m.ref_idx));
+        if(m.type == PT_DIFFUSE || m.type == PT_LIGHT) require(valid_id(m.texture_id, s.texture_count));
+        if(m.type == PT_GLASS) require(m.ref_idx > 0);
+    }
+    for(int64_t i = 0; i < s.triangle_count; i++) {
+        const auto& t = s.triangles[i];
+        require(valid_id(t.material_id, s.material_count) && (t.smooth == 0 || t.smooth == 1));
+        require(finite3(t.v0) && finite3(t.v1) && finite3(t.v2) && finite3(t.n0) && finite3(t.n1) && finite3(t.n2));
+        require(finite3(t.uv0) && finite3(t.uv1) && finite3(t.uv2));
+    }
+    for(int64_t i = 0; i < s.sphere_count; i++) {
+        const auto& p = s.spheres[i];
+        require(valid_id(p.material_id, s.material_count) && finite3(p.center) && std::isfinite(p.radius) && p.radius > 0);
+    }
+    if(s.root == -1) {
+        require(s.node_count == 0 && s.triangle_count == 0 && s.sphere_count == 0 && s.max_depth == 0);
+        return;
+    }
+    require(valid_id(s.root, s.node_count));
+    std::vector<int> heights(size_t(s.node_count), 0);
+    std::vector<char> active(size_t(s.node_count), 0);
+    int64_t visited = 0;
+    // Iterative post-order walk with the recursive checker's rules.
+    struct frame { int32_t id; int depth; int stage; int left_height; };
+    std::vector<frame> stack;
+    int result = 0;
+    auto enter = [&](int32_t id, int depth) -> bool {
+        require(valid_id(id, s.node_count) && depth <= stack_limit && !active[size_t(id)]);
+        if(heights[size_t(id)]) { require(depth + heights[size_t(id)] - 1 <= stack_limit); result = heights[size_t(id)]; return false; }
+        active[size_t(id)] = 1;
+        visited++;
+        const auto& n = s.nodes[id];
+        require(finite3(n.minimum) && finite3(n.maximum));
+        for(int a = 0; a < 3; a++) require(axis(n.minimum, a) <= axis(n.maximum, a));
+        stack.push_back({id, depth, 0, 0});
+        return true;
+    };
+    if(enter(s.root, 1)) {
+        while(!stack.empty()) {
+            frame& f = stack.back();
+            const auto& n = s.nodes[f.id];
+            if(n.type == PT_BRANCH) {
+                if(f.stage == 0) {
+                    f.stage = 1;
+                    int32_t child = n.left; int d = f.depth + 1;
+                    if(enter(child, d)) continue;
+                    // cached height returned in result
+                }
+                if(f.stage == 1) {
+                    f.left_height = result;
+                    f.stage = 2;
+                    if(n.left != n.right) {
+                        int32_t child = n.right; int d = f.depth + 1;
+                        if(enter(child, d)) continue;
+                    }
+                }
+                // stage 2: right height is in result (or equals left for an alias)
+                int right_height = n.left == n.right ? f.left_height : result;
+                int height = 1 + std::max(f.left_height, right_height);
+                for(int32_t child : {n.left, n.right}) {
+                    const auto& c = s.nodes[child];
+                    for(int a = 0; a < 3; a++)
+                        require(axis(n.minimum, a) <= axis(c.minimum, a) && axis(n.maximum, a) >= axis(c.maximum, a));
+                }
+                active[size_t(f.id)] = 0; heights[size_t(f.id)] = height; result = height;
+                stack.pop_back();
+            } else {
+                if(n.type == PT_TRIANGLE) {
+                    require(valid_id(n.primitive, s.triangle_count));
+                    const auto& t = s.triangles[n.primitive];
+                    for(const pt_vec3& p : {t.v0, t.v1, t.v2})
+                        for(int a = 0; a < 3; a++) require(axis(p, a) >= axis(n.minimum, a) && axis(p, a) <= axis(n.maximum, a));
+                } else if(n.type == PT_SPHERE) {
+                    require(valid_id(n.primitive, s.sphere_count));
+                    const auto& p = s.spheres[n.primitive];
+                    for(int a = 0; a < 3; a++)
+                        require(axis(p.center, a) - p.radius >= axis(n.minimum, a) && axis(p.center, a) + p.radius <= axis(n.maximum, a));
+                } else require(false);
+                active[size_t(f.id)] = 0; heights[size_t(f.id)] = 1; result = 1;
+                stack.pop_back();
+            }
+        }
+    }
+    require(result == s.max_depth && visited == s.node_count);
+}
+
+// src/renderer/render_validation.h:7-22
+size_t checked_pixel_count(int width, int height) {
+    if(width <= 0 || height <= 0 || uint64_t(width) * uint64_t(height) > uint64_t(INT32_MAX))
+        fail(PT_EINVAL, "image dimensions exceed supported pixel count");
+    return size_t(width) * size_t(height);
+}
+void validate_mode(int32_t mode) {
+    if(mode < PT_DEBUG_SHADED || mode > PT_DEBUG_INTERSECTIONS) fail(PT_EINVAL, "invalid render debug mode");
+}
+void validate_camera(const pt_flat_camera& c) {
+    for(const pt_vec3& v : {c.origin, c.lower_left_corner, c.horizontal, c.vertical, c.u, c.v})
+        if(!finite3(v)) fail(PT_EINVAL, "camera contains nonfinite values");
+    if(!std::isfinite(c.lens_radius) || c.lens_radius < 0 || !std::isfinite(c.time0) || !std::isfinite(c.time1) || c.time1 < c.time0)
+        fail(PT_EINVAL, "invalid camera lens/shutter");
+}
+
+// ---------------- light table (src/renderer/scene_view.h:10-28) ----------------
+std::vector<pt::light_rec> scene_lights(const pt_scene_desc& s) {
+    std::vector<pt::light_rec> lights;
+    double area = 0;
+    auto add = [&](int32_t type, int32_t id, double size) {
+        if(!(size > 0) || !std::isfinite(size)) return;
+        area += size;
+        if(!std::isfinite(area) || area > FLT_MAX) fail(PT_EINVAL, "light area exceeds range");
+        lights.push_back({type, id, float(area)});
+    };
+    for(int64_t i = 0; i < s.triangle_count; i++) {
+        const auto& t = s.triangles[i];
+        if(s.materials[t.material_id].type != PT_LIGHT) continue;
+        pt_vec3 e1{t.v1.x - t.v0.x, t.v1.y - t.v0.y, t.v1.z - t.v0.z}, e2{t.v2.x - t.v0.x, t.v2.y - t.v0.y, t.v2.z - t.v0.z};
+        pt_vec3 c{e1.y * e2.z - e1.z * e2.y, -(e1.x * e2.z - e1.z * e2.x), e1.x * e2.y - e1.y * e2.x};
+        float len = std::sqrt(c.x * c.x + c.y * c.y + c.z * c.z);
+        add(PT_TRIANGLE, int32_t(i), 0.5 * len);
+    }
+    for(int64_t i = 0; i < s.sphere_count; i++) {
+        const auto& p = s.spheres[i];
+        if(s.materials[p.material_id].type != PT_LIGHT) continue;
+        add(PT_SPHERE, int32_t(i), 4 * M_PI * double(p.radius) * p.radius);
+    }
+    return lights;
+}
+
+// ---------------- host-libm tables ----------------
+// cos/sin of every angle cosine_direction and sphere-light sampling can form:
+// 2*float(M_PI)*v with v = k*2^-24 (light_sampling.h:20,47).  Generated with
+// the host libm, so device results equal the host build's bit for bit.
+constexpr size_t trig_entries = size_t(1) << 24;
+
+const std::vector<float2>& host_trig_table() {
+    static std::once_flag once;
+    static std::vector<float2> table;
+    std::call_once(once, [] {
+        table.resize(trig_entries);
+        unsigned threads = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        std::vector<std::thread> pool;
+        for(unsigned t = 0; t < threads; t++) {
+            pool.emplace_back([t, threads] {
+                const float two_pi = 2 * float(M_PI);
+                size_t chunk = trig_entries / threads;
+                size_t begin = t * chunk, end = t + 1 == threads ? trig_entries : begin + chunk;
+                for(size_t k = begin; k < end; k++) {
+                    float v = float(uint32_t(k)) * 0x1p-24f;
+                    float angle = two_pi * v;
+                    table[k] = make_float2(cosf(angle), sinf(angle));
+                }
+            });
+        }
+        for(auto& th : pool) th.join();
+    });
+    return table;
+}
+
+const float2 *device_trig_table() {
+    static std::mutex lock;
+    static std::map<int, float2 *> tables;
+    int device = 0;
+    HIP_CHECK(hipGetDevice(&device));
+    std::lock_guard<std::mutex> guard(lock);
+    auto it = tables.find(device);
+    if(it != tables.end()) return it->second;
+    const auto& host = host_trig_table();
+    float2 *ptr = nullptr;
+    HIP_CHECK(hipMalloc(reinterpret_cast<void **>(&ptr), trig_entries * sizeof(float2)));
+    HIP_CHECK(hipMemcpy(ptr, host.data(), trig_entries * sizeof(float2), hipMemcpyHostToDevice));
+    tables[device] = ptr; // kept for the life of the process
+    return ptr;
+}
+
+void require_device() {
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if(e != hipSuccess) fail(PT_EDEVICE, std::string("HIP device unavailable: ") + hipGetErrorString(e));
+    if(count < 1) fail(PT_EDEVICE, "HIP device unavailable: no HIP devices");
+}
+
+// ---------------- uploaded scene (cuda_renderer.cu:36-54) ----------------
+struct device_scene {
+    dev_buf<float4> bnodes, tri_isect, sph_isect;
+    dev_buf<pt_flat_triangle> triangles;
+    dev_buf<pt_flat_sphere> spheres;
+    dev_buf<pt_flat_node> nodes;
+    dev_buf<pt_flat_material> materials;
+    dev_buf<pt_flat_texture> textures;
+    dev_buf<uint8_t> pixels;
+    dev_buf<pt::light_rec> lights;
+    dev_buf<float> log2_table;
+    pt::scene_dev view{};
+    int64_t branch_count = 0;
+
+    device_scene(const pt_scene_desc& s, bool with_tables) {
+        // Traversal layout: branch nodes renumbered in array order, children
+        // encoded as branch indices or leaf references.
+        std::vector<int32_t> branch_id(size_t(s.node_count), -1);
+        int64_t branches = 0;
+        for(int64_t i = 0; i < s.node_count; i++) if(s.nodes[i].type == PT_BRANCH) branch_id[size_t(i)] = int32_t(branches++);
+        branch_count = branches;
+        auto ref_of = [&](int32_t node) -> int32_t {
+            const auto& n = s.nodes[node];
+            if(n.type == PT_BRANCH) return branch_id[size_t(node)];
+            return pt::leaf_ref(n.primitive, n.type == PT_SPHERE);
+        };
+        std::vector<float4> bn(size_t(2 * branches));
+        for(int64_t i = 0; i < s.node_count; i++) {
+            const auto& n = s.nodes[i];
+            if(n.type != PT_BRANCH) continue;
+            size_t b = size_t(branch_id[size_t(i)]);
+            int32_t l = ref_of(n.left), r = ref_of(n.right);
+            float lf, rf;
+            std::memcpy(&lf, &l, 4); std::memcpy(&rf, &r, 4);
+            bn[2 * b] = make_float4(n.minimum.x, n.minimum.y, n.minimum.z, n.maximum.x);
+            bn[2 * b + 1] = make_float4(n.maximum.y, n.maximum.z, lf, rf);
+        }
+        // Triangle test data: v0, e1 = v1-v0, e2 = v2-v0 and |e1 x e2| exactly
+        // as flat_intersect.h:9-12 computes them, plus 1e-7f*|e1 x e2| (:16).
+        std::vector<float4> ti(size_t(3 * s.triangle_count));
+        for(int64_t i = 0; i < s.triangle_count; i++) {
+            const auto& t = s.triangles[i];
+            float e1x = t.v1.x - t.v0.x, e1y = t.v1.y - t.v0.y, e1z = t.v1.z - t.v0.z;
+            float e2x = t.v2.x - t.v0.x, e2y = t.v2.y - t.v0.y, e2z = t.v2.z - t.v0.z;
+            float cx = e1y * e2z - e1z * e2y, cy = -(e1x * e2z - e1z * e2x), cz = e1x * e2y - e1y * e2x;
+            float nl = std::sqrt(cx * cx + cy * cy + cz * cz);
+            ti[size_t(3 * i)] = make_float4(t.v0.x, t.v0.y, t.v0.z, 0.0000001f * nl);
+            ti[size_t(3 * i + 1)] = make_float4(e1x, e1y, e1z, nl);
+            ti[size_t(3 * i + 2)] = make_float4(e2x, e2y, e2z, 0);
+        }
+        std::vector<float4> si(size_t(s.sphere_count));
+        for(int64_t i = 0; i < s.sphere_count; i++) {
+            const auto& p = s.spheres[i];
+            si[size_t(i)] = make_float4(p.center.x, p.center.y, p.center.z, p.radius);
+        }
+        auto lights_host = scene_lights(s);
+        bnodes.upload(bn.data(), bn.size());
+        tri_isect.upload(ti.data(), ti.size());
+        sph_isect.upload(si.data(), si.size());
+        triangles.upload(s.triangles, size_t(s.triangle_count));
+        spheres.upload(s.spheres, size_t(s.sphere_count));
+        nodes.upload(s.nodes, size_t(s.node_count));
+        materials.upload(s.materials, size_t(s.material_count));
+        textures.upload(s.textures, size_t(s.texture_count));
+        pixels.upload(s.pixels, size_t(s.pixel_bytes));
+        lights.upload(lights_host.data(), lights_host.size());
+        view.bnodes = bnodes.get(); view.tri_isect = tri_isect.get(); view.sph_isect = sph_isect.get();
+        view.root_ref = s.root < 0 ? pt::empty_root : ref_of(s.root);
+        view.triangles = triangles.get(); view.spheres = spheres.get(); view.nodes = nodes.get();
+        view.node_count = int32_t(s.node_count);
+        view.materials = materials.get(); view.textures = textures.get(); view.pixels = pixels.get();
+        view.lights = lights.get(); view.light_count = int32_t(lights_host.size());
+        view.light_area = lights_host.empty() ? 0 : lights_host.back().cumulative_area;
+        view.stack_capacity = std::max(2, std::min(stack_limit, int(s.max_depth)));
+        view.trig = nullptr; view.log2_table = nullptr; view.log2_count = 0;
+        if(with_tables) {
+            view.trig = device_trig_table();
+            // log2f(1+n) of the host libm for every count a traversal can reach.
+            size_t n = size_t(std::min<int64_t>(s.node_count + 2, int64_t(1) << 26));
+            std::vector<float> lg(n);
+            for(size_t k = 0; k < n; k++) lg[k] = log2f(1.0f + float(k));
+            log2_table.upload(lg.data(), lg.size());
+            view.log2_table = log2_table.get();
+            view.log2_count = int32_t(n);
+        }
+    }
+};
+
+int device_cu_count() {
+    int device = 0;
+    HIP_CHECK(hipGetDevice(&device));
+    hipDeviceProp_t props{};
+    HIP_CHECK(hipGetDeviceProperties(&props, device));
+    return std::max(1, props.multiProcessorCount);
+}
+
+} // namespace
+
+// ---------------- renderer (cuda_renderer.cu:142-247) ----------------
+struct pt_renderer {
+    std::unique_ptr<device_scene> scene;
+    pt::frame_params frame{};
+    uint64_t seed = 0;
+    int32_t samples = 0;
+    uint64_t rays = 0;
+    float milliseconds = 0;
+    dim3 block = dim3(16, 16);
+    hipStream_t stream = nullptr;
+    hipEvent_t start = nullptr, stop = nullptr;
+    std::vector<hipEvent_t> kernel_events;
+    size_t pixels = 0;
+    dev_buf<float> sum;
+    dev_buf<uint64_t> rng;
+    dev_buf<uint8_t> rgba;
+    dev_buf<pt::loop_ctl> ctl;
+    pt::loop_ctl *ctl_host = nullptr;   // pinned mirror
+    // wavefront state (shaded mode only)
+    dev_buf<float4> ray_o, ray_d, thr, rad, sh_o, sh_d, sh_c;
+    dev_buf<int2> ctr;
+    dev_buf<int4> hit;
+    dev_buf<int32_t> queue0, queue1, shadow_queue;
+    pt::paths_dev paths{};
+    pt_tuning tuning{256, 256, 4, 8, 0};
+    int cus = 1;
+    pt_launch_stats last{};
+
+    ~pt_renderer() {
+        if(stream) (void)hipStreamSynchronize(stream);
+        if(start) (void)hipEventDestroy(start);
+        if(stop) (void)hipEventDestroy(stop);
+        for(auto e : kernel_events) (void)hipEventDestroy(e);
+        if(ctl_host) (void)hipHostFree(ctl_host);
+        if(stream) (void)hipStreamDestroy(stream);
+    }
+
+    void reset(const pt_flat_camera& camera) {
+        validate_camera(camera);
+        pt::launch_init_pixels(sum.get(), rng.get(), int64_t(pixels), seed, ctl.get(), stream);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipStreamSynchronize(stream));
+        frame.camera = camera; samples = 0; milliseconds = 0; rays = 0;
+    }
+
+    void read_ctl() {
+        HIP_CHECK(hipMemcpyAsync(ctl_host, ctl.get(), sizeof(pt::loop_ctl), hipMemcpyDeviceToHost, stream));
+        HIP_CHECK(hipStreamSynchronize(stream));
+    }
+
+    void add_samples(int32_t count) {
+        if(count <= 0 || count > INT32_MAX - samples) fail(PT_EINVAL, "invalid accumulated sample count");
+        HIP_CHECK(hipEventRecord(start, stream));
+        pt_launch_stats st{};
+        if(frame.debug != PT_DEBUG_SHADED) {
+            pt::launch_debug(scene->view, frame, sum.get(), rng.get(), count, &ctl.get()->failure, block, stream);
+            HIP_CHECK(hipGetLastError());
+            pt::launch_add_rays(ctl.get(), (unsigned long long)pixels * (unsigned long long)count, stream);
+            HIP_CHECK(hipEventRecord(stop, stream));
+            read_ctl();
+            st.iterations = 1;
+            st.extend_rays = uint64_t(pixels) * uint64_t(count);
+        } else {
+            pt::launch_start(scene->view, frame, paths, ctl.get(), count, stream);
+            HIP_CHECK(hipGetLastError());
+            int parity = 0;
+            const int trace_grid = cus * std::max(1, tuning.blocks_per_cu);
+            const int shade_grid = int(std::min<size_t>((pixels + size_t(tuning.shade_block) - 1) / size_t(tuning.shade_block),
+                                                        size_t(cus) * 16));
+            const int interval = std::max(1, tuning.check_interval);
+            if(tuning.time_kernels && kernel_events.size() < size_t(4 * interval)) {
+                for(auto e : kernel_events) (void)hipEventDestroy(e);
+                kernel_events.assign(size_t(4 * interval), nullptr);
+                for(auto& e : kernel_events) HIP_CHECK(hipEventCreate(&e));
+            }
+            for(;;) {
+                for(int k = 0; k < interval; k++) {
+                    if(tuning.time_kernels) HIP_CHECK(hipEventRecord(kernel_events[size_t(4 * k)], stream));
+                    pt::launch_trace(scene->view, frame, paths, ctl.get(), parity, trace_grid, tuning.trace_block, stream);
+                    if(tuning.time_kernels) HIP_CHECK(hipEventRecord(kernel_events[size_t(4 * k + 1)], stream));
+                    pt::launch_bookkeep(ctl.get(), parity, stream);
+                    if(tuning.time_kernels) HIP_CHECK(hipEventRecord(kernel_events[size_t(4 * k + 2)], stream));
+                    pt::launch_shade(scene->view, frame, paths, ctl.get(), parity, std::max(1, shade_grid), tuning.shade_block, stream);
+                    if(tuning.time_kernels) HIP_CHECK(hipEventRecord(kernel_events[size_t(4 * k + 3)], stream));
+                    parity ^= 1;
+                    st.trace_launches++;
+                }
+                HIP_CHECK(hipGetLastError());
+                read_ctl();
+                if(tuning.time_kernels) {
+                    for(int k = 0; k < interval; k++) {
+                        float a = 0, b = 0;
+                        HIP_CHECK(hipEventElapsedTime(&a, kernel_events[size_t(4 * k)], kernel_events[size_t(4 * k + 1)]));
+                        HIP_CHECK(hipEventElapsedTime(&b, kernel_events[size_t(4 * k + 2)], kernel_events[size_t(4 * k + 3)]));
+                        st.trace_ms += a; st.shade_ms += b;
+                    }
+                }
+                if(ctl_host->n_ext[parity] == 0 || ctl_host->failure) break;
+            }
+            HIP_CHECK(hipEventRecord(stop, stream));
+            HIP_CHECK(hipEventSynchronize(stop));
+            st.iterations = int32_t(ctl_host->iterations);
+            st.extend_rays = ctl_host->ext_rays;
+            st.shadow_rays = ctl_host->shadow_rays;
+        }
+        HIP_CHECK(hipEventSynchronize(stop));
+        if(ctl_host->failure) fail(PT_EOVERFLOW, "HIP traversal stack overflow");
+        float elapsed = 0;
+        HIP_CHECK(hipEventElapsedTime(&elapsed, start, stop));
+        milliseconds += elapsed;
+        samples += count;
+        rays = ctl_host->rays;
+        last = st;
+    }
+};
+
+namespace {
+
+void allocate_state(pt_renderer& r, bool wavefront) {
+    size_t n = r.pixels;
+    r.sum.alloc(3 * n);
+    r.rng.alloc(n);
+    r.rgba.alloc(4 * n);
+    r.ctl.alloc(1);
+    HIP_CHECK(hipMemset(r.ctl.get(), 0, sizeof(pt::loop_ctl)));
+    HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&r.ctl_host), sizeof(pt::loop_ctl), hipHostMallocDefault));
+    std::memset(r.ctl_host, 0, sizeof(pt::loop_ctl));
+    if(!wavefront) return;
+    r.ray_o.alloc(n); r.ray_d.alloc(n); r.thr.alloc(n); r.rad.alloc(n);
+    r.sh_o.alloc(n); r.sh_d.alloc(n); r.sh_c.alloc(n);
+    r.ctr.alloc(n); r.hit.alloc(n);
+    r.queue0.alloc(n); r.queue1.alloc(n); r.shadow_queue.alloc(n);
+    auto& p = r.paths;
+    p.ray_o = r.ray_o.get(); p.ray_d = r.ray_d.get(); p.thr = r.thr.get(); p.rad = r.rad.get();
+    p.ctr = r.ctr.get(); p.hit = r.hit.get();
+    p.sh_o = r.sh_o.get(); p.sh_d = r.sh_d.get(); p.sh_c = r.sh_c.get();
+    p.rng = r.rng.get(); p.sum = r.sum.get();
+    p.queue[0] = r.queue0.get(); p.queue[1] = r.queue1.get(); p.shadow_queue = r.shadow_queue.get();
+}
+
+void check_grid(const pt_renderer& r, unsigned bx, unsigned by) {
+    int device = 0;
+    HIP_CHECK(hipGetDevice(&device));
+    hipDeviceProp_t props{};
+    HIP_CHECK(hipGetDeviceProperties(&props, device));
+    unsigned gx = (unsigned(r.frame.width) + bx - 1) / bx, gy = (unsigned(r.frame.height) + by - 1) / by;
+    if(gx > unsigned(props.maxGridSize[0]) || gy > unsigned(props.maxGridSize[1]))
+        fail(PT_EINVAL, "image exceeds HIP grid limits");
+}
+
+void query(const pt_scene_desc *scene, const pt_ray *rays, size_t count, int32_t brute, pt_trace_result *results,
+           int32_t *boxes, int32_t *prims) {
+    if(!scene) fail(PT_EINVAL, "null scene");
+    validate_scene(*scene);
+    require_device();
+    if(count == 0) return;
+    if(count > size_t(INT32_MAX)) fail(PT_EINVAL, "too many validation rays");
+    if(!rays || !results) fail(PT_EINVAL, "null ray or result buffer");
+    device_scene uploaded(*scene, false);
+    dev_buf<pt_ray> input; input.upload(rays, count);
+    dev_buf<pt_trace_result> output(count);
+    dev_buf<int32_t> b(boxes ? count : 0), p(prims ? count : 0);
+    pt::launch_query(uploaded.view, input.get(), output.get(), b.get(), p.get(), int32_t(count), brute != 0, nullptr);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipDeviceSynchronize());
+    HIP_CHECK(hipMemcpy(results, output.get(), count * sizeof(pt_trace_result), hipMemcpyDeviceToHost));
+    if(boxes) HIP_CHECK(hipMemcpy(boxes, b.get(), count * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if(prims) HIP_CHECK(hipMemcpy(prims, p.get(), count * sizeof(int32_t), hipMemcpyDeviceToHost));
+}
+
+} // namespace
+
+extern "C" {
+
+const char *pt_last_error(void) { return g_last_error.c_str(); }
+
+pt_status pt_device_available(char *description, size_t capacity) {
+    std::string text;
+    pt_status st = PT_OK;
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if(e != hipSuccess) { text = hipGetErrorString(e); st = PT_EDEVICE; }
+    else if(count < 1) { text = "no HIP devices"; st = PT_EDEVICE; }
+    else {
+        int current = 0;
+        hipDeviceProp_t props{};
+        e = hipGetDevice(&current);
+        if(e == hipSuccess) e = hipGetDeviceProperties(&props, current);
+        if(e != hipSuccess) { text = hipGetErrorString(e); st = PT_EDEVICE; }
+        else text = std::string(props.name) + " (" + props.gcnArchName + ")";
+    }
+    if(description && capacity) {
+        size_t n = std::min(capacity - 1, text.size());
+        std::memcpy(description, text.data(), n);
+        description[n] = 0;
+    }
+    g_last_error = st == PT_OK ? "" : text;
+    return st;
+}
+
+pt_status pt_renderer_create(const pt_scene_desc *scene, const pt_flat_camera *camera, const pt_settings *settings,
+                             pt_renderer **out) {
+    return guarded([&] {
+        if(!out) fail(PT_EINVAL, "null output handle");
+        *out = nullptr;
+        if(!scene || !camera || !settings) fail(PT_EINVAL, "null argument");
+        // cuda_renderer.cu:177-179 order: scene, camera, mode, size, device.
+        validate_scene(*scene);
+        validate_camera(*camera);
+        validate_mode(settings->debug);
+        size_t pixels = checked_pixel_count(settings->width, settings->height);
+        if(settings->max_path_depth < 0) fail(PT_EINVAL, "invalid path depth");
+        require_device();
+        auto r = std::make_unique<pt_renderer>();
+        r->pixels = pixels;
+        r->seed = settings->seed;
+        r->frame.width = settings->width; r->frame.height = settings->height;
+        r->frame.max_depth = settings->max_path_depth ? settings->max_path_depth : 50;
+        r->frame.direct_light = settings->direct_light ? 1 : 0;
+        r->frame.brute = settings->brute ? 1 : 0;
+        r->frame.debug = settings->debug;
+        r->frame.camera = *camera;
+        check_grid(*r, 16, 16);
+        HIP_CHECK(hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking));
+        HIP_CHECK(hipEventCreate(&r->start));
+        HIP_CHECK(hipEventCreate(&r->stop));
+        r->cus = device_cu_count();
+        r->scene = std::make_unique<device_scene>(*scene, true);
+        allocate_state(*r, settings->debug == PT_DEBUG_SHADED);
+        r->reset(*camera);
+        *out = r.release();
+    });
+}
+
+pt_status pt_renderer_destroy(pt_renderer *renderer) {
+    return guarded([&] { delete renderer; });
+}
+
+pt_status pt_renderer_reset(pt_renderer *renderer, const pt_flat_camera *camera) {
+    return guarded([&] {
+        if(!renderer || !camera) fail(PT_EINVAL, "null argument");
+        renderer->reset(*camera);
+    });
+}
+
+pt_status pt_renderer_add_samples(pt_renderer *renderer, int32_t count) {
+    return guarded([&] {
+        if(!renderer) fail(PT_EINVAL, "null renderer");
+        renderer->add_samples(count);
+    });
+}
+
+pt_status pt_renderer_set_block_shape(pt_renderer *renderer, int32_t width, int32_t height) {
+    return guarded([&] {
+        if(!renderer) fail(PT_EINVAL, "null renderer");
+        if(!((width == 8 && height == 8) || (width == 16 && height == 8) || (width == 16 && height == 16) || (width == 32 && height == 8)))
+            fail(PT_EINVAL, "HIP block must be 8x8, 16x8, 16x16 or 32x8");
+        if(renderer->block.x == unsigned(width) && renderer->block.y == unsigned(height)) return;
+        check_grid(*renderer, unsigned(width), unsigned(height));
+        renderer->block = dim3(unsigned(width), unsigned(height));
+    });
+}
+
+pt_status pt_renderer_read_image(const pt_renderer *renderer, float *rgb, size_t float_count) {
+    return guarded([&] {
+        if(!renderer || !rgb) fail(PT_EINVAL, "null argument");
+        if(float_count < 3 * renderer->pixels) fail(PT_EINVAL, "image buffer is too small");
+        HIP_CHECK(hipMemcpy(rgb, renderer->sum.get(), 3 * renderer->pixels * sizeof(float), hipMemcpyDeviceToHost));
+        if(renderer->samples) {
+            // value /= float(samples): multiply by float(1.0/samples) (vec3.h:152-158)
+            float k = float(1.0 / double(float(renderer->samples)));
+            for(size_t i = 0; i < 3 * renderer->pixels; i++) rgb[i] = rgb[i] * k;
+        }
+    });
+}
+
+pt_status pt_renderer_write_rgba_device(const pt_renderer *renderer, void *device_rgba, size_t bytes) {
+    return guarded([&] {
+        if(!renderer) fail(PT_EINVAL, "null renderer");
+        if(!device_rgba || bytes < 4 * renderer->pixels) fail(PT_EINVAL, "HIP RGBA destination is too small");
+        pt::launch_pack(renderer->sum.get(), static_cast<uint8_t *>(device_rgba), int64_t(renderer->pixels), renderer->samples, renderer->stream);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipStreamSynchronize(renderer->stream));
+    });
+}
+
+pt_status pt_renderer_read_rgba(const pt_renderer *renderer, uint8_t *rgba, size_t bytes) {
+    return guarded([&] {
+        if(!renderer || !rgba) fail(PT_EINVAL, "null argument");
+        if(bytes < 4 * renderer->pixels) fail(PT_EINVAL, "RGBA buffer is too small");
+        pt::launch_pack(renderer->sum.get(), renderer->rgba.get(), int64_t(renderer->pixels), renderer->samples, renderer->stream);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipStreamSynchronize(renderer->stream));
+        HIP_CHECK(hipMemcpy(rgba, renderer->rgba.get(), 4 * renderer->pixels, hipMemcpyDeviceToHost));
+    });
+}
+
+pt_status pt_renderer_stats(const pt_renderer *renderer, int32_t *samples, float *kernel_ms, uint64_t *rays) {
+    return guarded([&] {
+        if(!renderer) fail(PT_EINVAL, "null renderer");
+        if(samples) *samples = renderer->samples;
+        if(kernel_ms) *kernel_ms = renderer->milliseconds;
+        if(rays) *rays = renderer->rays;
+    });
+}
+
+pt_status pt_query_rays(const pt_scene_desc *scene, const pt_ray *rays, size_t count, int32_t brute, pt_trace_result *results) {
+    return guarded([&] { query(scene, rays, count, brute, results, nullptr, nullptr); });
+}
+
+pt_status pt_query_rays_stats(const pt_scene_desc *scene, const pt_ray *rays, size_t count, int32_t brute,
+                              pt_trace_result *results, int32_t *boxes, int32_t *primitives) {
+    return guarded([&] { query(scene, rays, count, brute, results, boxes, primitives); });
+}
+
+pt_status pt_renderer_read_state(const pt_renderer *renderer, float *sum, uint64_t *rng_state, size_t pixels) {
+    return guarded([&] {
+        if(!renderer) fail(PT_EINVAL, "null renderer");
+        if(pixels != renderer->pixels) fail(PT_EINVAL, "pixel count mismatch");
+        if(sum) HIP_CHECK(hipMemcpy(sum, renderer->sum.get(), 3 * pixels * sizeof(float), hipMemcpyDeviceToHost));
+        if(rng_state) HIP_CHECK(hipMemcpy(rng_state, renderer->rng.get(), pixels * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    });
+}
+
+pt_status pt_renderer_write_state(pt_renderer *renderer, const float *sum, const uint64_t *rng_state, size_t pixels,
+                                  int32_t samples, uint64_t rays) {
+    return guarded([&] {
+        if(!renderer || !sum || !rng_state) fail(PT_EINVAL, "null argument");
+        if(pixels != renderer->pixels) fail(PT_EINVAL, "pixel count mismatch");
+        if(samples < 0) fail(PT_EINVAL, "invalid accumulated sample count");
+        HIP_CHECK(hipMemcpy(renderer->sum.get(), sum, 3 * pixels * sizeof(float), hipMemcpyHostToDevice));
+        HIP_CHECK(hipMemcpy(renderer->rng.get(), rng_state, pixels * sizeof(uint64_t), hipMemcpyHostToDevice));
+        pt::loop_ctl c{};
+        c.rays = rays;
+        HIP_CHECK(hipMemcpy(renderer->ctl.get(), &c, sizeof(c), hipMemcpyHostToDevice));
+        renderer->samples = samples; renderer->rays = rays;
+    });
+}
+
+pt_status pt_renderer_last_launch_stats(const pt_renderer *renderer, pt_launch_stats *stats) {
+    return guarded([&] {
+        if(!renderer || !stats) fail(PT_EINVAL, "null argument");
+        *stats = renderer->last;
+    });
+}
+
+pt_status pt_renderer_set_tuning(pt_renderer *renderer, const pt_tuning *tuning) {
+    return guarded([&] {
+        if(!renderer || !tuning) fail(PT_EINVAL, "null argument");
+        pt_tuning t = renderer->tuning;
+        auto block_ok = [](int32_t b) { return b >= 64 && b <= 256 && b % 64 == 0; };
+        if(tuning->trace_block) { if(!block_ok(tuning->trace_block)) fail(PT_EINVAL, "trace block must be 64..256, multiple of 64"); t.trace_block = tuning->trace_block; }
+        if(tuning->shade_block) { if(!block_ok(tuning->shade_block)) fail(PT_EINVAL, "shade block must be 64..256, multiple of 64"); t.shade_block = tuning->shade_block; }
+        if(tuning->blocks_per_cu) { if(tuning->blocks_per_cu < 1 || tuning->blocks_per_cu > 64) fail(PT_EINVAL, "blocks per CU must be 1..64"); t.blocks_per_cu = tuning->blocks_per_cu; }
+        if(tuning->check_interval) { if(tuning->check_interval < 1 || tuning->check_interval > 4096) fail(PT_EINVAL, "check interval must be 1..4096"); t.check_interval = tuning->check_interval; }
+        t.time_kernels = tuning->time_kernels ? 1 : 0;
+        renderer->tuning = t;
+    });
+}
+
+}

@@ -1,0 +1,105 @@
+// pt_device.h -- device-side scene view, path-state arrays and launch
+// interface shared by the kernels (pt_kernels.hip) and the host runtime
+// (pt_api.hip).  Internal header; the public boundary is include/pt_render.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pt_render.h"
+
+namespace pt {
+
+// Light table record (src/renderer/scene_view.h:8).
+struct light_rec { int32_t type, primitive; float cumulative_area; };
+static_assert(sizeof(light_rec) == 12, "flat_light layout");
+
+// Child reference used by the traversal layout: >= 0 is a branch node in the
+// compact branch array; < 0 is a leaf, ~ref = primitive*2 + (sphere ? 1 : 0).
+__host__ __device__ inline int32_t leaf_ref(int32_t primitive, bool sphere) { return ~(primitive * 2 + (sphere ? 1 : 0)); }
+
+// Everything a kernel needs to read the uploaded scene.
+struct scene_dev {
+    // Traversal layout (built from the flat arrays at upload).
+    const float4 *bnodes;      // 2 per branch: {lo.x,lo.y,lo.z,hi.x} {hi.y,hi.z,left_ref,right_ref}
+    const float4 *tri_isect;   // 3 per triangle: {v0, 1e-7f*|e1 x e2|} {e1, |e1 x e2|} {e2, 0}
+    const float4 *sph_isect;   // 1 per sphere: {center, radius}
+    int32_t root_ref;          // -2^31 when the scene is empty
+    // Original transfer arrays (shading, brute mode).
+    const pt_flat_triangle *triangles;
+    const pt_flat_sphere *spheres;
+    const pt_flat_node *nodes;
+    int32_t node_count;
+    const pt_flat_material *materials;
+    const pt_flat_texture *textures;
+    const uint8_t *pixels;
+    const light_rec *lights;
+    int32_t light_count;
+    float light_area;
+    // Host-libm tables: cos/sin of the 2^24 angles 2*float(pi)*k*2^-24, and
+    // log2f(1+n) for n < log2_count.
+    const float2 *trig;
+    const float *log2_table;
+    int32_t log2_count;
+    int32_t stack_capacity;    // per-thread traversal stack entries (<= 64)
+};
+
+constexpr int32_t empty_root = INT32_MIN;
+
+// Per-pixel path state, structure of arrays in HBM.
+struct paths_dev {
+    float4 *ray_o;      // origin.xyz, time
+    float4 *ray_d;      // direction.xyz, unused
+    float4 *thr;        // throughput.xyz, previous_pdf
+    float4 *rad;        // radiance.xyz, unused
+    int2 *ctr;          // depth, samples left in this add_samples call
+    int4 *hit;          // t, u, v (bit patterns), code = primitive*2+sphere or -1
+    float4 *sh_o;       // shadow origin.xyz, t_max
+    float4 *sh_d;       // shadow direction.xyz, unused
+    float4 *sh_c;       // unblocked contribution.xyz, unused
+    uint64_t *rng;      // PCG state; the increment is (pixel << 1) | 1
+    float *sum;         // 3 floats per pixel (vec3 sum[], cuda_renderer.cu:145)
+    int32_t *queue[2];  // extension queues (pixel ids), ping-pong
+    int32_t *shadow_queue;
+};
+
+// Device control block of the bounce loop.
+struct loop_ctl {
+    uint32_t n_ext[2];
+    uint32_t n_shadow;
+    uint32_t work;           // persistent-fetch cursor of the trace kernel
+    unsigned long long rays; // trace_world calls, shadow queries included
+    unsigned long long ext_rays, shadow_rays;
+    uint32_t iterations;
+    int32_t failure;         // traversal stack overflow
+};
+
+struct frame_params {
+    pt_flat_camera camera;
+    int32_t width, height;
+    int32_t max_depth;      // path-length limit (50 in the reference)
+    int32_t direct_light;
+    int32_t brute;
+    int32_t debug;
+};
+
+// ---- launchers (pt_kernels.hip) ----
+void launch_query(const scene_dev& s, const pt_ray *rays, pt_trace_result *out, int32_t *boxes, int32_t *prims,
+                  int32_t count, bool brute, hipStream_t stream);
+void launch_init_pixels(float *sum, uint64_t *rng, int64_t count, uint64_t seed, loop_ctl *ctl, hipStream_t stream);
+void launch_debug(const scene_dev& s, const frame_params& f, float *sum, uint64_t *rng, int32_t samples,
+                  int32_t *failure, dim3 block, hipStream_t stream);
+void launch_start(const scene_dev& s, const frame_params& f, const paths_dev& p, loop_ctl *ctl, int32_t samples,
+                  hipStream_t stream);
+void launch_trace(const scene_dev& s, const frame_params& f, const paths_dev& p, loop_ctl *ctl, int parity,
+                  int grid, int block, hipStream_t stream);
+void launch_bookkeep(loop_ctl *ctl, int parity, hipStream_t stream);
+void launch_shade(const scene_dev& s, const frame_params& f, const paths_dev& p, loop_ctl *ctl, int parity,
+                  int grid, int block, hipStream_t stream);
+void launch_pack(const float *sum, uint8_t *rgba, int64_t count, int32_t samples, hipStream_t stream);
+void launch_add_rays(loop_ctl *ctl, unsigned long long rays, hipStream_t stream);
+
+// Dynamic LDS bytes the traversal kernels need per block.
+inline size_t stack_bytes(int capacity, int threads) { return size_t(capacity) * size_t(threads) * sizeof(int32_t); }
+
+} // namespace pt

@@ -1,0 +1,800 @@
+// pt_kernels.hip -- CDNA4 kernels of the path-tracing core.
+//
+// Wavefront structure (one live path per pixel, so each pixel's PCG stream
+// is consumed in the same order as the reference's sequential sample loop,
+// src/cuda/cuda_renderer.cu:70-85):
+//   start    : camera rays for every pixel, identity extension queue
+//   trace    : persistent kernel; closest hit for the extension queue and
+//              any-hit for the shadow queue (one launch), LDS traversal stacks
+//   bookkeep : one thread; ray counters and queue resets
+//   shade    : hit reconstruction, MIS/NEE, BSDF sampling, accumulation and
+//              path regeneration; wave-ballot compaction into the next queues
+// Primary-only diagnostic views use one pixel-parallel kernel.
+//
+// Arithmetic follows the reference's host evaluation order exactly and the
+// library is compiled with -ffp-contract=off: see the citation on each helper.
+#include <cfloat>
+#include <cmath>
+
+#include "pt_device.h"
+
+namespace pt {
+namespace {
+
+constexpr float PI_F = float(M_PI);
+
+// ---------------- vec3 arithmetic (src/core/vec3.h:49-162) ----------------
+struct v3 { float x, y, z; };
+__device__ __forceinline__ v3 mk(float x, float y, float z) { return v3{x, y, z}; }
+__device__ __forceinline__ v3 mk(const pt_vec3& a) { return v3{a.x, a.y, a.z}; }
+__device__ __forceinline__ v3 add(v3 a, v3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ v3 sub(v3 a, v3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ v3 mul(v3 a, v3 b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
+// operator*(float,vec3) / operator*(vec3,float): t*e[i] (vec3.h:93-103)
+__device__ __forceinline__ v3 scale(float t, v3 v) { return mk(t * v.x, t * v.y, t * v.z); }
+// operator/(vec3,float): per-component division (vec3.h:97-99)
+__device__ __forceinline__ v3 divs(v3 v, float t) { return mk(v.x / t, v.y / t, v.z / t); }
+__device__ __forceinline__ v3 neg(v3 v) { return mk(-v.x, -v.y, -v.z); }
+__device__ __forceinline__ float dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+// vec3.h:109-115 (negated middle difference kept for signed zeros)
+__device__ __forceinline__ v3 cross(v3 a, v3 b) {
+    return mk(a.y * b.z - a.z * b.y, -(a.x * b.z - a.z * b.x), a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ float length(v3 v) { return sqrtf(v.x * v.x + v.y * v.y + v.z * v.z); }
+__device__ __forceinline__ float squared_length(v3 v) { return v.x * v.x + v.y * v.y + v.z * v.z; }
+// operator/=(float): multiply by float(1.0/t) computed in double (vec3.h:152-158)
+__device__ __forceinline__ v3 div_assign(v3 v, float t) {
+    float k = float(1.0 / double(t));
+    return mk(v.x * k, v.y * k, v.z * k);
+}
+__device__ __forceinline__ v3 unit_vector(v3 v) { return divs(v, length(v)); }
+__device__ __forceinline__ bool finite3(float x) { return isfinite(x); }
+
+// ---------------- PCG-XSH-RR 64/32 (src/renderer/random.h:8-23) ----------------
+struct rng_t {
+    uint64_t state, increment;
+    __device__ __forceinline__ uint32_t next() {
+        uint64_t old = state;
+        state = old * 6364136223846793005ULL + increment;
+        uint32_t shifted = uint32_t(((old >> 18u) ^ old) >> 27u);
+        uint32_t rotation = uint32_t(old >> 59u);
+        return (shifted >> rotation) | (shifted << ((-rotation) & 31u));
+    }
+    __device__ __forceinline__ uint32_t next24() { return next() >> 8; }
+    __device__ __forceinline__ float uniform() { return float(next() >> 8) * 0x1p-24f; }
+};
+__device__ __forceinline__ rng_t pixel_rng(uint64_t state, uint64_t pixel) { return rng_t{state, (pixel << 1u) | 1u}; }
+__device__ __forceinline__ uint64_t seed_state(uint64_t value, uint64_t pixel) {
+    rng_t r{0, (pixel << 1u) | 1u};
+    r.next(); r.state += value; r.next();
+    return r.state;
+}
+// random.h:25-33
+__device__ v3 sample_ball(rng_t& r) {
+    v3 p;
+    do {
+        float x = r.uniform(), y = r.uniform(), z = r.uniform();
+        p = sub(scale(2.0f, mk(x, y, z)), mk(1, 1, 1));
+    } while(squared_length(p) >= 1.0f);
+    return p;
+}
+// random.h:35-42
+__device__ v3 sample_disk(rng_t& r) {
+    v3 p;
+    do {
+        float x = r.uniform(), y = r.uniform();
+        p = sub(scale(2.0f, mk(x, y, 0)), mk(1, 1, 0));
+    } while(squared_length(p) >= 1.0f);
+    return p;
+}
+
+// ---------------- rays ----------------
+struct ray_in {
+    float ox, oy, oz, dx, dy, dz;
+    float ix, iy, iz;   // 1.0f/d per axis (the value trace_box recomputes per node)
+    float dlen;         // r.direction().length() (flat_intersect.h:13)
+};
+__device__ __forceinline__ ray_in setup_ray(float ox, float oy, float oz, float dx, float dy, float dz) {
+    ray_in r;
+    r.ox = ox; r.oy = oy; r.oz = oz; r.dx = dx; r.dy = dy; r.dz = dz;
+    r.ix = 1.0f / dx; r.iy = 1.0f / dy; r.iz = 1.0f / dz;
+    r.dlen = sqrtf(dx * dx + dy * dy + dz * dz);
+    return r;
+}
+
+// nextafterf(x, toward) for finite x != toward.
+__device__ __forceinline__ float next_toward(float x, float toward) {
+    if(!(x < toward) && !(x > toward)) return toward;
+    uint32_t b = __float_as_uint(x);
+    if(x == 0.0f) return x < toward ? __uint_as_float(1u) : __uint_as_float(0x80000001u);
+    bool up = x < toward;
+    bool positive = !(b >> 31);
+    b = (up == positive) ? b + 1 : b - 1;
+    return __uint_as_float(b);
+}
+
+// ---------------- slab test (src/renderer/transport.h:14-31) ----------------
+// All three axes are evaluated with the reference's ternaries; low only grows
+// and high only shrinks, so the reference's early exits cannot change the result.
+__device__ __forceinline__ bool slab(float mn, float mx, float o, float d, float inv, float& low, float& high, bool& ok) {
+    if(d == 0) { ok = ok && !(o < mn || o > mx); return ok; }
+    float a = (mn - o) * inv;
+    float b = (mx - o) * inv;
+    if(inv < 0) { float s = a; a = b; b = s; }
+    low = a > low ? a : low;
+    high = b < high ? b : high;
+    return ok;
+}
+__device__ __forceinline__ bool box_test(const float4& A, const float4& B, const ray_in& r, float low, float high) {
+    bool ok = true;
+    slab(A.x, A.w, r.ox, r.dx, r.ix, low, high, ok);
+    slab(A.y, B.x, r.oy, r.dy, r.iy, low, high, ok);
+    slab(A.z, B.y, r.oz, r.dz, r.iz, low, high, ok);
+    return ok && high > low;
+}
+
+// ---------------- Moller-Trumbore (src/flat/flat_intersect.h:8-27) ----------------
+// A = {v0, 1e-7f*|n|}, B = {e1, |n|}, C = {e2, -}, with e1 = v1-v0, e2 = v2-v0, n = e1 x e2.
+__device__ __forceinline__ bool tri_test(const float4& A, const float4& B, const float4& C, const ray_in& r,
+                                         float t_min, float t_max, float& t, float& u, float& v) {
+    if(!(B.w > 0) || !(r.dlen > 0)) return false;
+    v3 d = mk(r.dx, r.dy, r.dz), e1 = mk(B.x, B.y, B.z), e2 = mk(C.x, C.y, C.z);
+    v3 pvec = cross(d, e2);
+    float det = dot(e1, pvec);
+    if(fabsf(det) <= A.w * r.dlen) return false;
+    float inv_det = 1.0f / det;
+    v3 tvec = sub(mk(r.ox, r.oy, r.oz), mk(A.x, A.y, A.z));
+    u = dot(tvec, pvec) * inv_det;
+    if(!(u >= 0 && u <= 1)) return false;
+    v3 qvec = cross(tvec, e1);
+    v = dot(d, qvec) * inv_det;
+    if(!(v >= 0 && u + v <= 1)) return false;
+    t = dot(e2, qvec) * inv_det;
+    return t > t_min && t < t_max;
+}
+
+// ---------------- sphere (src/flat/flat_intersect.h:49-58) ----------------
+__device__ __forceinline__ bool sphere_test(const float4& S, const ray_in& r, float t_min, float t_max, float& t) {
+    v3 d = mk(r.dx, r.dy, r.dz);
+    v3 oc = sub(mk(r.ox, r.oy, r.oz), mk(S.x, S.y, S.z));
+    float a = dot(d, d);
+    float b = dot(oc, d);
+    float c = dot(oc, oc) - S.w * S.w;
+    float disc = b * b - a * c;
+    if(!(disc > 0)) return false;
+    float root = sqrtf(disc);
+    t = (-b - root) / a;
+    if(!(t < t_max && t > t_min)) t = (-b + root) / a;
+    return t < t_max && t > t_min;
+}
+
+struct best_hit { float t, u, v; int32_t code; };
+
+// ---------------- closest / any hit (src/renderer/transport.h:43-85) ----------------
+// Depth-first, pop then test, right pushed before left, pruning limit
+// nextafterf(closest, high), later leaf wins exact ties.  The stack lives in
+// LDS with a stride of the block size (entry k of a lane at stk[k*stride]).
+template<bool ANY_HIT, bool STATS>
+__device__ bool traverse_bvh(const scene_dev& s, const ray_in& r, float low, float high, int32_t *stk, int stride,
+                             best_hit& best, int& boxes, int& prims, int& error) {
+    if(s.root_ref == empty_root) return false;
+    bool found = false;
+    float closest = high;
+    int count = 0;
+    stk[0] = s.root_ref; count = 1;
+    const int cap = s.stack_capacity;
+    while(count) {
+        --count;
+        int32_t ref = stk[count * stride];
+        float limit = found ? next_toward(closest, high) : high;
+        if(ref >= 0) {
+            if(STATS) boxes++;
+            float4 A = s.bnodes[2 * ref], B = s.bnodes[2 * ref + 1];
+            if(!box_test(A, B, r, low, limit)) continue;
+            int32_t left = __float_as_int(B.z), right = __float_as_int(B.w);
+            int needed = left == right ? 1 : 2;
+            if(count + needed > cap) { error = 1; return false; }
+            if(left != right) { stk[count * stride] = right; count++; }
+            stk[count * stride] = left; count++;
+        } else {
+            if(STATS) prims++;
+            int32_t code = ~ref;
+            int32_t prim = code >> 1;
+            float t, u = 0, v = 0;
+            bool hit;
+            if(code & 1) hit = sphere_test(s.sph_isect[prim], r, low, limit, t);
+            else hit = tri_test(s.tri_isect[3 * prim], s.tri_isect[3 * prim + 1], s.tri_isect[3 * prim + 2], r, low, limit, t, u, v);
+            if(hit && (!found || t <= closest)) {
+                found = true; closest = t;
+                best.t = t; best.u = u; best.v = v; best.code = code;
+                if(ANY_HIT) return true;
+            }
+        }
+    }
+    return found;
+}
+
+// Brute mode (transport.h:48-60): every leaf in node-array order, no pruning.
+template<bool ANY_HIT>
+__device__ bool traverse_brute(const scene_dev& s, const ray_in& r, float low, float high, best_hit& best, int& prims) {
+    if(s.root_ref == empty_root) return false;
+    bool found = false;
+    float closest = high;
+    for(int32_t i = 0; i < s.node_count; i++) {
+        const pt_flat_node& n = s.nodes[i];
+        if(n.type == PT_BRANCH) continue;
+        prims++;
+        float t, u = 0, v = 0;
+        bool hit;
+        int32_t code = n.primitive * 2 + (n.type == PT_SPHERE ? 1 : 0);
+        if(n.type == PT_SPHERE) hit = sphere_test(s.sph_isect[n.primitive], r, low, high, t);
+        else hit = tri_test(s.tri_isect[3 * n.primitive], s.tri_isect[3 * n.primitive + 1], s.tri_isect[3 * n.primitive + 2], r, low, high, t, u, v);
+        if(hit && (!found || t <= closest)) {
+            found = true; closest = t;
+            best.t = t; best.u = u; best.v = v; best.code = code;
+            if(ANY_HIT) return true;
+        }
+    }
+    return found;
+}
+
+template<bool ANY_HIT, bool STATS>
+__device__ __forceinline__ bool trace(const scene_dev& s, bool brute, const ray_in& r, float low, float high,
+                                      int32_t *stk, int stride, best_hit& best, int& boxes, int& prims, int& error) {
+    if(brute) return traverse_brute<ANY_HIT>(s, r, low, high, best, prims);
+    return traverse_bvh<ANY_HIT, STATS>(s, r, low, high, stk, stride, best, boxes, prims, error);
+}
+
+// ---------------- hit record (flat_intersect.h:28-42,59-64; transport.h:39) ----------------
+struct hit_rec { float t, u, v; v3 p, normal; int32_t material_id, primitive_type, primitive_id; };
+
+// src/geometry/sphere_uv.h:7-13.  atan2/asin are evaluated in double and
+// rounded, so u,v can differ from a float libm's result in the last bit.
+__device__ __forceinline__ void sphere_uv(v3 p, float& u, float& v) {
+    float phi = float(atan2(double(p.z), double(p.x)));
+    float theta = float(asin(double(p.y)));
+    u = float(1 - (double(phi) + M_PI) / (2 * M_PI));
+    v = float((double(theta) + M_PI / 2) / M_PI);
+}
+
+__device__ hit_rec make_hit(const scene_dev& s, v3 o, v3 d, const best_hit& b) {
+    hit_rec h;
+    int32_t prim = b.code >> 1;
+    h.t = b.t;
+    h.p = add(o, scale(b.t, d));
+    h.primitive_id = prim;
+    if(b.code & 1) {
+        const pt_flat_sphere& sp = s.spheres[prim];
+        v3 c = mk(sp.center);
+        h.normal = divs(sub(h.p, c), sp.radius);
+        sphere_uv(h.normal, h.u, h.v);
+        h.material_id = sp.material_id;
+        h.primitive_type = PT_SPHERE;
+    } else {
+        const pt_flat_triangle& tr = s.triangles[prim];
+        float4 B = s.tri_isect[3 * prim + 1], C = s.tri_isect[3 * prim + 2];
+        v3 outward = cross(mk(B.x, B.y, B.z), mk(C.x, C.y, C.z));
+        outward = div_assign(outward, B.w);
+        v3 normal = outward;
+        float u = b.u, v = b.v;
+        if(tr.smooth) {
+            float w = 1 - u - v;
+            v3 interp = add(add(scale(w, mk(tr.n0)), scale(u, mk(tr.n1))), scale(v, mk(tr.n2)));
+            float ls = squared_length(interp);
+            if(ls > 0 && isfinite(ls)) {
+                normal = unit_vector(interp);
+                if(dot(normal, outward) < 0) normal = neg(normal);
+                if(dot(normal, outward) == 0) normal = outward;
+            }
+        }
+        h.normal = normal;
+        float w = 1 - u - v;
+        v3 uv = add(add(scale(w, mk(tr.uv0)), scale(u, mk(tr.uv1))), scale(v, mk(tr.uv2)));
+        h.u = uv.x; h.v = uv.y;
+        h.material_id = tr.material_id;
+        h.primitive_type = PT_TRIANGLE;
+    }
+    return h;
+}
+
+// ---------------- textures (src/renderer/transport.h:87-111) ----------------
+__device__ v3 trace_image(const scene_dev& s, const pt_flat_texture& tex, float u, float v) {
+    u = fminf(1.0f, fmaxf(0.0f, u));
+    v = fminf(1.0f, fmaxf(0.0f, v));
+    int i = int(u * float(tex.width));
+    int j = int(double((1 - v) * float(tex.height)) - 0.001);
+    if(i < 0) i = 0;
+    if(j < 0) j = 0;
+    if(i > tex.width - 1) i = tex.width - 1;
+    if(j > tex.height - 1) j = tex.height - 1;
+    size_t off = size_t(tex.byte_offset) + 3 * size_t(i) + 3 * size_t(tex.width) * size_t(j);
+    float red = float(int(s.pixels[off]) / 255.0);
+    float green = float(int(s.pixels[off + 1]) / 255.0);
+    float blue = float(int(s.pixels[off + 2]) / 255.0);
+    return mk(red, green, blue);
+}
+__device__ v3 trace_texture(const scene_dev& s, int32_t id, float u, float v) {
+    const pt_flat_texture& tex = s.textures[id];
+    if(tex.type == PT_CONSTANT) return mk(tex.color);
+    if(tex.type == PT_IMAGE) return trace_image(s, tex, u, v);
+    u = u * tex.scale.x + tex.offset.x;
+    v = v * tex.scale.y + tex.offset.y;
+    if(!tex.clamp) { u -= floorf(u); v -= floorf(v); }
+    return mul(mk(tex.color), trace_image(s, s.textures[tex.image_id], u, v));
+}
+
+// ---------------- light sampling (src/renderer/light_sampling.h) ----------------
+// light_sampling.h:10-16
+__device__ __forceinline__ float mis_weight(float a, float b) {
+    float sc = fmaxf(a, b);
+    if(!(sc > 0)) return 0;
+    a /= sc; b /= sc;
+    return a * a / (a * a + b * b);
+}
+// light_sampling.h:18-25.  cosf/sinf of 2*float(pi)*v come from the host-libm
+// table indexed by the 24-bit draw, so they equal the host build's values.
+__device__ __forceinline__ v3 cosine_direction(const scene_dev& s, v3 normal, rng_t& r) {
+    float u = r.uniform();
+    uint32_t k = r.next24();
+    float radius = sqrtf(u);
+    float2 cs = s.trig[k];
+    v3 helper = fabsf(normal.x) > 0.9f ? mk(0, 1, 0) : mk(1, 0, 0);
+    v3 tangent = unit_vector(cross(helper, normal));
+    v3 bitangent = cross(normal, tangent);
+    return add(add(scale(radius * cs.x, tangent), scale(radius * cs.y, bitangent)), scale(sqrtf(1 - u), normal));
+}
+struct light_sample { v3 point, normal; float u, v; int32_t material; };
+// light_sampling.h:27-54
+__device__ light_sample sample_light(const scene_dev& s, rng_t& r) {
+    float choice = r.uniform() * s.light_area;
+    int low = 0, high = s.light_count - 1;
+    while(low < high) {
+        int middle = low + (high - low) / 2;
+        if(choice < s.lights[middle].cumulative_area) high = middle;
+        else low = middle + 1;
+    }
+    const light_rec light = s.lights[low];
+    float u = r.uniform();
+    uint32_t kv = r.next24();
+    float v = float(kv) * 0x1p-24f;
+    light_sample res;
+    if(light.type == PT_TRIANGLE) {
+        const pt_flat_triangle& t = s.triangles[light.primitive];
+        float root = sqrtf(u), a = 1 - root, b = root * (1 - v), c = root * v;
+        res.point = add(add(scale(a, mk(t.v0)), scale(b, mk(t.v1))), scale(c, mk(t.v2)));
+        res.normal = unit_vector(cross(sub(mk(t.v1), mk(t.v0)), sub(mk(t.v2), mk(t.v0))));
+        v3 uv = add(add(scale(a, mk(t.uv0)), scale(b, mk(t.uv1))), scale(c, mk(t.uv2)));
+        res.u = uv.x; res.v = uv.y; res.material = t.material_id;
+    } else {
+        const pt_flat_sphere& sp = s.spheres[light.primitive];
+        float z = 1 - 2 * u, radius = sqrtf(fmaxf(0, 1 - z * z));
+        float2 cs = s.trig[kv];
+        res.normal = mk(radius * cs.x, z, radius * cs.y);
+        res.point = add(mk(sp.center), scale(fabsf(sp.radius), res.normal));
+        if(sp.radius < 0) res.normal = neg(res.normal);
+        sphere_uv(res.normal, res.u, res.v);
+        res.material = sp.material_id;
+    }
+    return res;
+}
+// light_sampling.h:56-64
+__device__ __forceinline__ float light_direction_pdf(float area, v3 from, v3 point, v3 normal) {
+    v3 delta = sub(point, from);
+    float d2 = squared_length(delta);
+    if(!(area > 0) || !(d2 > 0)) return 0;
+    float cosine = fabsf(dot(normal, delta)) / sqrtf(d2);
+    if(!(cosine > 0)) return 0;
+    float pdf = d2 / (cosine * area);
+    return isfinite(pdf) ? pdf : 0;
+}
+// light_sampling.h:66-73
+__device__ __forceinline__ float hit_light_pdf(const scene_dev& s, v3 from, const hit_rec& h) {
+    v3 normal = h.normal;
+    if(h.primitive_type == PT_TRIANGLE) {
+        const pt_flat_triangle& t = s.triangles[h.primitive_id];
+        normal = unit_vector(cross(sub(mk(t.v1), mk(t.v0)), sub(mk(t.v2), mk(t.v0))));
+    }
+    return light_direction_pdf(s.light_area, from, h.p, normal);
+}
+
+// ---------------- BSDF sampling (src/renderer/transport.h:113-155) ----------------
+__device__ __forceinline__ v3 reflect(v3 v, v3 n) { return sub(v, scale(2 * dot(v, n), n)); }
+
+__device__ bool scatter(const scene_dev& s, const pt_flat_material& mat, v3 in_d, const hit_rec& rec, rng_t& r,
+                        v3& attenuation, v3& out_d) {
+    if(mat.type == PT_DIFFUSE) {
+        v3 target = add(add(rec.p, rec.normal), sample_ball(r));
+        out_d = sub(target, rec.p);
+        attenuation = trace_texture(s, mat.texture_id, rec.u, rec.v);
+        return true;
+    }
+    if(mat.type == PT_METAL) {
+        v3 reflected = reflect(unit_vector(in_d), rec.normal);
+        out_d = add(reflected, scale(mat.fuzz, sample_ball(r)));
+        attenuation = mk(mat.albedo);
+        return dot(out_d, rec.normal) > 0;
+    }
+    if(mat.type == PT_GLASS) {
+        v3 outward, refracted = mk(0, 0, 0);
+        v3 reflected = reflect(in_d, rec.normal);
+        float ratio, cosine;
+        if(dot(in_d, rec.normal) > 0) {
+            outward = neg(rec.normal); ratio = mat.ref_idx;
+            cosine = mat.ref_idx * dot(in_d, rec.normal) / length(in_d);
+        } else {
+            outward = rec.normal; ratio = 1.0f / mat.ref_idx;
+            cosine = -dot(in_d, rec.normal) / length(in_d);
+        }
+        v3 uv = unit_vector(in_d);
+        float dt = dot(uv, outward);
+        float discriminant = 1.0f - ratio * ratio * (1 - dt * dt);
+        float probability = 1;
+        if(discriminant > 0) {
+            refracted = sub(scale(ratio, sub(uv, scale(dt, outward))), scale(sqrtf(discriminant), outward));
+            float r0 = (1 - mat.ref_idx) / (1 + mat.ref_idx);
+            r0 *= r0;
+            // The host build promotes pow(float,int) to double (transport.h:147).
+            probability = float(double(r0) + double(1 - r0) * pow(double(1 - cosine), 5.0));
+        }
+        out_d = r.uniform() < probability ? reflected : refracted;
+        attenuation = mk(1, 1, 1);
+        return true;
+    }
+    return false;
+}
+
+// ---------------- camera (transport.h:157-162, 232-237) ----------------
+__device__ __forceinline__ void camera_ray(const pt_flat_camera& c, int x, int y, int width, int height, rng_t& r,
+                                           v3& o, v3& d, float& time) {
+    float u = (float(x) + r.uniform()) / float(width);
+    float v = (float(y) + r.uniform()) / float(height);
+    v3 disk = scale(c.lens_radius, sample_disk(r));
+    v3 offset = add(scale(disk.x, mk(c.u)), scale(disk.y, mk(c.v)));
+    time = c.time0 + r.uniform() * (c.time1 - c.time0);
+    o = add(mk(c.origin), offset);
+    d = sub(sub(add(add(mk(c.lower_left_corner), scale(u, mk(c.horizontal))), scale(v, mk(c.vertical))), mk(c.origin)), offset);
+}
+
+// transport.h:164-167
+__device__ __forceinline__ v3 id_color(int32_t id) {
+    uint32_t bits = (uint32_t(id) + 1u) * 2654435761u;
+    return mk(float(bits & 255) / 255.0f, float((bits >> 8) & 255) / 255.0f, float((bits >> 16) & 255) / 255.0f);
+}
+
+__device__ __forceinline__ float cost_log2(const scene_dev& s, int count) {
+    // log2f(1.0f + float(count)) of the host libm, from the per-scene table.
+    if(count >= 0 && count < s.log2_count) return s.log2_table[count];
+    return float(log2(double(1.0f + float(count))));
+}
+
+// Wave-aggregated append: one atomic per wave, order kept within the wave.
+__device__ __forceinline__ void append(bool pred, int32_t value, int32_t *queue, uint32_t *counter) {
+    uint64_t mask = __ballot(pred);
+    if(mask == 0) return;
+    int lane = __lane_id();
+    int leader = __ffsll((unsigned long long)mask) - 1;
+    uint32_t base = 0;
+    if(lane == leader) base = atomicAdd(counter, uint32_t(__popcll(mask)));
+    base = __shfl(base, leader);
+    if(pred) queue[base + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)))] = value;
+}
+
+// ======================================================================
+// Kernels
+// ======================================================================
+
+// cuda_renderer.cu:64-68 (initialize_pixels) plus the failure flag reset.
+__global__ void init_pixels_kernel(float *sum, uint64_t *rng, int64_t count, uint64_t seed, loop_ctl *ctl) {
+    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if(i == 0) { ctl->failure = 0; ctl->rays = 0; }
+    if(i >= count) return;
+    sum[3 * i] = 0; sum[3 * i + 1] = 0; sum[3 * i + 2] = 0;
+    rng[i] = seed_state(seed, uint64_t(i));
+}
+
+// cuda_renderer.cu:111-120 (query_rays), with optional trace_stats output.
+__global__ __launch_bounds__(256) void query_kernel(scene_dev s, const pt_ray *rays, pt_trace_result *out,
+                                                    int32_t *boxes_out, int32_t *prims_out, int32_t count, int brute) {
+    extern __shared__ int32_t lds[];
+    int32_t i = int32_t(blockIdx.x) * int32_t(blockDim.x) + int32_t(threadIdx.x);
+    if(i >= count) return;
+    pt_ray ray = rays[i];
+    ray_in r = setup_ray(ray.origin.x, ray.origin.y, ray.origin.z, ray.direction.x, ray.direction.y, ray.direction.z);
+    best_hit b; int boxes = 0, prims = 0, error = 0;
+    bool hit = trace<false, true>(s, brute != 0, r, 0.001f, FLT_MAX, lds + threadIdx.x, blockDim.x, b, boxes, prims, error);
+    pt_trace_result res;
+    res.rec.t = 0; res.rec.u = 0; res.rec.v = 0;
+    res.rec.p = pt_vec3{0, 0, 0}; res.rec.normal = pt_vec3{0, 0, 0};
+    res.rec.material_id = -1; res.rec.primitive_type = -1; res.rec.primitive_id = -1;
+    if(hit && !error) {
+        hit_rec h = make_hit(s, mk(r.ox, r.oy, r.oz), mk(r.dx, r.dy, r.dz), b);
+        res.rec.t = h.t; res.rec.u = h.u; res.rec.v = h.v;
+        res.rec.p = pt_vec3{h.p.x, h.p.y, h.p.z}; res.rec.normal = pt_vec3{h.normal.x, h.normal.y, h.normal.z};
+        res.rec.material_id = h.material_id; res.rec.primitive_type = h.primitive_type; res.rec.primitive_id = h.primitive_id;
+    }
+    res.hit = (hit && !error) ? 1 : 0;
+    res.error = error;
+    out[i] = res;
+    if(boxes_out) boxes_out[i] = boxes;
+    if(prims_out) prims_out[i] = prims;
+}
+
+// Primary-only diagnostic views (transport.h:179-187): pixel-parallel, each
+// thread runs its pixel's samples in order.
+__global__ void debug_kernel(scene_dev s, frame_params f, float *sum, uint64_t *rngs, int32_t samples, int32_t *failure) {
+    extern __shared__ int32_t lds[];
+    int tid = int(threadIdx.y * blockDim.x + threadIdx.x);
+    int stride = int(blockDim.x * blockDim.y);
+    int x = int(blockIdx.x * blockDim.x + threadIdx.x);
+    int y = int(blockIdx.y * blockDim.y + threadIdx.y);
+    if(x >= f.width || y >= f.height) return;
+    size_t i = size_t(y) * size_t(f.width) + size_t(x);
+    rng_t r = pixel_rng(rngs[i], i);
+    v3 acc = mk(sum[3 * i], sum[3 * i + 1], sum[3 * i + 2]);
+    int error = 0;
+    for(int k = 0; k < samples; k++) {
+        v3 o, d; float time;
+        camera_ray(f.camera, x, y, f.width, f.height, r, o, d, time);
+        ray_in ray = setup_ray(o.x, o.y, o.z, d.x, d.y, d.z);
+        best_hit b; int boxes = 0, prims = 0;
+        bool hit = trace<false, true>(s, f.brute != 0, ray, 0.001f, FLT_MAX, lds + tid, stride, b, boxes, prims, error);
+        v3 c;
+        if(f.debug == PT_DEBUG_BVH || f.debug == PT_DEBUG_INTERSECTIONS) {
+            float cost = cost_log2(s, f.debug == PT_DEBUG_BVH ? boxes : prims) / 10.0f;
+            cost = fminf(1, cost);
+            c = mk(cost, cost * cost, 1 - cost);
+        } else if(!hit || error) {
+            c = mk(0, 0, 0);
+        } else {
+            hit_rec h = make_hit(s, o, d, b);
+            if(f.debug == PT_DEBUG_NORMALS) c = scale(0.5f, add(h.normal, mk(1, 1, 1)));
+            else if(f.debug == PT_DEBUG_TRIANGLE) c = h.primitive_type == PT_TRIANGLE ? id_color(h.primitive_id) : mk(float(0.3), float(0.3), float(0.3));
+            else c = id_color(h.material_id);
+        }
+        acc = add(acc, c);
+    }
+    if(error) atomicExch(failure, error);
+    sum[3 * i] = acc.x; sum[3 * i + 1] = acc.y; sum[3 * i + 2] = acc.z;
+    rngs[i] = r.state;
+}
+
+// Start of an add_samples call: the first camera ray of every pixel.
+__global__ void start_kernel(scene_dev s, frame_params f, paths_dev p, loop_ctl *ctl, int32_t samples) {
+    int64_t n = int64_t(f.width) * f.height;
+    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if(i == 0) {
+        ctl->n_ext[0] = uint32_t(n); ctl->n_ext[1] = 0; ctl->n_shadow = 0; ctl->work = 0;
+        ctl->iterations = 0; ctl->ext_rays = 0; ctl->shadow_rays = 0;
+    }
+    if(i >= n) return;
+    rng_t r = pixel_rng(p.rng[i], uint64_t(i));
+    int x = int(i % f.width), y = int(i / f.width);
+    v3 o, d; float time;
+    camera_ray(f.camera, x, y, f.width, f.height, r, o, d, time);
+    p.ray_o[i] = make_float4(o.x, o.y, o.z, time);
+    p.ray_d[i] = make_float4(d.x, d.y, d.z, 0);
+    p.thr[i] = make_float4(1, 1, 1, 0);
+    p.rad[i] = make_float4(0, 0, 0, 0);
+    p.ctr[i] = make_int2(0, samples);
+    p.rng[i] = r.state;
+    p.queue[0][i] = int32_t(i);
+}
+
+// Closest hit for the extension queue and any-hit for the shadow queue.
+// Persistent: each wave claims 64 consecutive items at a time.
+__global__ __launch_bounds__(256) void trace_kernel(scene_dev s, frame_params f, paths_dev p, loop_ctl *ctl, int parity) {
+    extern __shared__ int32_t lds[];
+    const uint32_t n_ext = ctl->n_ext[parity];
+    const uint32_t total = n_ext + ctl->n_shadow;
+    const int32_t *q = p.queue[parity];
+    const int lane = __lane_id();
+    int32_t *stk = lds + threadIdx.x;
+    const int stride = int(blockDim.x);
+    const bool brute = f.brute != 0;
+    int error = 0;
+    for(;;) {
+        uint32_t base = 0;
+        if(lane == 0) base = atomicAdd(&ctl->work, 64u);
+        base = __shfl(base, 0);
+        if(base >= total) break;
+        uint32_t item = base + uint32_t(lane);
+        if(item < n_ext) {
+            int32_t pix = q[item];
+            float4 o = p.ray_o[pix], d = p.ray_d[pix];
+            ray_in r = setup_ray(o.x, o.y, o.z, d.x, d.y, d.z);
+            best_hit b; int boxes = 0, prims = 0;
+            bool hit = trace<false, false>(s, brute, r, 0.001f, FLT_MAX, stk, stride, b, boxes, prims, error);
+            p.hit[pix] = hit ? make_int4(__float_as_int(b.t), __float_as_int(b.u), __float_as_int(b.v), b.code)
+                             : make_int4(0, 0, 0, -1);
+        } else if(item < total) {
+            int32_t pix = p.shadow_queue[item - n_ext];
+            float4 o = p.sh_o[pix], d = p.sh_d[pix];
+            ray_in r = setup_ray(o.x, o.y, o.z, d.x, d.y, d.z);
+            best_hit b; int boxes = 0, prims = 0;
+            bool blocked = trace<true, false>(s, brute, r, 0.001f, o.w, stk, stride, b, boxes, prims, error);
+            if(!blocked) {
+                float4 c = p.sh_c[pix], ra = p.rad[pix];
+                ra.x = ra.x + c.x; ra.y = ra.y + c.y; ra.z = ra.z + c.z;
+                p.rad[pix] = ra;
+            }
+        }
+    }
+    if(error) atomicExch(&ctl->failure, 1);
+}
+
+__global__ void bookkeep_kernel(loop_ctl *ctl, int parity) {
+    uint32_t e = ctl->n_ext[parity], sh = ctl->n_shadow;
+    ctl->rays += (unsigned long long)e + sh;
+    ctl->ext_rays += e; ctl->shadow_rays += sh;
+    ctl->n_shadow = 0;
+    ctl->n_ext[parity ^ 1] = 0;
+    ctl->work = 0;
+    if(e) ctl->iterations++;
+}
+
+__global__ void add_rays_kernel(loop_ctl *ctl, unsigned long long rays) { ctl->rays += rays; }
+
+// One bounce of trace_color (transport.h:169-230) for every path in the queue.
+__global__ __launch_bounds__(256) void shade_kernel(scene_dev s, frame_params f, paths_dev p, loop_ctl *ctl, int parity) {
+    const uint32_t n = ctl->n_ext[parity];
+    const int32_t *q = p.queue[parity];
+    int32_t *qn = p.queue[parity ^ 1];
+    const uint32_t step = gridDim.x * blockDim.x;
+    for(uint32_t base = blockIdx.x * blockDim.x; base < n; base += step) {
+        uint32_t item = base + threadIdx.x;
+        bool cont = false, shadow = false;
+        int32_t pix = 0;
+        if(item < n) {
+            pix = q[item];
+            int4 hb = p.hit[pix];
+            float4 ro = p.ray_o[pix], rd = p.ray_d[pix], th = p.thr[pix], ra = p.rad[pix];
+            int2 ct = p.ctr[pix];
+            rng_t r = pixel_rng(p.rng[pix], uint64_t(pix));
+            v3 o = mk(ro.x, ro.y, ro.z), d = mk(rd.x, rd.y, rd.z);
+            float time = ro.w;
+            v3 throughput = mk(th.x, th.y, th.z), radiance = mk(ra.x, ra.y, ra.z);
+            float previous_pdf = th.w;
+            int depth = ct.x, left = ct.y;
+            bool finished = false;
+            v3 next_o = o, next_d = d;
+            if(hb.w < 0) {
+                finished = true;
+            } else {
+                best_hit b{__int_as_float(hb.x), __int_as_float(hb.y), __int_as_float(hb.z), hb.w};
+                hit_rec rec = make_hit(s, o, d, b);
+                const pt_flat_material mat = s.materials[rec.material_id];
+                if(mat.type == PT_LIGHT) {
+                    float weight = previous_pdf > 0 ? mis_weight(previous_pdf, hit_light_pdf(s, o, rec)) : 1;
+                    radiance = add(radiance, mul(scale(weight, throughput), trace_texture(s, mat.texture_id, rec.u, rec.v)));
+                    finished = true;
+                } else if(depth == f.max_depth) {
+                    finished = true;
+                } else if(f.direct_light && mat.type == PT_DIFFUSE) {
+                    v3 attenuation = trace_texture(s, mat.texture_id, rec.u, rec.v);
+                    if(s.light_count) {
+                        light_sample light = sample_light(s, r);
+                        v3 delta = sub(light.point, rec.p);
+                        float distance = length(delta);
+                        if(distance > 0.002f && isfinite(distance)) {
+                            v3 direction = divs(delta, distance);
+                            float cosine = fmaxf(0, dot(rec.normal, direction));
+                            float pdf = light_direction_pdf(s.light_area, rec.p, light.point, light.normal);
+                            if(cosine > 0 && pdf > 0) {
+                                const pt_flat_material emitter = s.materials[light.material];
+                                v3 emission = trace_texture(s, emitter.texture_id, light.u, light.v);
+                                float bsdf_pdf = cosine / PI_F;
+                                float w = bsdf_pdf * mis_weight(pdf, bsdf_pdf) / pdf;
+                                v3 c = scale(w, mul(mul(throughput, attenuation), emission));
+                                p.sh_o[pix] = make_float4(rec.p.x, rec.p.y, rec.p.z, distance - 0.001f);
+                                p.sh_d[pix] = make_float4(direction.x, direction.y, direction.z, 0);
+                                p.sh_c[pix] = make_float4(c.x, c.y, c.z, 0);
+                                shadow = true;
+                            }
+                        }
+                    }
+                    v3 direction = cosine_direction(s, rec.normal, r);
+                    previous_pdf = fmaxf(0, dot(rec.normal, direction)) / PI_F;
+                    throughput = mul(throughput, attenuation);
+                    next_o = rec.p; next_d = direction;
+                    depth++;
+                    cont = true;
+                } else {
+                    previous_pdf = 0;
+                    v3 attenuation, out_d;
+                    if(!scatter(s, mat, d, rec, r, attenuation, out_d)) {
+                        finished = true;
+                    } else {
+                        throughput = mul(throughput, attenuation);
+                        next_o = rec.p; next_d = out_d;
+                        depth++;
+                        cont = true;
+                    }
+                }
+            }
+            if(finished) {
+                // accumulated += trace_pixel_sample(...) (cuda_renderer.cu:81)
+                float *sp = p.sum + 3 * size_t(pix);
+                sp[0] = sp[0] + radiance.x; sp[1] = sp[1] + radiance.y; sp[2] = sp[2] + radiance.z;
+                left--;
+                if(left > 0) {
+                    int x = int(pix % f.width), y = int(pix / f.width);
+                    camera_ray(f.camera, x, y, f.width, f.height, r, next_o, next_d, time);
+                    throughput = mk(1, 1, 1); radiance = mk(0, 0, 0); previous_pdf = 0; depth = 0;
+                    cont = true;
+                }
+            }
+            p.rng[pix] = r.state;
+            if(cont) {
+                p.ray_o[pix] = make_float4(next_o.x, next_o.y, next_o.z, time);
+                p.ray_d[pix] = make_float4(next_d.x, next_d.y, next_d.z, 0);
+                p.thr[pix] = make_float4(throughput.x, throughput.y, throughput.z, previous_pdf);
+                p.rad[pix] = make_float4(radiance.x, radiance.y, radiance.z, 0);
+                p.ctr[pix] = make_int2(depth, left);
+            }
+        }
+        append(cont, pix, qn, &ctl->n_ext[parity ^ 1]);
+        append(shadow, pix, p.shadow_queue, &ctl->n_shadow);
+    }
+}
+
+// cuda_renderer.cu:99-109 (pack_pixels)
+__global__ void pack_kernel(const float *sum, uint8_t *rgba, int64_t count, int32_t samples) {
+    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if(i >= count) return;
+    float c[3];
+    for(int k = 0; k < 3; k++) c[k] = samples ? sum[3 * i + k] / float(samples) : 0.0f;
+    for(int k = 0; k < 3; k++) {
+        float value = c[k];
+        float gamma = value > 0 && value < 1 ? sqrtf(value) : 0;
+        rgba[4 * i + k] = uint8_t(!(value > 0) ? 0 : value >= 1 ? 255 : int(255.99 * double(gamma)));
+    }
+    rgba[4 * i + 3] = 255;
+}
+
+} // namespace
+
+// ---------------- launchers ----------------
+void launch_query(const scene_dev& s, const pt_ray *rays, pt_trace_result *out, int32_t *boxes, int32_t *prims,
+                  int32_t count, bool brute, hipStream_t stream) {
+    const int block = 256;
+    unsigned grid = unsigned((count + block - 1) / block);
+    hipLaunchKernelGGL(query_kernel, dim3(grid), dim3(block), stack_bytes(s.stack_capacity, block), stream,
+                       s, rays, out, boxes, prims, count, brute ? 1 : 0);
+}
+void launch_init_pixels(float *sum, uint64_t *rng, int64_t count, uint64_t seed, loop_ctl *ctl, hipStream_t stream) {
+    unsigned grid = unsigned((count + 255) / 256);
+    hipLaunchKernelGGL(init_pixels_kernel, dim3(grid ? grid : 1), dim3(256), 0, stream, sum, rng, count, seed, ctl);
+}
+void launch_debug(const scene_dev& s, const frame_params& f, float *sum, uint64_t *rng, int32_t samples,
+                  int32_t *failure, dim3 block, hipStream_t stream) {
+    dim3 grid((unsigned(f.width) + block.x - 1) / block.x, (unsigned(f.height) + block.y - 1) / block.y);
+    hipLaunchKernelGGL(debug_kernel, grid, block, stack_bytes(s.stack_capacity, int(block.x * block.y)), stream,
+                       s, f, sum, rng, samples, failure);
+}
+void launch_start(const scene_dev& s, const frame_params& f, const paths_dev& p, loop_ctl *ctl, int32_t samples,
+                  hipStream_t stream) {
+    int64_t n = int64_t(f.width) * f.height;
+    unsigned grid = unsigned((n + 255) / 256);
+    hipLaunchKernelGGL(start_kernel, dim3(grid), dim3(256), 0, stream, s, f, p, ctl, samples);
+}
+void launch_trace(const scene_dev& s, const frame_params& f, const paths_dev& p, loop_ctl *ctl, int parity,
+                  int grid, int block, hipStream_t stream) {
+    hipLaunchKernelGGL(trace_kernel, dim3(unsigned(grid)), dim3(unsigned(block)), stack_bytes(s.stack_capacity, block),
+                       stream, s, f, p, ctl, parity);
+}
+void launch_bookkeep(loop_ctl *ctl, int parity, hipStream_t stream) {
+    hipLaunchKernelGGL(bookkeep_kernel, dim3(1), dim3(1), 0, stream, ctl, parity);
+}
+void launch_shade(const scene_dev& s, const frame_params& f, const paths_dev& p, loop_ctl *ctl, int parity,
+                  int grid, int block, hipStream_t stream) {
+    hipLaunchKernelGGL(shade_kernel, dim3(unsigned(grid)), dim3(unsigned(block)), 0, stream, s, f, p, ctl, parity);
+}
+void launch_pack(const float *sum, uint8_t *rgba, int64_t count, int32_t samples, hipStream_t stream) {
+    unsigned grid = unsigned((count + 255) / 256);
+    hipLaunchKernelGGL(pack_kernel, dim3(grid ? grid : 1), dim3(256), 0, stream, sum, rgba, count, samples);
+}
+void launch_add_rays(loop_ctl *ctl, unsigned long long rays, hipStream_t stream) {
+    hipLaunchKernelGGL(add_rays_kernel, dim3(1), dim3(1), 0, stream, ctl, rays);
+}
+
+} // namespace pt

@@ -1,0 +1,179 @@
+/*
+ * pt_render.h -- C ABI of the MI355X path-tracing core.
+ *
+ * This is the drop-in boundary for the reference renderer's GPU render path:
+ * every entry point replaces one member of class cuda_renderer or one free
+ * function declared in src/cuda/cuda_renderer.h of the reference
+ * (AdmiralHeathCL/cuda-path-tracer).  Arguments are plain pointers, sizes and
+ * POD records; nothing here names a C++ or torch type.
+ *
+ * Record layouts are byte-identical to the reference's transfer records
+ * (src/flat/flat_data.h:14-40, src/renderer/transport.h:12, src/core/ray.h),
+ * so a caller can hand its flat_scene vectors over without conversion.
+ *
+ * Errors: every function returns a pt_status.  PT_EINVAL corresponds to the
+ * reference throwing std::invalid_argument; every other non-zero status
+ * corresponds to std::runtime_error.  pt_last_error() returns the message of
+ * the last failure on the calling thread.
+ *
+ * Threading: a renderer is used from one host thread at a time; every call
+ * returns after its GPU work has finished (the reference synchronises in
+ * reset and add_samples).
+ */
+#ifndef PT_RENDER_H
+#define PT_RENDER_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- transfer records (src/flat/flat_data.h:14-40) ---- */
+typedef struct pt_vec3 { float x, y, z; } pt_vec3;                                       /* 12 B */
+typedef struct pt_ray { pt_vec3 origin, direction; float time; } pt_ray;                  /* 28 B, src/core/ray.h:30-32 */
+typedef struct pt_flat_triangle {                                                         /* 116 B */
+    pt_vec3 v0, v1, v2, n0, n1, n2, uv0, uv1, uv2;
+    int32_t material_id, smooth;
+} pt_flat_triangle;
+typedef struct pt_flat_sphere { pt_vec3 center; float radius; int32_t material_id; } pt_flat_sphere; /* 20 B */
+typedef struct pt_flat_node {                                                             /* 40 B */
+    pt_vec3 minimum, maximum;
+    int32_t type, left, right, primitive;
+} pt_flat_node;
+typedef struct pt_flat_material { pt_vec3 albedo; float fuzz, ref_idx; int32_t type, texture_id; } pt_flat_material; /* 28 B */
+typedef struct pt_flat_texture {                                                          /* 60 B */
+    pt_vec3 color, scale, offset;
+    int32_t type, image_id, width, height, byte_offset, clamp;
+} pt_flat_texture;
+typedef struct pt_flat_camera {                                                           /* 84 B */
+    pt_vec3 origin, lower_left_corner, horizontal, vertical, u, v;
+    float lens_radius, time0, time1;
+} pt_flat_camera;
+typedef struct pt_flat_hit {                                                              /* 48 B */
+    float t, u, v;
+    pt_vec3 p, normal;
+    int32_t material_id, primitive_type, primitive_id;
+} pt_flat_hit;
+typedef struct pt_trace_result { pt_flat_hit rec; int32_t hit, error; } pt_trace_result; /* 56 B, transport.h:12 */
+
+enum pt_primitive_type { PT_BRANCH = 0, PT_TRIANGLE = 1, PT_SPHERE = 2 };               /* flat_data.h:10 */
+enum pt_material_type { PT_DIFFUSE = 0, PT_METAL = 1, PT_GLASS = 2, PT_LIGHT = 3 };      /* flat_data.h:11 */
+enum pt_texture_type { PT_CONSTANT = 0, PT_IMAGE = 1, PT_MAP = 2 };                      /* flat_data.h:12 */
+enum pt_render_debug {                                                                   /* transport.h:10 */
+    PT_DEBUG_SHADED = 0, PT_DEBUG_NORMALS = 1, PT_DEBUG_TRIANGLE = 2,
+    PT_DEBUG_MATERIAL = 3, PT_DEBUG_BVH = 4, PT_DEBUG_INTERSECTIONS = 5
+};
+
+/* Host view of one flat_scene (flat_data.h:47-56).  Arrays are read during
+ * the call only; the library deep-copies them to the device. */
+typedef struct pt_scene_desc {
+    const pt_flat_triangle *triangles; int64_t triangle_count;
+    const pt_flat_sphere *spheres; int64_t sphere_count;
+    const pt_flat_node *nodes; int64_t node_count;
+    const pt_flat_material *materials; int64_t material_count;
+    const pt_flat_texture *textures; int64_t texture_count;
+    const uint8_t *pixels; int64_t pixel_bytes;
+    int32_t root, max_depth;
+} pt_scene_desc;
+
+/* Constructor arguments of cuda_renderer (cuda_renderer.h:12-13) plus the
+ * path-length limit, which the reference fixes at 50 (transport.h:174,193). */
+typedef struct pt_settings {
+    int32_t width, height;
+    uint64_t seed;
+    int32_t brute;          /* bool brute = false */
+    int32_t debug;          /* render_debug debug = DEBUG_SHADED */
+    int32_t direct_light;   /* bool direct_light = false */
+    int32_t max_path_depth; /* 0 selects the reference's 50 */
+} pt_settings;
+
+typedef enum pt_status {
+    PT_OK = 0,
+    PT_EINVAL = 1,    /* std::invalid_argument in the reference */
+    PT_ESCENE = 2,    /* invalid flat scene data (flat_validate.h:11, runtime_error) */
+    PT_EDEVICE = 3,   /* no usable device, or a HIP runtime error */
+    PT_EOVERFLOW = 4, /* traversal stack overflow (cuda_renderer.cu:204) */
+    PT_ENOMEM = 5
+} pt_status;
+
+typedef struct pt_renderer pt_renderer;
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *pt_last_error(void);
+
+/* cuda_device_available (cuda_renderer.h:8, .cu:128-140): PT_OK and the
+ * device name, or PT_EDEVICE and the reason, written NUL-terminated into
+ * description[0..capacity). */
+pt_status pt_device_available(char *description, size_t capacity);
+
+/* cuda_renderer::cuda_renderer (cuda_renderer.h:12-13, .cu:177-183):
+ * validates scene, camera, mode and size, uploads, then reset(camera). */
+pt_status pt_renderer_create(const pt_scene_desc *scene, const pt_flat_camera *camera,
+                             const pt_settings *settings, pt_renderer **out);
+/* ~cuda_renderer */
+pt_status pt_renderer_destroy(pt_renderer *renderer);
+/* cuda_renderer::reset (cuda_renderer.h:17, .cu:186-194) */
+pt_status pt_renderer_reset(pt_renderer *renderer, const pt_flat_camera *camera);
+/* cuda_renderer::add_samples (cuda_renderer.h:18, .cu:196-213) */
+pt_status pt_renderer_add_samples(pt_renderer *renderer, int32_t count);
+/* cuda_renderer::set_block_shape (cuda_renderer.h:19, .cu:215-228): one of
+ * 8x8, 16x8, 16x16, 32x8; otherwise PT_EINVAL. */
+pt_status pt_renderer_set_block_shape(pt_renderer *renderer, int32_t width, int32_t height);
+/* cuda_renderer::image (cuda_renderer.h:20, .cu:230-234): width*height*3
+ * floats of sum /= float(samples), row 0 at the bottom. */
+pt_status pt_renderer_read_image(const pt_renderer *renderer, float *rgb, size_t float_count);
+/* cuda_renderer::rgba_image (cuda_renderer.h:21, .cu:241-244) */
+pt_status pt_renderer_read_rgba(const pt_renderer *renderer, uint8_t *rgba, size_t bytes);
+/* cuda_renderer::write_rgba (cuda_renderer.h:22, .cu:235-240): packs into a
+ * caller-owned device buffer (e.g. a mapped GL buffer); asynchronous on the
+ * renderer's stream exactly as the reference's kernel launch is. */
+pt_status pt_renderer_write_rgba_device(const pt_renderer *renderer, void *device_rgba, size_t bytes);
+/* sample_count / kernel_milliseconds / rays_traced (cuda_renderer.h:23-25) */
+pt_status pt_renderer_stats(const pt_renderer *renderer, int32_t *samples, float *kernel_ms, uint64_t *rays);
+
+/* cuda_query_rays (cuda_renderer.h:32, .cu:248-260): closest hit per ray on
+ * [0.001, FLT_MAX).  count may be 0. */
+pt_status pt_query_rays(const pt_scene_desc *scene, const pt_ray *rays, size_t count, int32_t brute,
+                        pt_trace_result *results);
+
+/* ---- extensions without a counterpart in cuda_renderer.h ---- */
+
+/* Traversal counters of pt_query_rays (trace_stats, transport.h:11): branch
+ * nodes and leaves popped per ray, in the reference traversal order. */
+pt_status pt_query_rays_stats(const pt_scene_desc *scene, const pt_ray *rays, size_t count, int32_t brute,
+                              pt_trace_result *results, int32_t *boxes, int32_t *primitives);
+
+/* Raw accumulation state, the complete resumable state of a renderer
+ * (cuda_renderer.cu:145-155): sum (3 floats per pixel) and PCG state. */
+pt_status pt_renderer_read_state(const pt_renderer *renderer, float *sum, uint64_t *rng_state, size_t pixels);
+pt_status pt_renderer_write_state(pt_renderer *renderer, const float *sum, const uint64_t *rng_state,
+                                  size_t pixels, int32_t samples, uint64_t rays);
+
+/* Benchmark/diagnostic counters of the last add_samples: bounce iterations
+ * run and HIP-event milliseconds of the closest-hit/any-hit trace kernel. */
+typedef struct pt_launch_stats {
+    int32_t iterations;
+    int32_t trace_launches;
+    float trace_ms;
+    float shade_ms;
+    uint64_t extend_rays;
+    uint64_t shadow_rays;
+} pt_launch_stats;
+pt_status pt_renderer_last_launch_stats(const pt_renderer *renderer, pt_launch_stats *stats);
+
+/* Per-renderer tuning of the wavefront loop (0 keeps the current value). */
+typedef struct pt_tuning {
+    int32_t trace_block;     /* threads per block of the trace kernel, multiple of 64 */
+    int32_t shade_block;     /* threads per block of the shade kernel, multiple of 64 */
+    int32_t blocks_per_cu;   /* persistent trace blocks per compute unit */
+    int32_t check_interval;  /* bounce iterations between host checks of the live count */
+    int32_t time_kernels;    /* 1: record per-kernel HIP events (trace_ms, shade_ms) */
+} pt_tuning;
+pt_status pt_renderer_set_tuning(pt_renderer *renderer, const pt_tuning *tuning);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
