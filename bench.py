@@ -1,0 +1,158 @@
+"""Benchmark of the plain wavefront render path on one MI355X.
+
+Workload: 1920x1080, 64 samples per pixel, path depth 8, MIS direct lighting,
+over a geodesic icosphere of 20*224^2 = 1,003,520 triangles displaced by
+3-octave value noise, with a 2-triangle floor and four rectangular lights
+(1,003,530 triangles in all), binned-SAH BVH.  The scene is synthetic and
+seeded (seed 7).
+
+One step = one full frame: reset(camera) + add_samples(64) through the C ABI,
+inputs resident in HBM.  ``value`` is rays per second over the timed steps,
+counting every closest-hit and shadow query (the reference's rays_traced).
+
+Also reported:
+  roofline     -- trace kernel (closest-hit + any-hit traversal), HBM-bound:
+                  algorithmic bytes of its launches / HIP-event time of those
+                  launches on the renderer's stream, against 8 TB/s.  Bytes
+                  per query = 32*box tests + 36*triangle tests + 16*sphere
+                  tests + 48 (closest hit) or + 36 (shadow), from an untimed
+                  counting pass over the same rays.
+  cpu_baseline -- the CPU oracle (a C restatement of the reference's host
+                  transport path) on the host's cores over a 1-spp frame of
+                  the same scene and camera.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [os.path.join(HERE, "cuda-path-tracer_amd"), os.path.join(HERE, "oracle")]
+
+WIDTH, HEIGHT, SPP, DEPTH, FREQ, SEED = 1920, 1080, 64, 8, 224, 7
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def traffic_from_profiles():
+    """HBM bytes per trace launch measured by rocprofv3 --pmc (committed summary), or None."""
+    path = os.path.join(HERE, "profiles", "trace_kernel_icosphere1M_1080p_traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as f:
+            return json.load(f).get("hbm_bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(scene, camera, threads, spp=1):
+    import oracle
+    r = oracle.Renderer(scene, camera, WIDTH, HEIGHT, SEED, False, 0, True, DEPTH)
+    t0 = time.perf_counter()
+    r.add_samples(spp, threads=threads)
+    dt = time.perf_counter() - t0
+    rays = r.rays_traced()
+    return {"value": rays / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"{spp} spp over the full {WIDTH}x{HEIGHT} frame of the same scene/camera "
+                      f"({rays} rays, {dt:.1f} s, {threads} threads)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--spp", type=int, default=SPP)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (pixels would be sharded by rows "
+              "across GPUs with one final gather; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        return 2
+
+    import ptrace
+    ok, device = ptrace.device_available()
+    if not ok:
+        raise SystemExit(f"no HIP device: {device}")
+    log("device:", device)
+    t0 = time.perf_counter()
+    scene, camera = ptrace.icosphere(FREQ, SEED, WIDTH, HEIGHT)
+    log(f"scene: {len(scene.triangles)} triangles, {len(scene.nodes)} nodes, depth {scene.max_depth}, "
+        f"built in {time.perf_counter() - t0:.1f}s")
+    t0 = time.perf_counter()
+    r = ptrace.Renderer(scene, camera, WIDTH, HEIGHT, SEED, False, ptrace.DEBUG_SHADED, True, DEPTH)
+    log(f"upload {time.perf_counter() - t0:.2f}s")
+
+    # Untimed counting pass (same rays as every timed frame: same seed, reset).
+    r.set_tuning(count_tests=True)
+    r.add_samples(args.spp)
+    counts = r.last_launch_stats()
+    r.set_tuning(count_tests=False, time_kernels=True)
+    log("counting pass:", counts)
+
+    for _ in range(args.warmup):
+        r.reset(camera)
+        r.add_samples(args.spp)
+    rays = 0
+    trace_ms = shade_ms = 0.0
+    launches = 0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r.reset(camera)
+        r.add_samples(args.spp)
+        st = r.last_launch_stats()
+        rays += st["extend_rays"] + st["shadow_rays"]
+        trace_ms += st["trace_ms"]
+        shade_ms += st["shade_ms"]
+        launches += st["trace_launches"]
+    elapsed = time.perf_counter() - t0
+    ms_per_step = elapsed / args.steps * 1e3
+    mrays = rays / elapsed / 1e6
+    log(f"{args.steps} steps: {elapsed:.3f}s, {mrays:.1f} Mrays/s, trace {trace_ms:.1f} ms, shade {shade_ms:.1f} ms")
+
+    frame_ext, frame_sh = counts["extend_rays"], counts["shadow_rays"]
+    alg_bytes_frame = (32 * counts["box_tests"] + 36 * counts["triangle_tests"] + 16 * counts["sphere_tests"]
+                       + 48 * frame_ext + 36 * frame_sh)
+    per_step_launches = launches / args.steps
+    bytes_per_launch = alg_bytes_frame / per_step_launches
+    launch_s = trace_ms / 1e3 / launches
+    achieved = bytes_per_launch / launch_s / 1e9
+    traffic = traffic_from_profiles()
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "kernel": "trace_kernel<false>", "avg_launch_ms": round(launch_s * 1e3, 4),
+                "alg_bytes_per_launch": int(bytes_per_launch),
+                "box_tests_per_ray": round(counts["box_tests"] / (frame_ext + frame_sh), 2),
+                "triangle_tests_per_ray": round(counts["triangle_tests"] / (frame_ext + frame_sh), 2)}
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, len(os.sched_getaffinity(0)))
+        log(f"CPU baseline on {threads} threads ...")
+        cpu = cpu_baseline(scene, camera, threads)
+        log("cpu:", cpu)
+
+    line = {"metric": "Mrays/s at 1080p, 1M tris", "value": round(mrays, 2), "unit": "Mrays/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 2),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic (seeded displaced icosphere, seed 7)",
+            "config": {"workload": "plain wavefront path tracing, icosphere 1,003,530 tris, 1920x1080, "
+                                   f"{args.spp} spp, depth {DEPTH}, MIS", "width": WIDTH, "height": HEIGHT,
+                       "spp": args.spp, "max_depth": DEPTH, "triangles": int(len(scene.triangles)),
+                       "rays_per_step": rays // args.steps, "trace_ms_per_step": round(trace_ms / args.steps, 2),
+                       "shade_ms_per_step": round(shade_ms / args.steps, 2),
+                       "bounce_iterations": counts["iterations"]},
+            "roofline": roofline, "cpu_baseline": cpu}
+    if cpu:
+        line["gpu_over_cpu"] = round(mrays / cpu["value"], 1)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

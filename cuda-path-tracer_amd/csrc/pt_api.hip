@@ -407,7 +407,7 @@ struct pt_renderer {
     dev_buf<int4> hit;
     dev_buf<int32_t> queue0, queue1, shadow_queue;
     pt::paths_dev paths{};
-    pt_tuning tuning{256, 256, 4, 8, 0};
+    pt_tuning tuning{256, 256, 4, 8, 0, 0};
     int cus = 1;
     pt_launch_stats last{};
 
@@ -461,7 +461,8 @@ struct pt_renderer {
             for(;;) {
                 for(int k = 0; k < interval; k++) {
                     if(tuning.time_kernels) HIP_CHECK(hipEventRecord(kernel_events[size_t(4 * k)], stream));
-                    pt::launch_trace(scene->view, frame, paths, ctl.get(), parity, trace_grid, tuning.trace_block, stream);
+                    pt::launch_trace(scene->view, frame, paths, ctl.get(), parity, trace_grid, tuning.trace_block,
+                                     tuning.count_tests != 0, stream);
                     if(tuning.time_kernels) HIP_CHECK(hipEventRecord(kernel_events[size_t(4 * k + 1)], stream));
                     pt::launch_bookkeep(ctl.get(), parity, stream);
                     if(tuning.time_kernels) HIP_CHECK(hipEventRecord(kernel_events[size_t(4 * k + 2)], stream));
@@ -487,6 +488,9 @@ struct pt_renderer {
             st.iterations = int32_t(ctl_host->iterations);
             st.extend_rays = ctl_host->ext_rays;
             st.shadow_rays = ctl_host->shadow_rays;
+            st.box_tests = ctl_host->box_tests;
+            st.triangle_tests = ctl_host->triangle_tests;
+            st.sphere_tests = ctl_host->sphere_tests;
         }
         HIP_CHECK(hipEventSynchronize(stop));
         if(ctl_host->failure) fail(PT_EOVERFLOW, "HIP traversal stack overflow");
@@ -739,6 +743,7 @@ pt_status pt_renderer_set_tuning(pt_renderer *renderer, const pt_tuning *tuning)
         if(tuning->blocks_per_cu) { if(tuning->blocks_per_cu < 1 || tuning->blocks_per_cu > 64) fail(PT_EINVAL, "blocks per CU must be 1..64"); t.blocks_per_cu = tuning->blocks_per_cu; }
         if(tuning->check_interval) { if(tuning->check_interval < 1 || tuning->check_interval > 4096) fail(PT_EINVAL, "check interval must be 1..4096"); t.check_interval = tuning->check_interval; }
         t.time_kernels = tuning->time_kernels ? 1 : 0;
+        t.count_tests = tuning->count_tests ? 1 : 0;
         renderer->tuning = t;
     });
 }

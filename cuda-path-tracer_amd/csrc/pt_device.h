@@ -72,6 +72,9 @@ struct loop_ctl {
     unsigned long long ext_rays, shadow_rays;
     uint32_t iterations;
     int32_t failure;         // traversal stack overflow
+    // Test counts of a counting pass (trace_stats of every query, split by
+    // primitive kind): box tests, triangle tests, sphere tests.
+    unsigned long long box_tests, triangle_tests, sphere_tests;
 };
 
 struct frame_params {
@@ -92,7 +95,7 @@ void launch_debug(const scene_dev& s, const frame_params& f, float *sum, uint64_
 void launch_start(const scene_dev& s, const frame_params& f, const paths_dev& p, loop_ctl *ctl, int32_t samples,
                   hipStream_t stream);
 void launch_trace(const scene_dev& s, const frame_params& f, const paths_dev& p, loop_ctl *ctl, int parity,
-                  int grid, int block, hipStream_t stream);
+                  int grid, int block, bool count_tests, hipStream_t stream);
 void launch_bookkeep(loop_ctl *ctl, int parity, hipStream_t stream);
 void launch_shade(const scene_dev& s, const frame_params& f, const paths_dev& p, loop_ctl *ctl, int parity,
                   int grid, int block, hipStream_t stream);

@@ -176,7 +176,7 @@ struct best_hit { float t, u, v; int32_t code; };
 // LDS with a stride of the block size (entry k of a lane at stk[k*stride]).
 template<bool ANY_HIT, bool STATS>
 __device__ bool traverse_bvh(const scene_dev& s, const ray_in& r, float low, float high, int32_t *stk, int stride,
-                             best_hit& best, int& boxes, int& prims, int& error) {
+                             best_hit& best, int& boxes, int& prims, int& spheres, int& error) {
     if(s.root_ref == empty_root) return false;
     bool found = false;
     float closest = high;
@@ -202,6 +202,7 @@ __device__ bool traverse_bvh(const scene_dev& s, const ray_in& r, float low, flo
             int32_t prim = code >> 1;
             float t, u = 0, v = 0;
             bool hit;
+            if(STATS && (code & 1)) spheres++;
             if(code & 1) hit = sphere_test(s.sph_isect[prim], r, low, limit, t);
             else hit = tri_test(s.tri_isect[3 * prim], s.tri_isect[3 * prim + 1], s.tri_isect[3 * prim + 2], r, low, limit, t, u, v);
             if(hit && (!found || t <= closest)) {
@@ -216,7 +217,8 @@ __device__ bool traverse_bvh(const scene_dev& s, const ray_in& r, float low, flo
 
 // Brute mode (transport.h:48-60): every leaf in node-array order, no pruning.
 template<bool ANY_HIT>
-__device__ bool traverse_brute(const scene_dev& s, const ray_in& r, float low, float high, best_hit& best, int& prims) {
+__device__ bool traverse_brute(const scene_dev& s, const ray_in& r, float low, float high, best_hit& best, int& prims,
+                               int& spheres) {
     if(s.root_ref == empty_root) return false;
     bool found = false;
     float closest = high;
@@ -224,6 +226,7 @@ __device__ bool traverse_brute(const scene_dev& s, const ray_in& r, float low, f
         const pt_flat_node& n = s.nodes[i];
         if(n.type == PT_BRANCH) continue;
         prims++;
+        if(n.type == PT_SPHERE) spheres++;
         float t, u = 0, v = 0;
         bool hit;
         int32_t code = n.primitive * 2 + (n.type == PT_SPHERE ? 1 : 0);
@@ -241,8 +244,21 @@ __device__ bool traverse_brute(const scene_dev& s, const ray_in& r, float low, f
 template<bool ANY_HIT, bool STATS>
 __device__ __forceinline__ bool trace(const scene_dev& s, bool brute, const ray_in& r, float low, float high,
                                       int32_t *stk, int stride, best_hit& best, int& boxes, int& prims, int& error) {
-    if(brute) return traverse_brute<ANY_HIT>(s, r, low, high, best, prims);
-    return traverse_bvh<ANY_HIT, STATS>(s, r, low, high, stk, stride, best, boxes, prims, error);
+    int spheres = 0;
+    if(brute) return traverse_brute<ANY_HIT>(s, r, low, high, best, prims, spheres);
+    return traverse_bvh<ANY_HIT, STATS>(s, r, low, high, stk, stride, best, boxes, prims, spheres, error);
+}
+template<bool ANY_HIT>
+__device__ __forceinline__ bool trace_counted(const scene_dev& s, bool brute, const ray_in& r, float low, float high,
+                                              int32_t *stk, int stride, best_hit& best, int& boxes, int& prims,
+                                              int& spheres, int& error) {
+    if(brute) return traverse_brute<ANY_HIT>(s, r, low, high, best, prims, spheres);
+    return traverse_bvh<ANY_HIT, true>(s, r, low, high, stk, stride, best, boxes, prims, spheres, error);
+}
+
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
+    for(int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
 }
 
 // ---------------- hit record (flat_intersect.h:28-42,59-64; transport.h:39) ----------------
@@ -565,6 +581,7 @@ __global__ void start_kernel(scene_dev s, frame_params f, paths_dev p, loop_ctl 
     if(i == 0) {
         ctl->n_ext[0] = uint32_t(n); ctl->n_ext[1] = 0; ctl->n_shadow = 0; ctl->work = 0;
         ctl->iterations = 0; ctl->ext_rays = 0; ctl->shadow_rays = 0;
+        ctl->box_tests = 0; ctl->triangle_tests = 0; ctl->sphere_tests = 0;
     }
     if(i >= n) return;
     rng_t r = pixel_rng(p.rng[i], uint64_t(i));
@@ -581,7 +598,9 @@ __global__ void start_kernel(scene_dev s, frame_params f, paths_dev p, loop_ctl 
 }
 
 // Closest hit for the extension queue and any-hit for the shadow queue.
-// Persistent: each wave claims 64 consecutive items at a time.
+// Persistent: each wave claims 64 consecutive items at a time.  COUNT adds the
+// per-query test counts to the control block (untimed counting passes).
+template<bool COUNT>
 __global__ __launch_bounds__(256) void trace_kernel(scene_dev s, frame_params f, paths_dev p, loop_ctl *ctl, int parity) {
     extern __shared__ int32_t lds[];
     const uint32_t n_ext = ctl->n_ext[parity];
@@ -592,34 +611,47 @@ __global__ __launch_bounds__(256) void trace_kernel(scene_dev s, frame_params f,
     const int stride = int(blockDim.x);
     const bool brute = f.brute != 0;
     int error = 0;
+    unsigned long long n_box = 0, n_leaf = 0, n_sph = 0;
     for(;;) {
         uint32_t base = 0;
         if(lane == 0) base = atomicAdd(&ctl->work, 64u);
         base = __shfl(base, 0);
         if(base >= total) break;
         uint32_t item = base + uint32_t(lane);
+        int boxes = 0, prims = 0, spheres = 0;
         if(item < n_ext) {
             int32_t pix = q[item];
             float4 o = p.ray_o[pix], d = p.ray_d[pix];
             ray_in r = setup_ray(o.x, o.y, o.z, d.x, d.y, d.z);
-            best_hit b; int boxes = 0, prims = 0;
-            bool hit = trace<false, false>(s, brute, r, 0.001f, FLT_MAX, stk, stride, b, boxes, prims, error);
+            best_hit b;
+            bool hit = COUNT ? trace_counted<false>(s, brute, r, 0.001f, FLT_MAX, stk, stride, b, boxes, prims, spheres, error)
+                             : trace<false, false>(s, brute, r, 0.001f, FLT_MAX, stk, stride, b, boxes, prims, error);
             p.hit[pix] = hit ? make_int4(__float_as_int(b.t), __float_as_int(b.u), __float_as_int(b.v), b.code)
                              : make_int4(0, 0, 0, -1);
         } else if(item < total) {
             int32_t pix = p.shadow_queue[item - n_ext];
             float4 o = p.sh_o[pix], d = p.sh_d[pix];
             ray_in r = setup_ray(o.x, o.y, o.z, d.x, d.y, d.z);
-            best_hit b; int boxes = 0, prims = 0;
-            bool blocked = trace<true, false>(s, brute, r, 0.001f, o.w, stk, stride, b, boxes, prims, error);
+            best_hit b;
+            bool blocked = COUNT ? trace_counted<true>(s, brute, r, 0.001f, o.w, stk, stride, b, boxes, prims, spheres, error)
+                                 : trace<true, false>(s, brute, r, 0.001f, o.w, stk, stride, b, boxes, prims, error);
             if(!blocked) {
                 float4 c = p.sh_c[pix], ra = p.rad[pix];
                 ra.x = ra.x + c.x; ra.y = ra.y + c.y; ra.z = ra.z + c.z;
                 p.rad[pix] = ra;
             }
         }
+        if(COUNT) { n_box += unsigned(boxes); n_leaf += unsigned(prims); n_sph += unsigned(spheres); }
     }
     if(error) atomicExch(&ctl->failure, 1);
+    if(COUNT) {
+        n_box = wave_sum(n_box); n_leaf = wave_sum(n_leaf); n_sph = wave_sum(n_sph);
+        if(lane == 0) {
+            atomicAdd(&ctl->box_tests, n_box);
+            atomicAdd(&ctl->triangle_tests, n_leaf - n_sph);
+            atomicAdd(&ctl->sphere_tests, n_sph);
+        }
+    }
 }
 
 __global__ void bookkeep_kernel(loop_ctl *ctl, int parity) {
@@ -778,9 +810,13 @@ void launch_start(const scene_dev& s, const frame_params& f, const paths_dev& p,
     hipLaunchKernelGGL(start_kernel, dim3(grid), dim3(256), 0, stream, s, f, p, ctl, samples);
 }
 void launch_trace(const scene_dev& s, const frame_params& f, const paths_dev& p, loop_ctl *ctl, int parity,
-                  int grid, int block, hipStream_t stream) {
-    hipLaunchKernelGGL(trace_kernel, dim3(unsigned(grid)), dim3(unsigned(block)), stack_bytes(s.stack_capacity, block),
-                       stream, s, f, p, ctl, parity);
+                  int grid, int block, bool count_tests, hipStream_t stream) {
+    if(count_tests)
+        hipLaunchKernelGGL(trace_kernel<true>, dim3(unsigned(grid)), dim3(unsigned(block)),
+                           stack_bytes(s.stack_capacity, block), stream, s, f, p, ctl, parity);
+    else
+        hipLaunchKernelGGL(trace_kernel<false>, dim3(unsigned(grid)), dim3(unsigned(block)),
+                           stack_bytes(s.stack_capacity, block), stream, s, f, p, ctl, parity);
 }
 void launch_bookkeep(loop_ctl *ctl, int parity, hipStream_t stream) {
     hipLaunchKernelGGL(bookkeep_kernel, dim3(1), dim3(1), 0, stream, ctl, parity);

@@ -96,12 +96,13 @@ class _Settings(ctypes.Structure):
 
 class LaunchStats(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int32), ("trace_launches", ctypes.c_int32), ("trace_ms", ctypes.c_float),
-                ("shade_ms", ctypes.c_float), ("extend_rays", ctypes.c_uint64), ("shadow_rays", ctypes.c_uint64)]
+                ("shade_ms", ctypes.c_float), ("extend_rays", ctypes.c_uint64), ("shadow_rays", ctypes.c_uint64),
+                ("box_tests", ctypes.c_uint64), ("triangle_tests", ctypes.c_uint64), ("sphere_tests", ctypes.c_uint64)]
 
 
 class Tuning(ctypes.Structure):
     _fields_ = [("trace_block", ctypes.c_int32), ("shade_block", ctypes.c_int32), ("blocks_per_cu", ctypes.c_int32),
-                ("check_interval", ctypes.c_int32), ("time_kernels", ctypes.c_int32)]
+                ("check_interval", ctypes.c_int32), ("time_kernels", ctypes.c_int32), ("count_tests", ctypes.c_int32)]
 
 
 def _ptr(a):
@@ -292,8 +293,10 @@ class Renderer:
         _check(lib().pt_renderer_last_launch_stats(self._h, ctypes.byref(st)))
         return {k: getattr(st, k) for k, _ in LaunchStats._fields_}
 
-    def set_tuning(self, trace_block=0, shade_block=0, blocks_per_cu=0, check_interval=0, time_kernels=False):
-        t = Tuning(trace_block, shade_block, blocks_per_cu, check_interval, int(bool(time_kernels)))
+    def set_tuning(self, trace_block=0, shade_block=0, blocks_per_cu=0, check_interval=0, time_kernels=False,
+                   count_tests=False):
+        t = Tuning(trace_block, shade_block, blocks_per_cu, check_interval, int(bool(time_kernels)),
+                   int(bool(count_tests)))
         _check(lib().pt_renderer_set_tuning(self._h, ctypes.byref(t)))
 
 

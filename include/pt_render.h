@@ -160,6 +160,9 @@ typedef struct pt_launch_stats {
     float shade_ms;
     uint64_t extend_rays;
     uint64_t shadow_rays;
+    uint64_t box_tests;      /* filled when pt_tuning.count_tests is set */
+    uint64_t triangle_tests;
+    uint64_t sphere_tests;
 } pt_launch_stats;
 pt_status pt_renderer_last_launch_stats(const pt_renderer *renderer, pt_launch_stats *stats);
 
@@ -170,6 +173,7 @@ typedef struct pt_tuning {
     int32_t blocks_per_cu;   /* persistent trace blocks per compute unit */
     int32_t check_interval;  /* bounce iterations between host checks of the live count */
     int32_t time_kernels;    /* 1: record per-kernel HIP events (trace_ms, shade_ms) */
+    int32_t count_tests;     /* 1: count box/triangle/sphere tests (slower; for roofline bytes) */
 } pt_tuning;
 pt_status pt_renderer_set_tuning(pt_renderer *renderer, const pt_tuning *tuning);
 
