@@ -407,7 +407,7 @@ struct pt_renderer {
     dev_buf<int4> hit;
     dev_buf<int32_t> queue0, queue1, shadow_queue;
     pt::paths_dev paths{};
-    pt_tuning tuning{256, 256, 4, 8, 0, 0};
+    pt_tuning tuning{256, 256, 8, 8, 0, 0};
     int cus = 1;
     pt_launch_stats last{};
 

@@ -2,8 +2,9 @@
 
 Bars (written per test):
   * closest-hit records, traversal counts and the diagnostic views: bit-exact;
-  * sphere texture coordinates: within 2 ulp (the reference's host build
-    calls glibc atan2f/asinf, the device evaluates atan2/asin in double);
+  * sphere texture coordinates: within 1.2e-7 absolute (the reference's host
+    build calls glibc atan2f/asinf, the device evaluates atan2/asin in double
+    and rounds; one float ulp of an angle below pi, divided by 2*pi or pi);
   * shaded images: bit-exact on every pixel whose path never evaluates the
     double pow of the glass branch differently -- asserted as >= 99.9% of
     pixels identical and relative L2 < 1e-6 overall; ray counts exact.
@@ -35,7 +36,7 @@ def _ulp_diff(a, b):
 
 
 def _compare_results(got, want):
-    """Bit-exact except sphere u,v (<= 2 ulp)."""
+    """Bit-exact except sphere u,v (absolute 1.2e-7)."""
     got, want = got.view(ptrace.TRACE_RESULT), want.view(ptrace.TRACE_RESULT)
     assert np.array_equal(got["hit"], want["hit"]) and np.array_equal(got["error"], want["error"])
     sphere = want["rec"]["primitive_type"] == ptrace.SPHERE_PRIM
@@ -43,7 +44,8 @@ def _compare_results(got, want):
         assert np.array_equal(got["rec"][f], want["rec"][f]), f
     for f in ("u", "v"):
         assert np.array_equal(got["rec"][f][~sphere], want["rec"][f][~sphere]), f
-        assert _ulp_diff(got["rec"][f][sphere], want["rec"][f][sphere]).max(initial=0) <= 2, f
+        diff = np.abs(got["rec"][f][sphere].astype(np.float64) - want["rec"][f][sphere])
+        assert diff.max(initial=0) <= 1.2e-7, f
 
 
 @pytest.mark.parametrize("name", ASSETS)
