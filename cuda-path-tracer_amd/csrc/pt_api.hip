@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <limits>
 #include <map>
 #include <memory>
@@ -285,6 +286,121 @@ void require_device() {
     if(count < 1) fail(PT_EDEVICE, "HIP device unavailable: no HIP devices");
 }
 
+// ---------------- 4-wide layout for the shaded path ----------------
+// Collapses the flat binary tree: every wide node takes up to four of the
+// binary descendants (splitting the child of largest surface area first), and
+// every subtree of at most wide_leaf_max leaves becomes a leaf cluster.  Leaf
+// positions follow the depth-first, left-first leaf order of the flat tree, so
+// a subtree's leaves are one contiguous range and a leaf's position is the
+// order in which the reference's traversal would reach it (its tie rank).
+struct wide_layout {
+    std::vector<float4> nodes;   // 8 per node
+    std::vector<float4> leaves;  // 3 per leaf position
+    int32_t root_ref = pt::empty_root;
+    int depth = 0;
+};
+
+bool build_wide(const pt_scene_desc& s, const std::vector<float4>& ti, wide_layout& out) {
+    if(s.root < 0) return false;
+    const int64_t n = s.node_count;
+    std::vector<int32_t> first(size_t(n), -1), count(size_t(n), 0);
+    std::vector<int32_t> leaf_code;
+    leaf_code.reserve(size_t(s.triangle_count + s.sphere_count));
+    // Iterative depth-first walk, left before right; a node reached twice (a
+    // shared subtree) disables the wide layout.
+    struct item { int32_t node; bool exit; };
+    std::vector<item> stack{{s.root, false}};
+    while(!stack.empty()) {
+        item it = stack.back(); stack.pop_back();
+        const auto& nd = s.nodes[it.node];
+        if(it.exit) { count[size_t(it.node)] = int32_t(leaf_code.size()) - first[size_t(it.node)]; continue; }
+        if(first[size_t(it.node)] >= 0) return false;
+        first[size_t(it.node)] = int32_t(leaf_code.size());
+        if(nd.type == PT_BRANCH) {
+            stack.push_back({it.node, true});
+            if(nd.left != nd.right) stack.push_back({nd.right, false});
+            stack.push_back({nd.left, false});
+        } else {
+            leaf_code.push_back(nd.primitive * 2 + (nd.type == PT_SPHERE ? 1 : 0));
+            count[size_t(it.node)] = 1;
+        }
+    }
+    if(leaf_code.size() >= (size_t(1) << 27)) return false;
+    out.leaves.resize(3 * leaf_code.size());
+    for(size_t k = 0; k < leaf_code.size(); k++) {
+        int32_t code = leaf_code[k], prim = code >> 1;
+        float cf; std::memcpy(&cf, &code, 4);
+        if(code & 1) {
+            const auto& p = s.spheres[prim];
+            out.leaves[3 * k] = make_float4(p.center.x, p.center.y, p.center.z, p.radius);
+            out.leaves[3 * k + 1] = make_float4(0, 0, 0, -1);
+            out.leaves[3 * k + 2] = make_float4(0, 0, 0, cf);
+        } else {
+            out.leaves[3 * k] = ti[size_t(3 * prim)];
+            out.leaves[3 * k + 1] = ti[size_t(3 * prim + 1)];
+            float4 c = ti[size_t(3 * prim + 2)];
+            out.leaves[3 * k + 2] = make_float4(c.x, c.y, c.z, cf);
+        }
+    }
+    auto area = [&](int32_t node) {
+        const auto& b = s.nodes[node];
+        double dx = double(b.maximum.x) - b.minimum.x, dy = double(b.maximum.y) - b.minimum.y, dz = double(b.maximum.z) - b.minimum.z;
+        return dx * dy + dy * dz + dz * dx;
+    };
+    auto cluster_ref = [&](int32_t node) { return ~((first[size_t(node)] << 3) | (count[size_t(node)] - 1)); };
+    auto expandable = [&](int32_t node) { return s.nodes[node].type == PT_BRANCH && count[size_t(node)] > pt::wide_leaf_max; };
+    std::function<int32_t(int32_t, int)> make = [&](int32_t b, int depth) -> int32_t {
+        if(!expandable(b)) return cluster_ref(b);
+        out.depth = std::max(out.depth, depth);
+        std::vector<int32_t> kids;
+        const auto& nb = s.nodes[b];
+        kids.push_back(nb.left);
+        if(nb.left != nb.right) kids.push_back(nb.right);
+        while(kids.size() < 4) {
+            int best = -1; double best_area = -1;
+            for(size_t k = 0; k < kids.size(); k++)
+                if(expandable(kids[k]) && area(kids[k]) > best_area) { best = int(k); best_area = area(kids[k]); }
+            if(best < 0) break;
+            const auto& nk = s.nodes[kids[size_t(best)]];
+            std::vector<int32_t> repl{nk.left};
+            if(nk.left != nk.right) repl.push_back(nk.right);
+            kids.erase(kids.begin() + best);
+            kids.insert(kids.begin() + best, repl.begin(), repl.end());
+            if(kids.size() > 4) {  // cannot happen: one slot becomes at most two
+                return -1;
+            }
+        }
+        size_t w = out.nodes.size() / 8;
+        out.nodes.resize(out.nodes.size() + 8);
+        float lo[3][4], hi[3][4];
+        int32_t ref[4];
+        for(int k = 0; k < 4; k++) {
+            for(int a = 0; a < 3; a++) { lo[a][k] = INFINITY; hi[a][k] = -INFINITY; }
+            ref[k] = pt::wide_empty;
+        }
+        for(size_t k = 0; k < kids.size(); k++) {
+            const auto& c = s.nodes[kids[k]];
+            lo[0][k] = c.minimum.x; lo[1][k] = c.minimum.y; lo[2][k] = c.minimum.z;
+            hi[0][k] = c.maximum.x; hi[1][k] = c.maximum.y; hi[2][k] = c.maximum.z;
+            ref[k] = make(kids[k], depth + 1);
+            if(ref[k] == -1) return -1;
+        }
+        float4 *dst = &out.nodes[8 * w];
+        for(int a = 0; a < 3; a++) {
+            dst[a] = make_float4(lo[a][0], lo[a][1], lo[a][2], lo[a][3]);
+            dst[3 + a] = make_float4(hi[a][0], hi[a][1], hi[a][2], hi[a][3]);
+        }
+        float rf[4];
+        std::memcpy(rf, ref, sizeof rf);
+        dst[6] = make_float4(rf[0], rf[1], rf[2], rf[3]);
+        dst[7] = make_float4(0, 0, 0, 0);
+        return int32_t(w);
+    };
+    out.depth = 1;
+    out.root_ref = make(s.root, 1);
+    return out.root_ref != -1;
+}
+
 // ---------------- uploaded scene (cuda_renderer.cu:36-54) ----------------
 struct device_scene {
     dev_buf<float4> bnodes, tri_isect, sph_isect;
@@ -296,10 +412,11 @@ struct device_scene {
     dev_buf<uint8_t> pixels;
     dev_buf<pt::light_rec> lights;
     dev_buf<float> log2_table;
+    dev_buf<float4> wnodes, leaves;
     pt::scene_dev view{};
     int64_t branch_count = 0;
 
-    device_scene(const pt_scene_desc& s, bool with_tables) {
+    device_scene(const pt_scene_desc& s, bool with_tables, bool with_wide = false) {
         // Traversal layout: branch nodes renumbered in array order, children
         // encoded as branch indices or leaf references.
         std::vector<int32_t> branch_id(size_t(s.node_count), -1);
@@ -360,6 +477,17 @@ struct device_scene {
         view.light_area = lights_host.empty() ? 0 : lights_host.back().cumulative_area;
         view.stack_capacity = std::max(2, std::min(stack_limit, int(s.max_depth)));
         view.trig = nullptr; view.log2_table = nullptr; view.log2_count = 0;
+        view.wnodes = nullptr; view.leaves = nullptr; view.wroot_ref = pt::empty_root; view.wide_stack_capacity = 0;
+        if(with_wide) {
+            wide_layout w;
+            if(build_wide(s, ti, w)) {
+                wnodes.upload(w.nodes.data(), w.nodes.size());
+                leaves.upload(w.leaves.data(), w.leaves.size());
+                view.wnodes = wnodes.get(); view.leaves = leaves.get();
+                view.wroot_ref = w.root_ref;
+                view.wide_stack_capacity = 3 * w.depth + 1;
+            }
+        }
         if(with_tables) {
             view.trig = device_trig_table();
             // log2f(1+n) of the host libm for every count a traversal can reach.
@@ -406,8 +534,9 @@ struct pt_renderer {
     dev_buf<int2> ctr;
     dev_buf<int4> hit;
     dev_buf<int32_t> queue0, queue1, shadow_queue;
+    dev_buf<int2> spill;
     pt::paths_dev paths{};
-    pt_tuning tuning{256, 256, 8, 8, 0, 0};
+    pt_tuning tuning{128, 256, 16, 8, 0, 0, 0};
     int cus = 1;
     pt_launch_stats last{};
 
@@ -426,6 +555,17 @@ struct pt_renderer {
         HIP_CHECK(hipGetLastError());
         HIP_CHECK(hipStreamSynchronize(stream));
         frame.camera = camera; samples = 0; milliseconds = 0; rays = 0;
+    }
+
+    // Largest block of at most tuning.trace_block threads whose traversal
+    // stacks fit the 64 KiB of LDS a workgroup may hold; the wide traversal is
+    // switched off if even 64 threads do not fit.
+    int trace_block() {
+        frame.exact_traversal = tuning.exact_traversal;
+        int block = tuning.trace_block;
+        while(block > 64 && pt::trace_lds_bytes(scene->view, frame, block) > 65536) block -= 64;
+        if(pt::trace_lds_bytes(scene->view, frame, block) > 65536) frame.exact_traversal = 1;
+        return block;
     }
 
     void read_ctl() {
@@ -449,7 +589,11 @@ struct pt_renderer {
             pt::launch_start(scene->view, frame, paths, ctl.get(), count, stream);
             HIP_CHECK(hipGetLastError());
             int parity = 0;
+            const int trace_threads = trace_block();
             const int trace_grid = cus * std::max(1, tuning.blocks_per_cu);
+            size_t spill_need = size_t(trace_grid) * size_t(trace_threads) *
+                                size_t(pt::wide_spill_entries(scene->view.wide_stack_capacity));
+            if(spill.size() < spill_need) { spill.alloc(spill_need); paths.spill = spill.get(); }
             const int shade_grid = int(std::min<size_t>((pixels + size_t(tuning.shade_block) - 1) / size_t(tuning.shade_block),
                                                         size_t(cus) * 16));
             const int interval = std::max(1, tuning.check_interval);
@@ -461,7 +605,7 @@ struct pt_renderer {
             for(;;) {
                 for(int k = 0; k < interval; k++) {
                     if(tuning.time_kernels) HIP_CHECK(hipEventRecord(kernel_events[size_t(4 * k)], stream));
-                    pt::launch_trace(scene->view, frame, paths, ctl.get(), parity, trace_grid, tuning.trace_block,
+                    pt::launch_trace(scene->view, frame, paths, ctl.get(), parity, trace_grid, trace_threads,
                                      tuning.count_tests != 0, stream);
                     if(tuning.time_kernels) HIP_CHECK(hipEventRecord(kernel_events[size_t(4 * k + 1)], stream));
                     pt::launch_bookkeep(ctl.get(), parity, stream);
@@ -614,7 +758,7 @@ pt_status pt_renderer_create(const pt_scene_desc *scene, const pt_flat_camera *c
         HIP_CHECK(hipEventCreate(&r->start));
         HIP_CHECK(hipEventCreate(&r->stop));
         r->cus = device_cu_count();
-        r->scene = std::make_unique<device_scene>(*scene, true);
+        r->scene = std::make_unique<device_scene>(*scene, true, settings->debug == PT_DEBUG_SHADED);
         allocate_state(*r, settings->debug == PT_DEBUG_SHADED);
         r->reset(*camera);
         *out = r.release();
@@ -744,6 +888,7 @@ pt_status pt_renderer_set_tuning(pt_renderer *renderer, const pt_tuning *tuning)
         if(tuning->check_interval) { if(tuning->check_interval < 1 || tuning->check_interval > 4096) fail(PT_EINVAL, "check interval must be 1..4096"); t.check_interval = tuning->check_interval; }
         t.time_kernels = tuning->time_kernels ? 1 : 0;
         t.count_tests = tuning->count_tests ? 1 : 0;
+        t.exact_traversal = tuning->exact_traversal ? 1 : 0;
         renderer->tuning = t;
     });
 }

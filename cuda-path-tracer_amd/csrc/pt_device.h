@@ -6,6 +6,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "pt_render.h"
 
 namespace pt {
@@ -42,7 +44,22 @@ struct scene_dev {
     const float *log2_table;
     int32_t log2_count;
     int32_t stack_capacity;    // per-thread traversal stack entries (<= 64)
+    // 4-wide layout for the shaded path (nearest-first traversal):
+    //   wnodes: 8 float4 per node = lo.x[4] lo.y[4] lo.z[4] hi.x[4] hi.y[4] hi.z[4] ref[4] pad[4]
+    //   leaves: 3 float4 per depth-first leaf position, the tri_isect record of
+    //           its triangle with w of the third = primitive code; a sphere leaf
+    //           is {center, radius} {0,0,0,-1} {0,0,0,code}
+    // ref >= 0 is a wide node, wide_empty an unused slot, otherwise a leaf
+    // cluster ~(first << 3 | (count - 1)) over leaf positions.
+    const float4 *wnodes;
+    const float4 *leaves;
+    int32_t wroot_ref;         // empty_root when the scene is empty or wide traversal is off
+    int32_t wide_stack_capacity;
 };
+
+constexpr int32_t wide_empty = INT32_MIN + 1;
+constexpr int wide_leaf_max = 4;
+constexpr int wide_lds_entries = 12;  // per-lane stack entries kept in LDS
 
 constexpr int32_t empty_root = INT32_MIN;
 
@@ -61,6 +78,7 @@ struct paths_dev {
     float *sum;         // 3 floats per pixel (vec3 sum[], cuda_renderer.cu:145)
     int32_t *queue[2];  // extension queues (pixel ids), ping-pong
     int32_t *shadow_queue;
+    int2 *spill;        // wide traversal stack entries beyond wide_lds_entries
 };
 
 // Device control block of the bounce loop.
@@ -84,6 +102,7 @@ struct frame_params {
     int32_t direct_light;
     int32_t brute;
     int32_t debug;
+    int32_t exact_traversal; // shaded path on the reference-order binary traversal
 };
 
 // ---- launchers (pt_kernels.hip) ----
@@ -104,5 +123,16 @@ void launch_add_rays(loop_ctl *ctl, unsigned long long rays, hipStream_t stream)
 
 // Dynamic LDS bytes the traversal kernels need per block.
 inline size_t stack_bytes(int capacity, int threads) { return size_t(capacity) * size_t(threads) * sizeof(int32_t); }
+// The wide traversal keeps (ref, entry distance) pairs, wide_lds_entries of them in LDS.
+inline size_t wide_stack_bytes(int capacity, int threads) {
+    return 2 * stack_bytes(std::min(capacity, wide_lds_entries), threads);
+}
+// Global overflow entries per lane of the wide traversal.
+inline int wide_spill_entries(int capacity) { return std::max(0, capacity - wide_lds_entries); }
+inline size_t trace_lds_bytes(const scene_dev& s, const frame_params& f, int threads) {
+    size_t exact = stack_bytes(s.stack_capacity, threads);
+    bool wide = !f.brute && s.wroot_ref != empty_root && !f.exact_traversal;
+    return wide ? std::max(exact, wide_stack_bytes(s.wide_stack_capacity, threads)) : exact;
+}
 
 } // namespace pt

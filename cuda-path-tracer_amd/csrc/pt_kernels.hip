@@ -241,6 +241,133 @@ __device__ bool traverse_brute(const scene_dev& s, const ray_in& r, float low, f
     return found;
 }
 
+// ---------------- 4-wide nearest-first traversal (shaded path) ----------------
+// Same slab arithmetic as trace_box (transport.h:14-31) with the reference's
+// near/far selection by the sign of 1/d, returning the entry distance.
+__device__ __forceinline__ bool slab_entry(float lox, float loy, float loz, float hix, float hiy, float hiz,
+                                           const ray_in& r, float low, float high, float& entry) {
+    bool ok = true;
+    if(r.dx == 0) ok = ok && !(r.ox < lox || r.ox > hix);
+    else {
+        float a = ((r.ix < 0 ? hix : lox) - r.ox) * r.ix, b = ((r.ix < 0 ? lox : hix) - r.ox) * r.ix;
+        low = a > low ? a : low; high = b < high ? b : high;
+    }
+    if(r.dy == 0) ok = ok && !(r.oy < loy || r.oy > hiy);
+    else {
+        float a = ((r.iy < 0 ? hiy : loy) - r.oy) * r.iy, b = ((r.iy < 0 ? loy : hiy) - r.oy) * r.iy;
+        low = a > low ? a : low; high = b < high ? b : high;
+    }
+    if(r.dz == 0) ok = ok && !(r.oz < loz || r.oz > hiz);
+    else {
+        float a = ((r.iz < 0 ? hiz : loz) - r.oz) * r.iz, b = ((r.iz < 0 ? loz : hiz) - r.oz) * r.iz;
+        low = a > low ? a : low; high = b < high ? b : high;
+    }
+    entry = low;
+    return ok && high > low;
+}
+
+// Closest (or any) hit over the 4-wide layout.  Children are visited nearest
+// first; a stacked child is dropped on pop when its entry distance is not
+// below the current limit, which is exactly the reference's pop-time box
+// test.  Among hits at the same distance the leaf of larger depth-first rank
+// wins, the reference's tie rule (transport.h:67-68,79).
+// Traversal stack of the wide path: the first wide_lds_entries entries of a
+// lane live in LDS (entry k at lds[k*stride]), deeper ones in a global
+// overflow region laid out so that a wave's accesses are coalesced.
+struct wide_stack {
+    int2 *lds;
+    int stride;
+    int2 *spill;
+    size_t spill_stride;
+    size_t lane_id;
+    __device__ __forceinline__ void store(int k, int32_t ref, float entry) {
+        int2 v = make_int2(ref, __float_as_int(entry));
+        if(k < wide_lds_entries) lds[k * stride] = v;
+        else spill[size_t(k - wide_lds_entries) * spill_stride + lane_id] = v;
+    }
+    __device__ __forceinline__ int2 load(int k) const {
+        return k < wide_lds_entries ? lds[k * stride] : spill[size_t(k - wide_lds_entries) * spill_stride + lane_id];
+    }
+};
+
+template<bool ANY_HIT, bool STATS>
+__device__ bool traverse_wide(const scene_dev& s, const ray_in& r, float low, float high, wide_stack& stk,
+                              best_hit& best, int& boxes, int& prims, int& spheres, int& error) {
+    bool found = false;
+    float closest = high;
+    int32_t best_rank = -1;
+    int32_t cur = s.wroot_ref;
+    int count = 0;
+    const int cap = s.wide_stack_capacity;
+    for(;;) {
+        if(cur >= 0) {
+            const float4 *nd = s.wnodes + 8 * size_t(cur);
+            const float4 LX = nd[0], LY = nd[1], LZ = nd[2], HX = nd[3], HY = nd[4], HZ = nd[5], RF = nd[6];
+            const int32_t r0 = __float_as_int(RF.x), r1 = __float_as_int(RF.y), r2 = __float_as_int(RF.z), r3 = __float_as_int(RF.w);
+            const float limit = found ? next_toward(closest, high) : high;
+            float e[4];
+            int32_t c[4];
+            bool h0 = r0 != wide_empty && slab_entry(LX.x, LY.x, LZ.x, HX.x, HY.x, HZ.x, r, low, limit, e[0]);
+            bool h1 = r1 != wide_empty && slab_entry(LX.y, LY.y, LZ.y, HX.y, HY.y, HZ.y, r, low, limit, e[1]);
+            bool h2 = r2 != wide_empty && slab_entry(LX.z, LY.z, LZ.z, HX.z, HY.z, HZ.z, r, low, limit, e[2]);
+            bool h3 = r3 != wide_empty && slab_entry(LX.w, LY.w, LZ.w, HX.w, HY.w, HZ.w, r, low, limit, e[3]);
+            if(STATS) boxes += (r0 != wide_empty) + (r1 != wide_empty) + (r2 != wide_empty) + (r3 != wide_empty);
+            if(!h0) e[0] = INFINITY;
+            if(!h1) e[1] = INFINITY;
+            if(!h2) e[2] = INFINITY;
+            if(!h3) e[3] = INFINITY;
+            c[0] = r0; c[1] = r1; c[2] = r2; c[3] = r3;
+            const int nh = int(h0) + int(h1) + int(h2) + int(h3);
+            if(nh) {
+                if(!ANY_HIT) {
+#define PT_CSWAP(i, j) if(e[j] < e[i]) { float te = e[i]; e[i] = e[j]; e[j] = te; int32_t tc = c[i]; c[i] = c[j]; c[j] = tc; }
+                    PT_CSWAP(0, 1) PT_CSWAP(2, 3) PT_CSWAP(0, 2) PT_CSWAP(1, 3) PT_CSWAP(1, 2)
+#undef PT_CSWAP
+                } else {
+                    // any order: move the hit children to the front
+                    if(!h0) { if(h1) { e[0] = e[1]; c[0] = c[1]; h1 = false; } else if(h2) { e[0] = e[2]; c[0] = c[2]; h2 = false; } else { e[0] = e[3]; c[0] = c[3]; h3 = false; } }
+                    int k = 1;
+                    if(h1) { e[k] = e[1]; c[k] = c[1]; k++; }
+                    if(h2) { e[k] = e[2]; c[k] = c[2]; k++; }
+                    if(h3) { e[k] = e[3]; c[k] = c[3]; k++; }
+                }
+                if(count + nh - 1 > cap) { error = 1; return false; }
+                for(int k = nh - 1; k >= 1; k--) { stk.store(count, c[k], e[k]); count++; }
+                cur = c[0];
+                continue;
+            }
+        } else {
+            const int32_t v = ~cur;
+            const int32_t first = v >> 3, n = (v & 7) + 1;
+            for(int j = 0; j < n; j++) {
+                const int32_t k = first + j;
+                const float4 A = s.leaves[3 * size_t(k)], B = s.leaves[3 * size_t(k) + 1], C = s.leaves[3 * size_t(k) + 2];
+                const float limit = found ? next_toward(closest, high) : high;
+                float t, u = 0, vv = 0;
+                bool hit;
+                if(B.w < 0) { hit = sphere_test(A, r, low, limit, t); if(STATS) spheres++; }
+                else hit = tri_test(A, B, C, r, low, limit, t, u, vv);
+                if(STATS) prims++;
+                if(hit && (!found || t < closest || (t == closest && k > best_rank))) {
+                    found = true; closest = t; best_rank = k;
+                    best.t = t; best.u = u; best.v = vv; best.code = __float_as_int(C.w);
+                    if(ANY_HIT) return true;
+                }
+            }
+        }
+        for(;;) {
+            if(count == 0) return found;
+            --count;
+            const int2 top = stk.load(count);
+            const int32_t ref = top.x;
+            const float entry = __int_as_float(top.y);
+            if(found && !(entry < next_toward(closest, high))) continue;
+            cur = ref;
+            break;
+        }
+    }
+}
+
 template<bool ANY_HIT, bool STATS>
 __device__ __forceinline__ bool trace(const scene_dev& s, bool brute, const ray_in& r, float low, float high,
                                       int32_t *stk, int stride, best_hit& best, int& boxes, int& prims, int& error) {
@@ -610,6 +737,9 @@ __global__ __launch_bounds__(256) void trace_kernel(scene_dev s, frame_params f,
     int32_t *stk = lds + threadIdx.x;
     const int stride = int(blockDim.x);
     const bool brute = f.brute != 0;
+    const bool wide = !brute && s.wroot_ref != empty_root && !f.exact_traversal;
+    wide_stack wstk{reinterpret_cast<int2 *>(lds) + threadIdx.x, stride, p.spill,
+                    size_t(gridDim.x) * blockDim.x, size_t(blockIdx.x) * blockDim.x + threadIdx.x};
     int error = 0;
     unsigned long long n_box = 0, n_leaf = 0, n_sph = 0;
     for(;;) {
@@ -624,7 +754,9 @@ __global__ __launch_bounds__(256) void trace_kernel(scene_dev s, frame_params f,
             float4 o = p.ray_o[pix], d = p.ray_d[pix];
             ray_in r = setup_ray(o.x, o.y, o.z, d.x, d.y, d.z);
             best_hit b;
-            bool hit = COUNT ? trace_counted<false>(s, brute, r, 0.001f, FLT_MAX, stk, stride, b, boxes, prims, spheres, error)
+            bool hit;
+            if(wide) hit = traverse_wide<false, COUNT>(s, r, 0.001f, FLT_MAX, wstk, b, boxes, prims, spheres, error);
+            else hit = COUNT ? trace_counted<false>(s, brute, r, 0.001f, FLT_MAX, stk, stride, b, boxes, prims, spheres, error)
                              : trace<false, false>(s, brute, r, 0.001f, FLT_MAX, stk, stride, b, boxes, prims, error);
             p.hit[pix] = hit ? make_int4(__float_as_int(b.t), __float_as_int(b.u), __float_as_int(b.v), b.code)
                              : make_int4(0, 0, 0, -1);
@@ -633,7 +765,9 @@ __global__ __launch_bounds__(256) void trace_kernel(scene_dev s, frame_params f,
             float4 o = p.sh_o[pix], d = p.sh_d[pix];
             ray_in r = setup_ray(o.x, o.y, o.z, d.x, d.y, d.z);
             best_hit b;
-            bool blocked = COUNT ? trace_counted<true>(s, brute, r, 0.001f, o.w, stk, stride, b, boxes, prims, spheres, error)
+            bool blocked;
+            if(wide) blocked = traverse_wide<true, COUNT>(s, r, 0.001f, o.w, wstk, b, boxes, prims, spheres, error);
+            else blocked = COUNT ? trace_counted<true>(s, brute, r, 0.001f, o.w, stk, stride, b, boxes, prims, spheres, error)
                                  : trace<true, false>(s, brute, r, 0.001f, o.w, stk, stride, b, boxes, prims, error);
             if(!blocked) {
                 float4 c = p.sh_c[pix], ra = p.rad[pix];
@@ -811,12 +945,11 @@ void launch_start(const scene_dev& s, const frame_params& f, const paths_dev& p,
 }
 void launch_trace(const scene_dev& s, const frame_params& f, const paths_dev& p, loop_ctl *ctl, int parity,
                   int grid, int block, bool count_tests, hipStream_t stream) {
+    size_t lds = trace_lds_bytes(s, f, block);
     if(count_tests)
-        hipLaunchKernelGGL(trace_kernel<true>, dim3(unsigned(grid)), dim3(unsigned(block)),
-                           stack_bytes(s.stack_capacity, block), stream, s, f, p, ctl, parity);
+        hipLaunchKernelGGL(trace_kernel<true>, dim3(unsigned(grid)), dim3(unsigned(block)), lds, stream, s, f, p, ctl, parity);
     else
-        hipLaunchKernelGGL(trace_kernel<false>, dim3(unsigned(grid)), dim3(unsigned(block)),
-                           stack_bytes(s.stack_capacity, block), stream, s, f, p, ctl, parity);
+        hipLaunchKernelGGL(trace_kernel<false>, dim3(unsigned(grid)), dim3(unsigned(block)), lds, stream, s, f, p, ctl, parity);
 }
 void launch_bookkeep(loop_ctl *ctl, int parity, hipStream_t stream) {
     hipLaunchKernelGGL(bookkeep_kernel, dim3(1), dim3(1), 0, stream, ctl, parity);

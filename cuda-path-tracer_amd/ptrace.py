@@ -102,7 +102,8 @@ class LaunchStats(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     _fields_ = [("trace_block", ctypes.c_int32), ("shade_block", ctypes.c_int32), ("blocks_per_cu", ctypes.c_int32),
-                ("check_interval", ctypes.c_int32), ("time_kernels", ctypes.c_int32), ("count_tests", ctypes.c_int32)]
+                ("check_interval", ctypes.c_int32), ("time_kernels", ctypes.c_int32), ("count_tests", ctypes.c_int32),
+                ("exact_traversal", ctypes.c_int32)]
 
 
 def _ptr(a):
@@ -294,9 +295,9 @@ class Renderer:
         return {k: getattr(st, k) for k, _ in LaunchStats._fields_}
 
     def set_tuning(self, trace_block=0, shade_block=0, blocks_per_cu=0, check_interval=0, time_kernels=False,
-                   count_tests=False):
+                   count_tests=False, exact_traversal=False):
         t = Tuning(trace_block, shade_block, blocks_per_cu, check_interval, int(bool(time_kernels)),
-                   int(bool(count_tests)))
+                   int(bool(count_tests)), int(bool(exact_traversal)))
         _check(lib().pt_renderer_set_tuning(self._h, ctypes.byref(t)))
 
 

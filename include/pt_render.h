@@ -127,8 +127,8 @@ pt_status pt_renderer_read_image(const pt_renderer *renderer, float *rgb, size_t
 /* cuda_renderer::rgba_image (cuda_renderer.h:21, .cu:241-244) */
 pt_status pt_renderer_read_rgba(const pt_renderer *renderer, uint8_t *rgba, size_t bytes);
 /* cuda_renderer::write_rgba (cuda_renderer.h:22, .cu:235-240): packs into a
- * caller-owned device buffer (e.g. a mapped GL buffer); asynchronous on the
- * renderer's stream exactly as the reference's kernel launch is. */
+ * caller-owned device buffer (e.g. a mapped GL buffer); returns after the
+ * pack kernel has finished. */
 pt_status pt_renderer_write_rgba_device(const pt_renderer *renderer, void *device_rgba, size_t bytes);
 /* sample_count / kernel_milliseconds / rays_traced (cuda_renderer.h:23-25) */
 pt_status pt_renderer_stats(const pt_renderer *renderer, int32_t *samples, float *kernel_ms, uint64_t *rays);
@@ -174,6 +174,8 @@ typedef struct pt_tuning {
     int32_t check_interval;  /* bounce iterations between host checks of the live count */
     int32_t time_kernels;    /* 1: record per-kernel HIP events (trace_ms, shade_ms) */
     int32_t count_tests;     /* 1: count box/triangle/sphere tests (slower; for roofline bytes) */
+    int32_t exact_traversal; /* 1: shaded path on the reference-order binary traversal instead of the
+                                4-wide nearest-first one (both return the same closest hit) */
 } pt_tuning;
 pt_status pt_renderer_set_tuning(pt_renderer *renderer, const pt_tuning *tuning);
 
