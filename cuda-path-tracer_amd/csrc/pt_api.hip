@@ -403,7 +403,7 @@ bool build_wide(const pt_scene_desc& s, const std::vector<float4>& ti, wide_layo
 
 // ---------------- uploaded scene (cuda_renderer.cu:36-54) ----------------
 struct device_scene {
-    dev_buf<float4> bnodes, tri_isect, sph_isect;
+    dev_buf<float4> bnodes, tri_isect, sph_isect, shade_tri;
     dev_buf<pt_flat_triangle> triangles;
     dev_buf<pt_flat_sphere> spheres;
     dev_buf<pt_flat_node> nodes;
@@ -452,6 +452,24 @@ struct device_scene {
             ti[size_t(3 * i + 1)] = make_float4(e1x, e1y, e1z, nl);
             ti[size_t(3 * i + 2)] = make_float4(e2x, e2y, e2z, 0);
         }
+        // Shading records (see scene_dev::shade_tri).  The outward normal is
+        // cross(e1,e2) scaled by float(1.0/|e1 x e2|) as vec3::operator/=
+        // does (src/core/vec3.h:152-158, flat_intersect.h:28).
+        std::vector<float4> sh(size_t(5 * s.triangle_count));
+        for(int64_t i = 0; i < s.triangle_count; i++) {
+            const auto& t = s.triangles[i];
+            const float4 B = ti[size_t(3 * i + 1)], C = ti[size_t(3 * i + 2)];
+            float cx = B.y * C.z - B.z * C.y, cy = -(B.x * C.z - B.z * C.x), cz = B.x * C.y - B.y * C.x;
+            float k = float(1.0 / double(B.w));
+            float mf, sf;
+            std::memcpy(&mf, &t.material_id, 4);
+            std::memcpy(&sf, &t.smooth, 4);
+            sh[size_t(5 * i)] = make_float4(cx * k, cy * k, cz * k, mf);
+            sh[size_t(5 * i + 1)] = make_float4(t.n0.x, t.n0.y, t.n0.z, sf);
+            sh[size_t(5 * i + 2)] = make_float4(t.n1.x, t.n1.y, t.n1.z, t.uv0.x);
+            sh[size_t(5 * i + 3)] = make_float4(t.n2.x, t.n2.y, t.n2.z, t.uv0.y);
+            sh[size_t(5 * i + 4)] = make_float4(t.uv1.x, t.uv1.y, t.uv2.x, t.uv2.y);
+        }
         std::vector<float4> si(size_t(s.sphere_count));
         for(int64_t i = 0; i < s.sphere_count; i++) {
             const auto& p = s.spheres[i];
@@ -461,6 +479,7 @@ struct device_scene {
         bnodes.upload(bn.data(), bn.size());
         tri_isect.upload(ti.data(), ti.size());
         sph_isect.upload(si.data(), si.size());
+        shade_tri.upload(sh.data(), sh.size());
         triangles.upload(s.triangles, size_t(s.triangle_count));
         spheres.upload(s.spheres, size_t(s.sphere_count));
         nodes.upload(s.nodes, size_t(s.node_count));
@@ -469,6 +488,7 @@ struct device_scene {
         pixels.upload(s.pixels, size_t(s.pixel_bytes));
         lights.upload(lights_host.data(), lights_host.size());
         view.bnodes = bnodes.get(); view.tri_isect = tri_isect.get(); view.sph_isect = sph_isect.get();
+        view.shade_tri = shade_tri.get();
         view.root_ref = s.root < 0 ? pt::empty_root : ref_of(s.root);
         view.triangles = triangles.get(); view.spheres = spheres.get(); view.nodes = nodes.get();
         view.node_count = int32_t(s.node_count);
@@ -533,7 +553,7 @@ struct pt_renderer {
     dev_buf<float4> ray_o, ray_d, thr, rad, sh_o, sh_d, sh_c;
     dev_buf<int2> ctr;
     dev_buf<int4> hit;
-    dev_buf<int32_t> queue0, queue1, shadow_queue;
+    dev_buf<int32_t> queue0, queue1, shadow_queue, finish_queue;
     dev_buf<int2> spill;
     pt::paths_dev paths{};
     pt_tuning tuning{128, 256, 16, 8, 0, 0, 0};
@@ -597,21 +617,23 @@ struct pt_renderer {
             const int shade_grid = int(std::min<size_t>((pixels + size_t(tuning.shade_block) - 1) / size_t(tuning.shade_block),
                                                         size_t(cus) * 16));
             const int interval = std::max(1, tuning.check_interval);
-            if(tuning.time_kernels && kernel_events.size() < size_t(4 * interval)) {
+            if(tuning.time_kernels && kernel_events.size() < size_t(5 * interval)) {
                 for(auto e : kernel_events) (void)hipEventDestroy(e);
-                kernel_events.assign(size_t(4 * interval), nullptr);
+                kernel_events.assign(size_t(5 * interval), nullptr);
                 for(auto& e : kernel_events) HIP_CHECK(hipEventCreate(&e));
             }
             for(;;) {
                 for(int k = 0; k < interval; k++) {
-                    if(tuning.time_kernels) HIP_CHECK(hipEventRecord(kernel_events[size_t(4 * k)], stream));
+                    if(tuning.time_kernels) HIP_CHECK(hipEventRecord(kernel_events[size_t(5 * k)], stream));
                     pt::launch_trace(scene->view, frame, paths, ctl.get(), parity, trace_grid, trace_threads,
                                      tuning.count_tests != 0, stream);
-                    if(tuning.time_kernels) HIP_CHECK(hipEventRecord(kernel_events[size_t(4 * k + 1)], stream));
+                    if(tuning.time_kernels) HIP_CHECK(hipEventRecord(kernel_events[size_t(5 * k + 1)], stream));
                     pt::launch_bookkeep(ctl.get(), parity, stream);
-                    if(tuning.time_kernels) HIP_CHECK(hipEventRecord(kernel_events[size_t(4 * k + 2)], stream));
+                    if(tuning.time_kernels) HIP_CHECK(hipEventRecord(kernel_events[size_t(5 * k + 2)], stream));
                     pt::launch_shade(scene->view, frame, paths, ctl.get(), parity, std::max(1, shade_grid), tuning.shade_block, stream);
-                    if(tuning.time_kernels) HIP_CHECK(hipEventRecord(kernel_events[size_t(4 * k + 3)], stream));
+                    if(tuning.time_kernels) HIP_CHECK(hipEventRecord(kernel_events[size_t(5 * k + 3)], stream));
+                    pt::launch_regen(frame, paths, ctl.get(), parity, std::max(1, shade_grid), tuning.shade_block, stream);
+                    if(tuning.time_kernels) HIP_CHECK(hipEventRecord(kernel_events[size_t(5 * k + 4)], stream));
                     parity ^= 1;
                     st.trace_launches++;
                 }
@@ -619,10 +641,11 @@ struct pt_renderer {
                 read_ctl();
                 if(tuning.time_kernels) {
                     for(int k = 0; k < interval; k++) {
-                        float a = 0, b = 0;
-                        HIP_CHECK(hipEventElapsedTime(&a, kernel_events[size_t(4 * k)], kernel_events[size_t(4 * k + 1)]));
-                        HIP_CHECK(hipEventElapsedTime(&b, kernel_events[size_t(4 * k + 2)], kernel_events[size_t(4 * k + 3)]));
-                        st.trace_ms += a; st.shade_ms += b;
+                        float a = 0, b = 0, c = 0;
+                        HIP_CHECK(hipEventElapsedTime(&a, kernel_events[size_t(5 * k)], kernel_events[size_t(5 * k + 1)]));
+                        HIP_CHECK(hipEventElapsedTime(&b, kernel_events[size_t(5 * k + 2)], kernel_events[size_t(5 * k + 3)]));
+                        HIP_CHECK(hipEventElapsedTime(&c, kernel_events[size_t(5 * k + 3)], kernel_events[size_t(5 * k + 4)]));
+                        st.trace_ms += a; st.shade_ms += b; st.regen_ms += c;
                     }
                 }
                 if(ctl_host->n_ext[parity] == 0 || ctl_host->failure) break;
@@ -662,13 +685,14 @@ void allocate_state(pt_renderer& r, bool wavefront) {
     r.ray_o.alloc(n); r.ray_d.alloc(n); r.thr.alloc(n); r.rad.alloc(n);
     r.sh_o.alloc(n); r.sh_d.alloc(n); r.sh_c.alloc(n);
     r.ctr.alloc(n); r.hit.alloc(n);
-    r.queue0.alloc(n); r.queue1.alloc(n); r.shadow_queue.alloc(n);
+    r.queue0.alloc(n); r.queue1.alloc(n); r.shadow_queue.alloc(n); r.finish_queue.alloc(n);
     auto& p = r.paths;
     p.ray_o = r.ray_o.get(); p.ray_d = r.ray_d.get(); p.thr = r.thr.get(); p.rad = r.rad.get();
     p.ctr = r.ctr.get(); p.hit = r.hit.get();
     p.sh_o = r.sh_o.get(); p.sh_d = r.sh_d.get(); p.sh_c = r.sh_c.get();
     p.rng = r.rng.get(); p.sum = r.sum.get();
     p.queue[0] = r.queue0.get(); p.queue[1] = r.queue1.get(); p.shadow_queue = r.shadow_queue.get();
+    p.finish_queue = r.finish_queue.get();
 }
 
 void check_grid(const pt_renderer& r, unsigned bx, unsigned by) {
@@ -883,7 +907,11 @@ pt_status pt_renderer_set_tuning(pt_renderer *renderer, const pt_tuning *tuning)
         pt_tuning t = renderer->tuning;
         auto block_ok = [](int32_t b) { return b >= 64 && b <= 256 && b % 64 == 0; };
         if(tuning->trace_block) { if(!block_ok(tuning->trace_block)) fail(PT_EINVAL, "trace block must be 64..256, multiple of 64"); t.trace_block = tuning->trace_block; }
-        if(tuning->shade_block) { if(!block_ok(tuning->shade_block)) fail(PT_EINVAL, "shade block must be 64..256, multiple of 64"); t.shade_block = tuning->shade_block; }
+        if(tuning->shade_block) {
+            if(!(tuning->shade_block >= 64 && tuning->shade_block <= 1024 && tuning->shade_block % 64 == 0))
+                fail(PT_EINVAL, "shade block must be 64..1024, multiple of 64");
+            t.shade_block = tuning->shade_block;
+        }
         if(tuning->blocks_per_cu) { if(tuning->blocks_per_cu < 1 || tuning->blocks_per_cu > 64) fail(PT_EINVAL, "blocks per CU must be 1..64"); t.blocks_per_cu = tuning->blocks_per_cu; }
         if(tuning->check_interval) { if(tuning->check_interval < 1 || tuning->check_interval > 4096) fail(PT_EINVAL, "check interval must be 1..4096"); t.check_interval = tuning->check_interval; }
         t.time_kernels = tuning->time_kernels ? 1 : 0;

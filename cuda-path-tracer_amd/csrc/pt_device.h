@@ -26,6 +26,7 @@ struct scene_dev {
     const float4 *bnodes;      // 2 per branch: {lo.x,lo.y,lo.z,hi.x} {hi.y,hi.z,left_ref,right_ref}
     const float4 *tri_isect;   // 3 per triangle: {v0, 1e-7f*|e1 x e2|} {e1, |e1 x e2|} {e2, 0}
     const float4 *sph_isect;   // 1 per sphere: {center, radius}
+    const float4 *shade_tri;   // 5 per triangle: {outward unit normal, material} {n0, smooth} {n1, uv0.x} {n2, uv0.y} {uv1.xy, uv2.xy}
     int32_t root_ref;          // -2^31 when the scene is empty
     // Original transfer arrays (shading, brute mode).
     const pt_flat_triangle *triangles;
@@ -78,6 +79,7 @@ struct paths_dev {
     float *sum;         // 3 floats per pixel (vec3 sum[], cuda_renderer.cu:145)
     int32_t *queue[2];  // extension queues (pixel ids), ping-pong
     int32_t *shadow_queue;
+    int32_t *finish_queue; // paths whose sample ended this bounce
     int2 *spill;        // wide traversal stack entries beyond wide_lds_entries
 };
 
@@ -85,6 +87,7 @@ struct paths_dev {
 struct loop_ctl {
     uint32_t n_ext[2];
     uint32_t n_shadow;
+    uint32_t n_finish;
     uint32_t work;           // persistent-fetch cursor of the trace kernel
     unsigned long long rays; // trace_world calls, shadow queries included
     unsigned long long ext_rays, shadow_rays;
@@ -118,6 +121,8 @@ void launch_trace(const scene_dev& s, const frame_params& f, const paths_dev& p,
 void launch_bookkeep(loop_ctl *ctl, int parity, hipStream_t stream);
 void launch_shade(const scene_dev& s, const frame_params& f, const paths_dev& p, loop_ctl *ctl, int parity,
                   int grid, int block, hipStream_t stream);
+void launch_regen(const frame_params& f, const paths_dev& p, loop_ctl *ctl, int parity, int grid, int block,
+                  hipStream_t stream);
 void launch_pack(const float *sum, uint8_t *rgba, int64_t count, int32_t samples, hipStream_t stream);
 void launch_add_rays(loop_ctl *ctl, unsigned long long rays, hipStream_t stream);
 

@@ -393,7 +393,7 @@ struct hit_rec { float t, u, v; v3 p, normal; int32_t material_id, primitive_typ
 
 // src/geometry/sphere_uv.h:7-13.  atan2/asin are evaluated in double and
 // rounded, so u,v can differ from a float libm's result in the last bit.
-__device__ __forceinline__ void sphere_uv(v3 p, float& u, float& v) {
+__device__ __noinline__ void sphere_uv(v3 p, float& u, float& v) {
     float phi = float(atan2(double(p.z), double(p.x)));
     float theta = float(asin(double(p.y)));
     u = float(1 - (double(phi) + M_PI) / (2 * M_PI));
@@ -414,15 +414,18 @@ __device__ hit_rec make_hit(const scene_dev& s, v3 o, v3 d, const best_hit& b) {
         h.material_id = sp.material_id;
         h.primitive_type = PT_SPHERE;
     } else {
-        const pt_flat_triangle& tr = s.triangles[prim];
-        float4 B = s.tri_isect[3 * prim + 1], C = s.tri_isect[3 * prim + 2];
-        v3 outward = cross(mk(B.x, B.y, B.z), mk(C.x, C.y, C.z));
-        outward = div_assign(outward, B.w);
+        // Shading record: {unit outward normal, material} {n0, smooth}
+        // {n1, uv0.x} {n2, uv0.y} {uv1.xy, uv2.xy}; the outward normal is
+        // cross(e1,e2) /= |e1 x e2| precomputed with the same arithmetic.
+        const float4 *sr = s.shade_tri + 5 * size_t(prim);
+        const float4 G = sr[0], N0 = sr[1], UV = sr[4];
+        const v3 outward = mk(G.x, G.y, G.z);
         v3 normal = outward;
-        float u = b.u, v = b.v;
-        if(tr.smooth) {
+        const float u = b.u, v = b.v;
+        if(__float_as_int(N0.w)) {
+            const float4 N1 = sr[2], N2 = sr[3];
             float w = 1 - u - v;
-            v3 interp = add(add(scale(w, mk(tr.n0)), scale(u, mk(tr.n1))), scale(v, mk(tr.n2)));
+            v3 interp = add(add(scale(w, mk(N0.x, N0.y, N0.z)), scale(u, mk(N1.x, N1.y, N1.z))), scale(v, mk(N2.x, N2.y, N2.z)));
             float ls = squared_length(interp);
             if(ls > 0 && isfinite(ls)) {
                 normal = unit_vector(interp);
@@ -431,10 +434,11 @@ __device__ hit_rec make_hit(const scene_dev& s, v3 o, v3 d, const best_hit& b) {
             }
         }
         h.normal = normal;
-        float w = 1 - u - v;
-        v3 uv = add(add(scale(w, mk(tr.uv0)), scale(u, mk(tr.uv1))), scale(v, mk(tr.uv2)));
-        h.u = uv.x; h.v = uv.y;
-        h.material_id = tr.material_id;
+        const float w = 1 - u - v;
+        const float uv0x = sr[2].w, uv0y = sr[3].w;
+        h.u = w * uv0x + u * UV.x + v * UV.z;
+        h.v = w * uv0y + u * UV.y + v * UV.w;
+        h.material_id = __float_as_int(G.w);
         h.primitive_type = PT_TRIANGLE;
     }
     return h;
@@ -543,6 +547,12 @@ __device__ __forceinline__ float hit_light_pdf(const scene_dev& s, v3 from, cons
 // ---------------- BSDF sampling (src/renderer/transport.h:113-155) ----------------
 __device__ __forceinline__ v3 reflect(v3 v, v3 n) { return sub(v, scale(2 * dot(v, n), n)); }
 
+// r0 + (1-r0)*pow(1-cosine, 5) in double, rounded to float; out of line so
+// its double-precision registers do not count against every shading path.
+__device__ __noinline__ float schlick_double(float r0, float cosine) {
+    return float(double(r0) + double(1 - r0) * pow(double(1 - cosine), 5.0));
+}
+
 __device__ bool scatter(const scene_dev& s, const pt_flat_material& mat, v3 in_d, const hit_rec& rec, rng_t& r,
                         v3& attenuation, v3& out_d) {
     if(mat.type == PT_DIFFUSE) {
@@ -577,7 +587,7 @@ __device__ bool scatter(const scene_dev& s, const pt_flat_material& mat, v3 in_d
             float r0 = (1 - mat.ref_idx) / (1 + mat.ref_idx);
             r0 *= r0;
             // The host build promotes pow(float,int) to double (transport.h:147).
-            probability = float(double(r0) + double(1 - r0) * pow(double(1 - cosine), 5.0));
+            probability = schlick_double(r0, cosine);
         }
         out_d = r.uniform() < probability ? reflected : refracted;
         attenuation = mk(1, 1, 1);
@@ -608,6 +618,41 @@ __device__ __forceinline__ float cost_log2(const scene_dev& s, int count) {
     // log2f(1.0f + float(count)) of the host libm, from the per-scene table.
     if(count >= 0 && count < s.log2_count) return s.log2_table[count];
     return float(log2(double(1.0f + float(count))));
+}
+
+// Block-aggregated append to up to three queues: one atomic per queue per
+// block (a single counter shared by the whole grid is the contended
+// resource), order kept within the block.  Every thread of the block must
+// call it; it synchronises the block.
+constexpr int max_block_waves = 16;
+constexpr int trace_chunk = 64;  // items per work claim of a trace wave (larger claims starve small launches)
+__device__ __forceinline__ void block_append3(bool p0, bool p1, bool p2, int32_t value,
+                                              int32_t *q0, uint32_t *c0, int32_t *q1, uint32_t *c1,
+                                              int32_t *q2, uint32_t *c2) {
+    __shared__ uint32_t counts[3][max_block_waves];
+    __shared__ uint32_t bases[3][max_block_waves];
+    const int lane = __lane_id(), wave = int(threadIdx.x) >> 6, waves = int(blockDim.x) >> 6;
+    const uint64_t m0 = __ballot(p0), m1 = __ballot(p1), m2 = __ballot(p2);
+    if(lane == 0) {
+        counts[0][wave] = uint32_t(__popcll(m0));
+        counts[1][wave] = uint32_t(__popcll(m1));
+        counts[2][wave] = uint32_t(__popcll(m2));
+    }
+    __syncthreads();
+    if(threadIdx.x < 3 && (threadIdx.x != 2 || q2)) {
+        const int q = int(threadIdx.x);
+        uint32_t total = 0;
+        for(int w = 0; w < waves; w++) total += counts[q][w];
+        uint32_t *counter = q == 0 ? c0 : q == 1 ? c1 : c2;
+        uint32_t running = total ? atomicAdd(counter, total) : 0;
+        for(int w = 0; w < waves; w++) { bases[q][w] = running; running += counts[q][w]; }
+    }
+    __syncthreads();
+    const uint64_t below = (1ull << lane) - 1ull;
+    if(p0) q0[bases[0][wave] + uint32_t(__popcll(m0 & below))] = value;
+    if(p1) q1[bases[1][wave] + uint32_t(__popcll(m1 & below))] = value;
+    if(p2) q2[bases[2][wave] + uint32_t(__popcll(m2 & below))] = value;
+    __syncthreads();
 }
 
 // Wave-aggregated append: one atomic per wave, order kept within the wave.
@@ -706,7 +751,7 @@ __global__ void start_kernel(scene_dev s, frame_params f, paths_dev p, loop_ctl 
     int64_t n = int64_t(f.width) * f.height;
     int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if(i == 0) {
-        ctl->n_ext[0] = uint32_t(n); ctl->n_ext[1] = 0; ctl->n_shadow = 0; ctl->work = 0;
+        ctl->n_ext[0] = uint32_t(n); ctl->n_ext[1] = 0; ctl->n_shadow = 0; ctl->n_finish = 0; ctl->work = 0;
         ctl->iterations = 0; ctl->ext_rays = 0; ctl->shadow_rays = 0;
         ctl->box_tests = 0; ctl->triangle_tests = 0; ctl->sphere_tests = 0;
     }
@@ -742,11 +787,20 @@ __global__ __launch_bounds__(256) void trace_kernel(scene_dev s, frame_params f,
                     size_t(gridDim.x) * blockDim.x, size_t(blockIdx.x) * blockDim.x + threadIdx.x};
     int error = 0;
     unsigned long long n_box = 0, n_leaf = 0, n_sph = 0;
+    // Each wave claims trace_chunk consecutive items per atomic and works
+    // through them 64 at a time.
+    uint32_t chunk_base = 0, chunk_left = 0;
     for(;;) {
-        uint32_t base = 0;
-        if(lane == 0) base = atomicAdd(&ctl->work, 64u);
-        base = __shfl(base, 0);
-        if(base >= total) break;
+        if(chunk_left == 0) {
+            uint32_t b = 0;
+            if(lane == 0) b = atomicAdd(&ctl->work, uint32_t(trace_chunk));
+            chunk_base = __shfl(b, 0);
+            chunk_left = trace_chunk;
+            if(chunk_base >= total) break;
+        }
+        const uint32_t base = chunk_base;
+        chunk_base += 64; chunk_left -= 64;
+        if(base >= total) { chunk_left = 0; continue; }
         uint32_t item = base + uint32_t(lane);
         int boxes = 0, prims = 0, spheres = 0;
         if(item < n_ext) {
@@ -793,6 +847,7 @@ __global__ void bookkeep_kernel(loop_ctl *ctl, int parity) {
     ctl->rays += (unsigned long long)e + sh;
     ctl->ext_rays += e; ctl->shadow_rays += sh;
     ctl->n_shadow = 0;
+    ctl->n_finish = 0;
     ctl->n_ext[parity ^ 1] = 0;
     ctl->work = 0;
     if(e) ctl->iterations++;
@@ -801,14 +856,14 @@ __global__ void bookkeep_kernel(loop_ctl *ctl, int parity) {
 __global__ void add_rays_kernel(loop_ctl *ctl, unsigned long long rays) { ctl->rays += rays; }
 
 // One bounce of trace_color (transport.h:169-230) for every path in the queue.
-__global__ __launch_bounds__(256) void shade_kernel(scene_dev s, frame_params f, paths_dev p, loop_ctl *ctl, int parity) {
+__global__ __launch_bounds__(1024) void shade_kernel(scene_dev s, frame_params f, paths_dev p, loop_ctl *ctl, int parity) {
     const uint32_t n = ctl->n_ext[parity];
     const int32_t *q = p.queue[parity];
     int32_t *qn = p.queue[parity ^ 1];
     const uint32_t step = gridDim.x * blockDim.x;
     for(uint32_t base = blockIdx.x * blockDim.x; base < n; base += step) {
         uint32_t item = base + threadIdx.x;
-        bool cont = false, shadow = false;
+        bool cont = false, shadow = false, done = false;
         int32_t pix = 0;
         if(item < n) {
             pix = q[item];
@@ -877,18 +932,6 @@ __global__ __launch_bounds__(256) void shade_kernel(scene_dev s, frame_params f,
                     }
                 }
             }
-            if(finished) {
-                // accumulated += trace_pixel_sample(...) (cuda_renderer.cu:81)
-                float *sp = p.sum + 3 * size_t(pix);
-                sp[0] = sp[0] + radiance.x; sp[1] = sp[1] + radiance.y; sp[2] = sp[2] + radiance.z;
-                left--;
-                if(left > 0) {
-                    int x = int(pix % f.width), y = int(pix / f.width);
-                    camera_ray(f.camera, x, y, f.width, f.height, r, next_o, next_d, time);
-                    throughput = mk(1, 1, 1); radiance = mk(0, 0, 0); previous_pdf = 0; depth = 0;
-                    cont = true;
-                }
-            }
             p.rng[pix] = r.state;
             if(cont) {
                 p.ray_o[pix] = make_float4(next_o.x, next_o.y, next_o.z, time);
@@ -896,10 +939,47 @@ __global__ __launch_bounds__(256) void shade_kernel(scene_dev s, frame_params f,
                 p.thr[pix] = make_float4(throughput.x, throughput.y, throughput.z, previous_pdf);
                 p.rad[pix] = make_float4(radiance.x, radiance.y, radiance.z, 0);
                 p.ctr[pix] = make_int2(depth, left);
+            } else {
+                p.rad[pix] = make_float4(radiance.x, radiance.y, radiance.z, 0);
             }
+            done = finished;
         }
-        append(cont, pix, qn, &ctl->n_ext[parity ^ 1]);
-        append(shadow, pix, p.shadow_queue, &ctl->n_shadow);
+        block_append3(cont, shadow, done, pix, qn, &ctl->n_ext[parity ^ 1], p.shadow_queue, &ctl->n_shadow,
+                      p.finish_queue, &ctl->n_finish);
+    }
+}
+
+// End of a sample (cuda_renderer.cu:81): accumulated += radiance in sample
+// order, then the pixel's next camera ray while samples remain.
+__global__ __launch_bounds__(1024) void regen_kernel(frame_params f, paths_dev p, loop_ctl *ctl, int parity) {
+    const uint32_t n = ctl->n_finish;
+    int32_t *qn = p.queue[parity ^ 1];
+    const uint32_t step = gridDim.x * blockDim.x;
+    for(uint32_t base = blockIdx.x * blockDim.x; base < n; base += step) {
+        uint32_t item = base + threadIdx.x;
+        bool cont = false;
+        int32_t pix = 0;
+        if(item < n) {
+            pix = p.finish_queue[item];
+            float4 ra = p.rad[pix];
+            float *sp = p.sum + 3 * size_t(pix);
+            sp[0] = sp[0] + ra.x; sp[1] = sp[1] + ra.y; sp[2] = sp[2] + ra.z;
+            int left = p.ctr[pix].y - 1;
+            if(left > 0) {
+                rng_t r = pixel_rng(p.rng[pix], uint64_t(pix));
+                int x = int(pix % f.width), y = int(pix / f.width);
+                v3 o, d; float time;
+                camera_ray(f.camera, x, y, f.width, f.height, r, o, d, time);
+                p.rng[pix] = r.state;
+                p.ray_o[pix] = make_float4(o.x, o.y, o.z, time);
+                p.ray_d[pix] = make_float4(d.x, d.y, d.z, 0);
+                p.thr[pix] = make_float4(1, 1, 1, 0);
+                p.rad[pix] = make_float4(0, 0, 0, 0);
+                cont = true;
+            }
+            p.ctr[pix] = make_int2(0, left);
+        }
+        block_append3(cont, false, false, pix, qn, &ctl->n_ext[parity ^ 1], nullptr, nullptr, nullptr, nullptr);
     }
 }
 
@@ -957,6 +1037,10 @@ void launch_bookkeep(loop_ctl *ctl, int parity, hipStream_t stream) {
 void launch_shade(const scene_dev& s, const frame_params& f, const paths_dev& p, loop_ctl *ctl, int parity,
                   int grid, int block, hipStream_t stream) {
     hipLaunchKernelGGL(shade_kernel, dim3(unsigned(grid)), dim3(unsigned(block)), 0, stream, s, f, p, ctl, parity);
+}
+void launch_regen(const frame_params& f, const paths_dev& p, loop_ctl *ctl, int parity, int grid, int block,
+                  hipStream_t stream) {
+    hipLaunchKernelGGL(regen_kernel, dim3(unsigned(grid)), dim3(unsigned(block)), 0, stream, f, p, ctl, parity);
 }
 void launch_pack(const float *sum, uint8_t *rgba, int64_t count, int32_t samples, hipStream_t stream) {
     unsigned grid = unsigned((count + 255) / 256);

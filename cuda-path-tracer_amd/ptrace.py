@@ -97,6 +97,7 @@ class _Settings(ctypes.Structure):
 class LaunchStats(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int32), ("trace_launches", ctypes.c_int32), ("trace_ms", ctypes.c_float),
                 ("shade_ms", ctypes.c_float), ("extend_rays", ctypes.c_uint64), ("shadow_rays", ctypes.c_uint64),
+                ("regen_ms", ctypes.c_float),
                 ("box_tests", ctypes.c_uint64), ("triangle_tests", ctypes.c_uint64), ("sphere_tests", ctypes.c_uint64)]
 
 

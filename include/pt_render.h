@@ -160,6 +160,7 @@ typedef struct pt_launch_stats {
     float shade_ms;
     uint64_t extend_rays;
     uint64_t shadow_rays;
+    float regen_ms;          /* end-of-sample accumulation and camera-ray regeneration */
     uint64_t box_tests;      /* filled when pt_tuning.count_tests is set */
     uint64_t triangle_tests;
     uint64_t sphere_tests;
