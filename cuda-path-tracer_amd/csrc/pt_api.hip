@@ -555,8 +555,10 @@ struct pt_renderer {
     dev_buf<int4> hit;
     dev_buf<int32_t> queue0, queue1, shadow_queue, finish_queue;
     dev_buf<int2> spill;
+    dev_buf<uint32_t> iteration_log;
+    static constexpr uint32_t log_capacity = 1u << 16;
     pt::paths_dev paths{};
-    pt_tuning tuning{128, 256, 16, 8, 0, 0, 0};
+    pt_tuning tuning{128, 256, 16, 8, 0, 0, 0, 262144};
     int cus = 1;
     pt_launch_stats last{};
 
@@ -611,7 +613,10 @@ struct pt_renderer {
             int parity = 0;
             const int trace_threads = trace_block();
             const int trace_grid = cus * std::max(1, tuning.blocks_per_cu);
-            size_t spill_need = size_t(trace_grid) * size_t(trace_threads) *
+            const int drain_block = 256;
+            const size_t drain_threads = tuning.drain_threshold > 0
+                ? (size_t(tuning.drain_threshold) + drain_block - 1) / drain_block * drain_block : 0;
+            size_t spill_need = std::max(size_t(trace_grid) * size_t(trace_threads), drain_threads) *
                                 size_t(pt::wide_spill_entries(scene->view.wide_stack_capacity));
             if(spill.size() < spill_need) { spill.alloc(spill_need); paths.spill = spill.get(); }
             const int shade_grid = int(std::min<size_t>((pixels + size_t(tuning.shade_block) - 1) / size_t(tuning.shade_block),
@@ -649,6 +654,19 @@ struct pt_renderer {
                     }
                 }
                 if(ctl_host->n_ext[parity] == 0 || ctl_host->failure) break;
+                if(tuning.drain_threshold > 0 && ctl_host->n_ext[parity] <= uint32_t(tuning.drain_threshold)) {
+                    hipEvent_t d0 = kernel_events.empty() ? nullptr : kernel_events[0];
+                    hipEvent_t d1 = kernel_events.size() > 1 ? kernel_events[1] : nullptr;
+                    if(tuning.time_kernels) HIP_CHECK(hipEventRecord(d0, stream));
+                    pt::launch_drain(scene->view, frame, paths, ctl.get(), parity, ctl_host->n_shadow,
+                                     ctl_host->n_ext[parity], drain_block, stream);
+                    if(tuning.time_kernels) HIP_CHECK(hipEventRecord(d1, stream));
+                    HIP_CHECK(hipGetLastError());
+                    st.drained_paths = ctl_host->n_ext[parity];
+                    read_ctl();
+                    if(tuning.time_kernels) HIP_CHECK(hipEventElapsedTime(&st.drain_ms, d0, d1));
+                    break;
+                }
             }
             HIP_CHECK(hipEventRecord(stop, stream));
             HIP_CHECK(hipEventSynchronize(stop));
@@ -682,6 +700,11 @@ void allocate_state(pt_renderer& r, bool wavefront) {
     HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&r.ctl_host), sizeof(pt::loop_ctl), hipHostMallocDefault));
     std::memset(r.ctl_host, 0, sizeof(pt::loop_ctl));
     if(!wavefront) return;
+    r.iteration_log.alloc(2 * size_t(pt_renderer::log_capacity));
+    pt::loop_ctl init{};
+    init.log = r.iteration_log.get();
+    init.log_capacity = pt_renderer::log_capacity;
+    HIP_CHECK(hipMemcpy(r.ctl.get(), &init, sizeof init, hipMemcpyHostToDevice));
     r.ray_o.alloc(n); r.ray_d.alloc(n); r.thr.alloc(n); r.rad.alloc(n);
     r.sh_o.alloc(n); r.sh_d.alloc(n); r.sh_c.alloc(n);
     r.ctr.alloc(n); r.hit.alloc(n);
@@ -889,6 +912,8 @@ pt_status pt_renderer_write_state(pt_renderer *renderer, const float *sum, const
         HIP_CHECK(hipMemcpy(renderer->rng.get(), rng_state, pixels * sizeof(uint64_t), hipMemcpyHostToDevice));
         pt::loop_ctl c{};
         c.rays = rays;
+        c.log = renderer->iteration_log.get();
+        c.log_capacity = renderer->iteration_log.size() ? pt_renderer::log_capacity : 0;
         HIP_CHECK(hipMemcpy(renderer->ctl.get(), &c, sizeof(c), hipMemcpyHostToDevice));
         renderer->samples = samples; renderer->rays = rays;
     });
@@ -898,6 +923,24 @@ pt_status pt_renderer_last_launch_stats(const pt_renderer *renderer, pt_launch_s
     return guarded([&] {
         if(!renderer || !stats) fail(PT_EINVAL, "null argument");
         *stats = renderer->last;
+    });
+}
+
+pt_status pt_renderer_iteration_log(const pt_renderer *renderer, uint32_t *extension, uint32_t *shadow,
+                                    size_t capacity, int32_t *count) {
+    return guarded([&] {
+        if(!renderer || !count) fail(PT_EINVAL, "null argument");
+        int32_t n = renderer->last.iterations;
+        *count = n;
+        if(renderer->frame.debug != PT_DEBUG_SHADED || !renderer->iteration_log.size()) { *count = 0; return; }
+        size_t m = std::min<size_t>({capacity, size_t(n), size_t(pt_renderer::log_capacity)});
+        if(!m) return;
+        std::vector<uint32_t> buf(2 * m);
+        HIP_CHECK(hipMemcpy(buf.data(), renderer->iteration_log.get(), buf.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        for(size_t i = 0; i < m; i++) {
+            if(extension) extension[i] = buf[2 * i];
+            if(shadow) shadow[i] = buf[2 * i + 1];
+        }
     });
 }
 
@@ -917,6 +960,7 @@ pt_status pt_renderer_set_tuning(pt_renderer *renderer, const pt_tuning *tuning)
         t.time_kernels = tuning->time_kernels ? 1 : 0;
         t.count_tests = tuning->count_tests ? 1 : 0;
         t.exact_traversal = tuning->exact_traversal ? 1 : 0;
+        if(tuning->drain_threshold) t.drain_threshold = tuning->drain_threshold < 0 ? -1 : tuning->drain_threshold;
         renderer->tuning = t;
     });
 }

@@ -93,6 +93,8 @@ struct loop_ctl {
     unsigned long long ext_rays, shadow_rays;
     uint32_t iterations;
     int32_t failure;         // traversal stack overflow
+    uint32_t *log;           // per-iteration {extension, shadow} counts, log_capacity pairs
+    uint32_t log_capacity;
     // Test counts of a counting pass (trace_stats of every query, split by
     // primitive kind): box tests, triangle tests, sphere tests.
     unsigned long long box_tests, triangle_tests, sphere_tests;
@@ -123,6 +125,10 @@ void launch_shade(const scene_dev& s, const frame_params& f, const paths_dev& p,
                   int grid, int block, hipStream_t stream);
 void launch_regen(const frame_params& f, const paths_dev& p, loop_ctl *ctl, int parity, int grid, int block,
                   hipStream_t stream);
+// Finishes the pending shadow queries and then every queued path in one
+// thread each; `shadows`/`paths` are the host's copies of the queue sizes.
+void launch_drain(const scene_dev& s, const frame_params& f, const paths_dev& p, loop_ctl *ctl, int parity,
+                  uint32_t shadows, uint32_t paths, int block, hipStream_t stream);
 void launch_pack(const float *sum, uint8_t *rgba, int64_t count, int32_t samples, hipStream_t stream);
 void launch_add_rays(loop_ctl *ctl, unsigned long long rays, hipStream_t stream);
 

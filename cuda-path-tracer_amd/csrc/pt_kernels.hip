@@ -844,6 +844,10 @@ __global__ __launch_bounds__(256) void trace_kernel(scene_dev s, frame_params f,
 
 __global__ void bookkeep_kernel(loop_ctl *ctl, int parity) {
     uint32_t e = ctl->n_ext[parity], sh = ctl->n_shadow;
+    if(e && ctl->log && ctl->iterations < ctl->log_capacity) {
+        ctl->log[2 * ctl->iterations] = e;
+        ctl->log[2 * ctl->iterations + 1] = sh;
+    }
     ctl->rays += (unsigned long long)e + sh;
     ctl->ext_rays += e; ctl->shadow_rays += sh;
     ctl->n_shadow = 0;
@@ -855,7 +859,90 @@ __global__ void bookkeep_kernel(loop_ctl *ctl, int parity) {
 
 __global__ void add_rays_kernel(loop_ctl *ctl, unsigned long long rays) { ctl->rays += rays; }
 
-// One bounce of trace_color (transport.h:169-230) for every path in the queue.
+// State of one path between bounces (the locals of trace_color,
+// transport.h:171-173, plus the current ray and the samples still to take).
+struct path_state {
+    v3 o, d;
+    float time;
+    v3 throughput, radiance;
+    float previous_pdf;
+    int depth, left;
+};
+struct shadow_query { v3 o, d; float t_max; v3 contribution; };
+
+__device__ __forceinline__ path_state load_path(const paths_dev& p, int32_t pix) {
+    const float4 ro = p.ray_o[pix], rd = p.ray_d[pix], th = p.thr[pix], ra = p.rad[pix];
+    const int2 ct = p.ctr[pix];
+    path_state st;
+    st.o = mk(ro.x, ro.y, ro.z); st.d = mk(rd.x, rd.y, rd.z); st.time = ro.w;
+    st.throughput = mk(th.x, th.y, th.z); st.previous_pdf = th.w;
+    st.radiance = mk(ra.x, ra.y, ra.z);
+    st.depth = ct.x; st.left = ct.y;
+    return st;
+}
+__device__ __forceinline__ void store_path(const paths_dev& p, int32_t pix, const path_state& st) {
+    p.ray_o[pix] = make_float4(st.o.x, st.o.y, st.o.z, st.time);
+    p.ray_d[pix] = make_float4(st.d.x, st.d.y, st.d.z, 0);
+    p.thr[pix] = make_float4(st.throughput.x, st.throughput.y, st.throughput.z, st.previous_pdf);
+    p.rad[pix] = make_float4(st.radiance.x, st.radiance.y, st.radiance.z, 0);
+    p.ctr[pix] = make_int2(st.depth, st.left);
+}
+
+// One iteration of trace_color's loop after its closest-hit query
+// (transport.h:184-227).  Returns true when the path continues with st.o/st.d
+// as the next ray, false when the sample has ended with st.radiance.  A next
+// event estimation query is returned in `sq` (has_shadow); its contribution
+// must be added to st.radiance if unblocked, before the next bounce.
+__device__ bool shade_bounce(const scene_dev& s, const frame_params& f, path_state& st, bool hit, const best_hit& b,
+                             rng_t& r, shadow_query& sq, bool& has_shadow) {
+    has_shadow = false;
+    if(!hit) return false;
+    hit_rec rec = make_hit(s, st.o, st.d, b);
+    const pt_flat_material mat = s.materials[rec.material_id];
+    if(mat.type == PT_LIGHT) {
+        float weight = st.previous_pdf > 0 ? mis_weight(st.previous_pdf, hit_light_pdf(s, st.o, rec)) : 1;
+        st.radiance = add(st.radiance, mul(scale(weight, st.throughput), trace_texture(s, mat.texture_id, rec.u, rec.v)));
+        return false;
+    }
+    if(st.depth == f.max_depth) return false;
+    if(f.direct_light && mat.type == PT_DIFFUSE) {
+        v3 attenuation = trace_texture(s, mat.texture_id, rec.u, rec.v);
+        if(s.light_count) {
+            light_sample light = sample_light(s, r);
+            v3 delta = sub(light.point, rec.p);
+            float distance = length(delta);
+            if(distance > 0.002f && isfinite(distance)) {
+                v3 direction = divs(delta, distance);
+                float cosine = fmaxf(0, dot(rec.normal, direction));
+                float pdf = light_direction_pdf(s.light_area, rec.p, light.point, light.normal);
+                if(cosine > 0 && pdf > 0) {
+                    const pt_flat_material emitter = s.materials[light.material];
+                    v3 emission = trace_texture(s, emitter.texture_id, light.u, light.v);
+                    float bsdf_pdf = cosine / PI_F;
+                    float w = bsdf_pdf * mis_weight(pdf, bsdf_pdf) / pdf;
+                    sq.o = rec.p; sq.d = direction; sq.t_max = distance - 0.001f;
+                    sq.contribution = scale(w, mul(mul(st.throughput, attenuation), emission));
+                    has_shadow = true;
+                }
+            }
+        }
+        v3 direction = cosine_direction(s, rec.normal, r);
+        st.previous_pdf = fmaxf(0, dot(rec.normal, direction)) / PI_F;
+        st.throughput = mul(st.throughput, attenuation);
+        st.o = rec.p; st.d = direction;
+        st.depth++;
+        return true;
+    }
+    st.previous_pdf = 0;
+    v3 attenuation, out_d;
+    if(!scatter(s, mat, st.d, rec, r, attenuation, out_d)) return false;
+    st.throughput = mul(st.throughput, attenuation);
+    st.o = rec.p; st.d = out_d;
+    st.depth++;
+    return true;
+}
+
+// One bounce for every path in the extension queue.
 __global__ __launch_bounds__(1024) void shade_kernel(scene_dev s, frame_params f, paths_dev p, loop_ctl *ctl, int parity) {
     const uint32_t n = ctl->n_ext[parity];
     const int32_t *q = p.queue[parity];
@@ -867,85 +954,119 @@ __global__ __launch_bounds__(1024) void shade_kernel(scene_dev s, frame_params f
         int32_t pix = 0;
         if(item < n) {
             pix = q[item];
-            int4 hb = p.hit[pix];
-            float4 ro = p.ray_o[pix], rd = p.ray_d[pix], th = p.thr[pix], ra = p.rad[pix];
-            int2 ct = p.ctr[pix];
+            const int4 hb = p.hit[pix];
+            path_state st = load_path(p, pix);
             rng_t r = pixel_rng(p.rng[pix], uint64_t(pix));
-            v3 o = mk(ro.x, ro.y, ro.z), d = mk(rd.x, rd.y, rd.z);
-            float time = ro.w;
-            v3 throughput = mk(th.x, th.y, th.z), radiance = mk(ra.x, ra.y, ra.z);
-            float previous_pdf = th.w;
-            int depth = ct.x, left = ct.y;
-            bool finished = false;
-            v3 next_o = o, next_d = d;
-            if(hb.w < 0) {
-                finished = true;
-            } else {
-                best_hit b{__int_as_float(hb.x), __int_as_float(hb.y), __int_as_float(hb.z), hb.w};
-                hit_rec rec = make_hit(s, o, d, b);
-                const pt_flat_material mat = s.materials[rec.material_id];
-                if(mat.type == PT_LIGHT) {
-                    float weight = previous_pdf > 0 ? mis_weight(previous_pdf, hit_light_pdf(s, o, rec)) : 1;
-                    radiance = add(radiance, mul(scale(weight, throughput), trace_texture(s, mat.texture_id, rec.u, rec.v)));
-                    finished = true;
-                } else if(depth == f.max_depth) {
-                    finished = true;
-                } else if(f.direct_light && mat.type == PT_DIFFUSE) {
-                    v3 attenuation = trace_texture(s, mat.texture_id, rec.u, rec.v);
-                    if(s.light_count) {
-                        light_sample light = sample_light(s, r);
-                        v3 delta = sub(light.point, rec.p);
-                        float distance = length(delta);
-                        if(distance > 0.002f && isfinite(distance)) {
-                            v3 direction = divs(delta, distance);
-                            float cosine = fmaxf(0, dot(rec.normal, direction));
-                            float pdf = light_direction_pdf(s.light_area, rec.p, light.point, light.normal);
-                            if(cosine > 0 && pdf > 0) {
-                                const pt_flat_material emitter = s.materials[light.material];
-                                v3 emission = trace_texture(s, emitter.texture_id, light.u, light.v);
-                                float bsdf_pdf = cosine / PI_F;
-                                float w = bsdf_pdf * mis_weight(pdf, bsdf_pdf) / pdf;
-                                v3 c = scale(w, mul(mul(throughput, attenuation), emission));
-                                p.sh_o[pix] = make_float4(rec.p.x, rec.p.y, rec.p.z, distance - 0.001f);
-                                p.sh_d[pix] = make_float4(direction.x, direction.y, direction.z, 0);
-                                p.sh_c[pix] = make_float4(c.x, c.y, c.z, 0);
-                                shadow = true;
-                            }
-                        }
-                    }
-                    v3 direction = cosine_direction(s, rec.normal, r);
-                    previous_pdf = fmaxf(0, dot(rec.normal, direction)) / PI_F;
-                    throughput = mul(throughput, attenuation);
-                    next_o = rec.p; next_d = direction;
-                    depth++;
-                    cont = true;
-                } else {
-                    previous_pdf = 0;
-                    v3 attenuation, out_d;
-                    if(!scatter(s, mat, d, rec, r, attenuation, out_d)) {
-                        finished = true;
-                    } else {
-                        throughput = mul(throughput, attenuation);
-                        next_o = rec.p; next_d = out_d;
-                        depth++;
-                        cont = true;
-                    }
-                }
+            const best_hit b{__int_as_float(hb.x), __int_as_float(hb.y), __int_as_float(hb.z), hb.w};
+            shadow_query sq;
+            cont = shade_bounce(s, f, st, hb.w >= 0, b, r, sq, shadow);
+            if(shadow) {
+                p.sh_o[pix] = make_float4(sq.o.x, sq.o.y, sq.o.z, sq.t_max);
+                p.sh_d[pix] = make_float4(sq.d.x, sq.d.y, sq.d.z, 0);
+                p.sh_c[pix] = make_float4(sq.contribution.x, sq.contribution.y, sq.contribution.z, 0);
             }
             p.rng[pix] = r.state;
-            if(cont) {
-                p.ray_o[pix] = make_float4(next_o.x, next_o.y, next_o.z, time);
-                p.ray_d[pix] = make_float4(next_d.x, next_d.y, next_d.z, 0);
-                p.thr[pix] = make_float4(throughput.x, throughput.y, throughput.z, previous_pdf);
-                p.rad[pix] = make_float4(radiance.x, radiance.y, radiance.z, 0);
-                p.ctr[pix] = make_int2(depth, left);
-            } else {
-                p.rad[pix] = make_float4(radiance.x, radiance.y, radiance.z, 0);
-            }
-            done = finished;
+            if(cont) store_path(p, pix, st);
+            else p.rad[pix] = make_float4(st.radiance.x, st.radiance.y, st.radiance.z, 0);
+            done = !cont;
         }
         block_append3(cont, shadow, done, pix, qn, &ctl->n_ext[parity ^ 1], p.shadow_queue, &ctl->n_shadow,
                       p.finish_queue, &ctl->n_finish);
+    }
+}
+
+template<bool ANY_HIT>
+__device__ __forceinline__ bool trace_query(const scene_dev& s, bool brute, bool wide, const ray_in& r, float high,
+                                            int32_t *stk, int stride, wide_stack& wstk, best_hit& b, int& error) {
+    int boxes = 0, prims = 0, spheres = 0;
+    if(wide) return traverse_wide<ANY_HIT, false>(s, r, 0.001f, high, wstk, b, boxes, prims, spheres, error);
+    return trace<ANY_HIT, false>(s, brute, r, 0.001f, high, stk, stride, b, boxes, prims, error);
+}
+
+// Tail of an add_samples call, part 1: the shadow queries still pending
+// from the last wavefront bounce.
+__global__ __launch_bounds__(256) void drain_shadow_kernel(scene_dev s, frame_params f, paths_dev p, loop_ctl *ctl) {
+    extern __shared__ int32_t lds[];
+    const uint32_t n = ctl->n_shadow;
+    const bool brute = f.brute != 0;
+    const bool wide = !brute && s.wroot_ref != empty_root && !f.exact_traversal;
+    const int stride = int(blockDim.x);
+    wide_stack wstk{reinterpret_cast<int2 *>(lds) + threadIdx.x, stride, p.spill,
+                    size_t(gridDim.x) * blockDim.x, size_t(blockIdx.x) * blockDim.x + threadIdx.x};
+    int32_t *stk = lds + threadIdx.x;
+    int error = 0;
+    const uint32_t step = gridDim.x * blockDim.x;
+    for(uint32_t item = blockIdx.x * blockDim.x + threadIdx.x; item < n; item += step) {
+        const int32_t pix = p.shadow_queue[item];
+        const float4 o = p.sh_o[pix], d = p.sh_d[pix];
+        best_hit b;
+        if(!trace_query<true>(s, brute, wide, setup_ray(o.x, o.y, o.z, d.x, d.y, d.z), o.w, stk, stride, wstk, b, error)) {
+            float4 c = p.sh_c[pix], ra = p.rad[pix];
+            ra.x = ra.x + c.x; ra.y = ra.y + c.y; ra.z = ra.z + c.z;
+            p.rad[pix] = ra;
+        }
+    }
+    if(error) atomicExch(&ctl->failure, 1);
+    if(blockIdx.x == 0 && threadIdx.x == 0) {
+        ctl->rays += n; ctl->shadow_rays += n;
+    }
+}
+
+// Tail of an add_samples call, part 2: each remaining pixel finishes all of
+// its samples in one thread -- the megakernel order of the reference
+// (cuda_renderer.cu:70-85) with the same per-bounce code, so every pixel
+// consumes its PCG stream and accumulates exactly as in the wavefront loop.
+// Used once few paths are live, where a bounce iteration costs as much as
+// its slowest query whatever the queue length.
+__global__ __launch_bounds__(256) void drain_kernel(scene_dev s, frame_params f, paths_dev p, loop_ctl *ctl, int parity) {
+    extern __shared__ int32_t lds[];
+    const uint32_t n = ctl->n_ext[parity];
+    const int32_t *q = p.queue[parity];
+    const bool brute = f.brute != 0;
+    const bool wide = !brute && s.wroot_ref != empty_root && !f.exact_traversal;
+    const int stride = int(blockDim.x);
+    wide_stack wstk{reinterpret_cast<int2 *>(lds) + threadIdx.x, stride, p.spill,
+                    size_t(gridDim.x) * blockDim.x, size_t(blockIdx.x) * blockDim.x + threadIdx.x};
+    int32_t *stk = lds + threadIdx.x;
+    int error = 0;
+    unsigned long long ext = 0, shadow = 0;
+    const uint32_t step = gridDim.x * blockDim.x;
+    for(uint32_t item = blockIdx.x * blockDim.x + threadIdx.x; item < n; item += step) {
+        const int32_t pix = q[item];
+        path_state st = load_path(p, pix);
+        rng_t r = pixel_rng(p.rng[pix], uint64_t(pix));
+        float *sum = p.sum + 3 * size_t(pix);
+        for(;;) {
+            best_hit b;
+            const bool hit = trace_query<false>(s, brute, wide, setup_ray(st.o.x, st.o.y, st.o.z, st.d.x, st.d.y, st.d.z),
+                                                FLT_MAX, stk, stride, wstk, b, error);
+            ext++;
+            shadow_query sq;
+            bool has_shadow;
+            const bool cont = shade_bounce(s, f, st, hit, b, r, sq, has_shadow);
+            if(has_shadow) {
+                shadow++;
+                best_hit bb;
+                if(!trace_query<true>(s, brute, wide, setup_ray(sq.o.x, sq.o.y, sq.o.z, sq.d.x, sq.d.y, sq.d.z), sq.t_max,
+                                      stk, stride, wstk, bb, error))
+                    st.radiance = add(st.radiance, sq.contribution);
+            }
+            if(cont) continue;
+            sum[0] = sum[0] + st.radiance.x; sum[1] = sum[1] + st.radiance.y; sum[2] = sum[2] + st.radiance.z;
+            if(--st.left <= 0) break;
+            const int x = int(pix % f.width), y = int(pix / f.width);
+            camera_ray(f.camera, x, y, f.width, f.height, r, st.o, st.d, st.time);
+            st.throughput = mk(1, 1, 1); st.radiance = mk(0, 0, 0); st.previous_pdf = 0; st.depth = 0;
+        }
+        p.rng[pix] = r.state;
+        p.ctr[pix] = make_int2(0, 0);
+    }
+    if(error) atomicExch(&ctl->failure, 1);
+    ext = wave_sum(ext); shadow = wave_sum(shadow);
+    if(__lane_id() == 0 && (ext || shadow)) {
+        atomicAdd(&ctl->rays, ext + shadow);
+        atomicAdd(&ctl->ext_rays, ext);
+        atomicAdd(&ctl->shadow_rays, shadow);
     }
 }
 
@@ -1041,6 +1162,18 @@ void launch_shade(const scene_dev& s, const frame_params& f, const paths_dev& p,
 void launch_regen(const frame_params& f, const paths_dev& p, loop_ctl *ctl, int parity, int grid, int block,
                   hipStream_t stream) {
     hipLaunchKernelGGL(regen_kernel, dim3(unsigned(grid)), dim3(unsigned(block)), 0, stream, f, p, ctl, parity);
+}
+void launch_drain(const scene_dev& s, const frame_params& f, const paths_dev& p, loop_ctl *ctl, int parity,
+                  uint32_t shadows, uint32_t paths, int block, hipStream_t stream) {
+    size_t lds = trace_lds_bytes(s, f, block);
+    if(shadows) {
+        unsigned grid = unsigned((shadows + uint32_t(block) - 1) / uint32_t(block));
+        hipLaunchKernelGGL(drain_shadow_kernel, dim3(grid), dim3(unsigned(block)), lds, stream, s, f, p, ctl);
+    }
+    if(paths) {
+        unsigned grid = unsigned((paths + uint32_t(block) - 1) / uint32_t(block));
+        hipLaunchKernelGGL(drain_kernel, dim3(grid), dim3(unsigned(block)), lds, stream, s, f, p, ctl, parity);
+    }
 }
 void launch_pack(const float *sum, uint8_t *rgba, int64_t count, int32_t samples, hipStream_t stream) {
     unsigned grid = unsigned((count + 255) / 256);

@@ -98,13 +98,14 @@ class LaunchStats(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int32), ("trace_launches", ctypes.c_int32), ("trace_ms", ctypes.c_float),
                 ("shade_ms", ctypes.c_float), ("extend_rays", ctypes.c_uint64), ("shadow_rays", ctypes.c_uint64),
                 ("regen_ms", ctypes.c_float),
-                ("box_tests", ctypes.c_uint64), ("triangle_tests", ctypes.c_uint64), ("sphere_tests", ctypes.c_uint64)]
+                ("box_tests", ctypes.c_uint64), ("triangle_tests", ctypes.c_uint64), ("sphere_tests", ctypes.c_uint64),
+                ("drained_paths", ctypes.c_uint64), ("drain_ms", ctypes.c_float)]
 
 
 class Tuning(ctypes.Structure):
     _fields_ = [("trace_block", ctypes.c_int32), ("shade_block", ctypes.c_int32), ("blocks_per_cu", ctypes.c_int32),
                 ("check_interval", ctypes.c_int32), ("time_kernels", ctypes.c_int32), ("count_tests", ctypes.c_int32),
-                ("exact_traversal", ctypes.c_int32)]
+                ("exact_traversal", ctypes.c_int32), ("drain_threshold", ctypes.c_int32)]
 
 
 def _ptr(a):
@@ -145,6 +146,8 @@ PT_FUNCTIONS = {
                                                ctypes.c_int32, ctypes.c_uint64]),
     "pt_renderer_last_launch_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "pt_renderer_set_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "pt_renderer_iteration_log": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                                 ctypes.c_void_p]),
 }
 
 
@@ -295,10 +298,20 @@ class Renderer:
         _check(lib().pt_renderer_last_launch_stats(self._h, ctypes.byref(st)))
         return {k: getattr(st, k) for k, _ in LaunchStats._fields_}
 
+    def iteration_log(self):
+        """(extension, shadow) query counts per bounce iteration of the last add_samples."""
+        cap = 1 << 16
+        e = np.zeros(cap, np.uint32)
+        sh = np.zeros(cap, np.uint32)
+        n = ctypes.c_int32()
+        _check(lib().pt_renderer_iteration_log(self._h, e.ctypes.data, sh.ctypes.data, cap, ctypes.byref(n)))
+        m = min(n.value, cap)
+        return e[:m], sh[:m]
+
     def set_tuning(self, trace_block=0, shade_block=0, blocks_per_cu=0, check_interval=0, time_kernels=False,
-                   count_tests=False, exact_traversal=False):
+                   count_tests=False, exact_traversal=False, drain_threshold=0):
         t = Tuning(trace_block, shade_block, blocks_per_cu, check_interval, int(bool(time_kernels)),
-                   int(bool(count_tests)), int(bool(exact_traversal)))
+                   int(bool(count_tests)), int(bool(exact_traversal)), int(drain_threshold))
         _check(lib().pt_renderer_set_tuning(self._h, ctypes.byref(t)))
 
 

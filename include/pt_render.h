@@ -164,8 +164,17 @@ typedef struct pt_launch_stats {
     uint64_t box_tests;      /* filled when pt_tuning.count_tests is set */
     uint64_t triangle_tests;
     uint64_t sphere_tests;
+    uint64_t drained_paths;  /* paths finished by the tail kernel */
+    float drain_ms;
 } pt_launch_stats;
 pt_status pt_renderer_last_launch_stats(const pt_renderer *renderer, pt_launch_stats *stats);
+
+/* Queue sizes of each bounce iteration of the last shaded add_samples:
+ * extension[i] closest-hit queries and shadow[i] any-hit queries traced by
+ * iteration i.  Writes min(capacity, iterations) entries; *count receives the
+ * number of iterations. */
+pt_status pt_renderer_iteration_log(const pt_renderer *renderer, uint32_t *extension, uint32_t *shadow,
+                                    size_t capacity, int32_t *count);
 
 /* Per-renderer tuning of the wavefront loop (0 keeps the current value). */
 typedef struct pt_tuning {
@@ -177,6 +186,8 @@ typedef struct pt_tuning {
     int32_t count_tests;     /* 1: count box/triangle/sphere tests (slower; for roofline bytes) */
     int32_t exact_traversal; /* 1: shaded path on the reference-order binary traversal instead of the
                                 4-wide nearest-first one (both return the same closest hit) */
+    int32_t drain_threshold; /* live paths at or below which the bounce loop hands every remaining
+                                pixel to one thread (-1: never) */
 } pt_tuning;
 pt_status pt_renderer_set_tuning(pt_renderer *renderer, const pt_tuning *tuning);
 

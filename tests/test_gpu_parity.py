@@ -296,3 +296,23 @@ def test_wide_traversal_matches_reference_order(maker):
     (s1, g1), rays1 = states[1]
     assert rays0 == rays1
     assert np.array_equal(s0, s1) and np.array_equal(g0, g1)
+
+
+@pytest.mark.parametrize("threshold", [-1, 1000, 1 << 30])
+def test_tail_drain_matches_wavefront_loop(threshold):
+    # The tail kernel (each remaining pixel finished in one thread) must give
+    # the same sums, PCG states and ray counts as running the bounce loop to
+    # the end; -1 never drains, 2^30 drains at the first check.
+    scene, cam = ptrace.icosphere(30, 4, 128, 72)
+    ref = ptrace.Renderer(scene, cam, 128, 72, 5, False, 0, True, 8)
+    ref.set_tuning(drain_threshold=-1)
+    ref.add_samples(6)
+    r = ptrace.Renderer(scene, cam, 128, 72, 5, False, 0, True, 8)
+    r.set_tuning(drain_threshold=threshold, check_interval=2)
+    r.add_samples(2)
+    r.add_samples(4)
+    (a, ga), (b, gb) = ref.read_state(), r.read_state()
+    assert np.array_equal(a, b) and np.array_equal(ga, gb)
+    assert ref.rays_traced() == r.rays_traced()
+    if threshold > 0:
+        assert r.last_launch_stats()["drained_paths"] > 0
