@@ -11,12 +11,15 @@ inputs resident in HBM.  ``value`` is rays per second over the timed steps,
 counting every closest-hit and shadow query (the reference's rays_traced).
 
 Also reported:
-  roofline     -- trace kernel (closest-hit + any-hit traversal), HBM-bound:
-                  algorithmic bytes of its launches / HIP-event time of those
-                  launches on the renderer's stream, against 8 TB/s.  Bytes
-                  per query = 32*box tests + 36*triangle tests + 16*sphere
-                  tests + 48 (closest hit) or + 36 (shadow), from an untimed
-                  counting pass over the same rays.
+  roofline     -- trace kernel (closest-hit + any-hit traversal of the bounce
+                  loop), HBM-bound: algorithmic bytes of its launches /
+                  HIP-event time of those launches on the renderer's stream,
+                  against 8 TB/s.  Bytes per query = 32*box tests +
+                  36*triangle tests + 16*sphere tests + 48 (closest hit) or
+                  + 36 (shadow), from an untimed counting pass over the same
+                  rays.  traffic = rocprofv3 FETCH_SIZE + WRITE_SIZE per launch
+                  (committed summary under profiles/), scaled by a plain-copy
+                  calibration.
   cpu_baseline -- the CPU oracle (a C restatement of the reference's host
                   transport path) on the host's cores over a 1-spp frame of
                   the same scene and camera.
@@ -101,17 +104,21 @@ def main():
     log(f"upload {time.perf_counter() - t0:.2f}s")
 
     # Untimed counting pass (same rays as every timed frame: same seed, reset).
+    # The test counts cover the trace kernel's launches only; the queries it
+    # handled are the per-iteration queue sizes (the tail kernel traces the rest).
     r.set_tuning(count_tests=True)
     r.add_samples(args.spp)
     counts = r.last_launch_stats()
+    ext_log, sh_log = r.iteration_log()
+    traced_ext, traced_sh = int(ext_log.sum()), int(sh_log.sum())
     r.set_tuning(count_tests=False, time_kernels=True)
-    log("counting pass:", counts)
+    log("counting pass:", counts, "trace-kernel queries:", traced_ext, traced_sh)
 
     for _ in range(args.warmup):
         r.reset(camera)
         r.add_samples(args.spp)
     rays = 0
-    trace_ms = shade_ms = 0.0
+    trace_ms = shade_ms = regen_ms = drain_ms = 0.0
     launches = 0
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -121,15 +128,16 @@ def main():
         rays += st["extend_rays"] + st["shadow_rays"]
         trace_ms += st["trace_ms"]
         shade_ms += st["shade_ms"]
+        regen_ms += st["regen_ms"]
+        drain_ms += st["drain_ms"]
         launches += st["trace_launches"]
     elapsed = time.perf_counter() - t0
     ms_per_step = elapsed / args.steps * 1e3
     mrays = rays / elapsed / 1e6
     log(f"{args.steps} steps: {elapsed:.3f}s, {mrays:.1f} Mrays/s, trace {trace_ms:.1f} ms, shade {shade_ms:.1f} ms")
 
-    frame_ext, frame_sh = counts["extend_rays"], counts["shadow_rays"]
     alg_bytes_frame = (32 * counts["box_tests"] + 36 * counts["triangle_tests"] + 16 * counts["sphere_tests"]
-                       + 48 * frame_ext + 36 * frame_sh)
+                       + 48 * traced_ext + 36 * traced_sh)
     per_step_launches = launches / args.steps
     bytes_per_launch = alg_bytes_frame / per_step_launches
     launch_s = trace_ms / 1e3 / launches
@@ -139,14 +147,14 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": "trace_kernel<false>", "avg_launch_ms": round(launch_s * 1e3, 4),
                 "alg_bytes_per_launch": int(bytes_per_launch),
-                "box_tests_per_ray": round(counts["box_tests"] / (frame_ext + frame_sh), 2),
-                "triangle_tests_per_ray": round(counts["triangle_tests"] / (frame_ext + frame_sh), 2)}
+                "box_tests_per_query": round(counts["box_tests"] / (traced_ext + traced_sh), 2),
+                "triangle_tests_per_query": round(counts["triangle_tests"] / (traced_ext + traced_sh), 2)}
 
     cpu = None
     if not args.no_cpu:
         threads = min(16, len(os.sched_getaffinity(0)))
         log(f"CPU baseline on {threads} threads ...")
-        cpu = cpu_baseline(scene, camera, threads)
+        cpu = cpu_baseline(scene, camera, threads, spp=4)
         log("cpu:", cpu)
 
     line = {"metric": "Mrays/s at 1080p, 1M tris", "value": round(mrays, 2), "unit": "Mrays/s", "n_gpus": 1,
@@ -158,7 +166,9 @@ def main():
                        "spp": args.spp, "max_depth": DEPTH, "triangles": int(len(scene.triangles)),
                        "rays_per_step": rays // args.steps, "trace_ms_per_step": round(trace_ms / args.steps, 2),
                        "shade_ms_per_step": round(shade_ms / args.steps, 2),
-                       "bounce_iterations": counts["iterations"]},
+                       "regen_ms_per_step": round(regen_ms / args.steps, 2),
+                       "drain_ms_per_step": round(drain_ms / args.steps, 2),
+                       "bounce_iterations": counts["iterations"], "drained_paths": counts["drained_paths"]},
             "roofline": roofline, "cpu_baseline": cpu}
     if cpu:
         line["gpu_over_cpu"] = round(mrays / cpu["value"], 1)
