@@ -235,8 +235,8 @@ std::vector<pt::light_rec> scene_lights(const pt_scene_desc& s) {
 
 // ---------------- host-libm tables ----------------
 // cos/sin of every angle cosine_direction and sphere-light sampling can form:
-// 2*float(M_PI)*v with v = k*2^-24 (light_sampling.h:20,47).  Generated with
-// the host libm, so device results equal the host build's bit for bit.
+// 2*float(M_PI)*v with v = k*2^-24 (light_sampling.h:20,47), with the host
+// libm, as the reference's host build computes them.
 constexpr size_t trig_entries = size_t(1) << 24;
 
 const std::vector<float2>& host_trig_table() {
@@ -263,20 +263,55 @@ const std::vector<float2>& host_trig_table() {
     return table;
 }
 
-const float2 *device_trig_table() {
+struct trig_tables { const unsigned long long *bits; const uint32_t *rank; const float2 *exceptions; };
+
+// Per device, once per process: compare the device's double-rounded cos/sin
+// with the host libm for all 2^24 angles and keep only the differences.
+trig_tables device_trig_tables() {
     static std::mutex lock;
-    static std::map<int, float2 *> tables;
+    static std::map<int, trig_tables> cache;
     int device = 0;
     HIP_CHECK(hipGetDevice(&device));
     std::lock_guard<std::mutex> guard(lock);
-    auto it = tables.find(device);
-    if(it != tables.end()) return it->second;
+    auto it = cache.find(device);
+    if(it != cache.end()) return it->second;
     const auto& host = host_trig_table();
-    float2 *ptr = nullptr;
-    HIP_CHECK(hipMalloc(reinterpret_cast<void **>(&ptr), trig_entries * sizeof(float2)));
-    HIP_CHECK(hipMemcpy(ptr, host.data(), trig_entries * sizeof(float2), hipMemcpyHostToDevice));
-    tables[device] = ptr; // kept for the life of the process
-    return ptr;
+    std::vector<float2> dev(trig_entries);
+    {
+        dev_buf<float2> tmp(trig_entries);
+        pt::launch_trig_values(tmp.get(), nullptr);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipDeviceSynchronize());
+        HIP_CHECK(hipMemcpy(dev.data(), tmp.get(), trig_entries * sizeof(float2), hipMemcpyDeviceToHost));
+    }
+    const size_t words = trig_entries / 64;
+    std::vector<unsigned long long> bits(words, 0);
+    std::vector<uint32_t> rank(words, 0);
+    std::vector<float2> exceptions;
+    for(size_t w = 0; w < words; w++) {
+        rank[w] = uint32_t(exceptions.size());
+        for(size_t b = 0; b < 64; b++) {
+            size_t k = 64 * w + b;
+            const float2 h = host[k], d = dev[k];
+            if(std::memcmp(&h.x, &d.x, 4) != 0 || std::memcmp(&h.y, &d.y, 4) != 0) {
+                bits[w] |= 1ull << b;
+                exceptions.push_back(h);
+            }
+        }
+    }
+    if(exceptions.empty()) exceptions.push_back(make_float2(0, 0));
+    unsigned long long *db = nullptr;
+    uint32_t *dr = nullptr;
+    float2 *de = nullptr;
+    HIP_CHECK(hipMalloc(reinterpret_cast<void **>(&db), words * sizeof(unsigned long long)));
+    HIP_CHECK(hipMalloc(reinterpret_cast<void **>(&dr), words * sizeof(uint32_t)));
+    HIP_CHECK(hipMalloc(reinterpret_cast<void **>(&de), exceptions.size() * sizeof(float2)));
+    HIP_CHECK(hipMemcpy(db, bits.data(), words * sizeof(unsigned long long), hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(dr, rank.data(), words * sizeof(uint32_t), hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(de, exceptions.data(), exceptions.size() * sizeof(float2), hipMemcpyHostToDevice));
+    trig_tables t{db, dr, de};  // kept for the life of the process
+    cache[device] = t;
+    return t;
 }
 
 void require_device() {
@@ -496,7 +531,8 @@ struct device_scene {
         view.lights = lights.get(); view.light_count = int32_t(lights_host.size());
         view.light_area = lights_host.empty() ? 0 : lights_host.back().cumulative_area;
         view.stack_capacity = std::max(2, std::min(stack_limit, int(s.max_depth)));
-        view.trig = nullptr; view.log2_table = nullptr; view.log2_count = 0;
+        view.trig_bits = nullptr; view.trig_rank = nullptr; view.trig_exceptions = nullptr;
+        view.log2_table = nullptr; view.log2_count = 0;
         view.wnodes = nullptr; view.leaves = nullptr; view.wroot_ref = pt::empty_root; view.wide_stack_capacity = 0;
         if(with_wide) {
             wide_layout w;
@@ -509,7 +545,8 @@ struct device_scene {
             }
         }
         if(with_tables) {
-            view.trig = device_trig_table();
+            const trig_tables t = device_trig_tables();
+            view.trig_bits = t.bits; view.trig_rank = t.rank; view.trig_exceptions = t.exceptions;
             // log2f(1+n) of the host libm for every count a traversal can reach.
             size_t n = size_t(std::min<int64_t>(s.node_count + 2, int64_t(1) << 26));
             std::vector<float> lg(n);

@@ -478,13 +478,26 @@ __device__ __forceinline__ float mis_weight(float a, float b) {
     a /= sc; b /= sc;
     return a * a / (a * a + b * b);
 }
-// light_sampling.h:18-25.  cosf/sinf of 2*float(pi)*v come from the host-libm
-// table indexed by the 24-bit draw, so they equal the host build's values.
+// cosf/sinf of angle = 2*float(pi)*(k*2^-24) (light_sampling.h:20,47) as
+// the host libm computes them: double evaluation rounded to float, replaced
+// by the stored host value for the k where the two differ.
+__device__ __noinline__ float2 device_cos_sin(uint32_t k) {
+    const float angle = (2 * PI_F) * (float(k) * 0x1p-24f);
+    return make_float2(float(cos(double(angle))), float(sin(double(angle))));
+}
+__device__ __forceinline__ float2 libm_cos_sin(const scene_dev& s, uint32_t k) {
+    const unsigned long long word = s.trig_bits[k >> 6];
+    const unsigned long long bit = 1ull << (k & 63u);
+    if(word & bit) return s.trig_exceptions[s.trig_rank[k >> 6] + uint32_t(__popcll(word & (bit - 1ull)))];
+    return device_cos_sin(k);
+}
+
+// light_sampling.h:18-25.
 __device__ __forceinline__ v3 cosine_direction(const scene_dev& s, v3 normal, rng_t& r) {
     float u = r.uniform();
     uint32_t k = r.next24();
     float radius = sqrtf(u);
-    float2 cs = s.trig[k];
+    float2 cs = libm_cos_sin(s, k);
     v3 helper = fabsf(normal.x) > 0.9f ? mk(0, 1, 0) : mk(1, 0, 0);
     v3 tangent = unit_vector(cross(helper, normal));
     v3 bitangent = cross(normal, tangent);
@@ -515,7 +528,7 @@ __device__ light_sample sample_light(const scene_dev& s, rng_t& r) {
     } else {
         const pt_flat_sphere& sp = s.spheres[light.primitive];
         float z = 1 - 2 * u, radius = sqrtf(fmaxf(0, 1 - z * z));
-        float2 cs = s.trig[kv];
+        float2 cs = libm_cos_sin(s, kv);
         res.normal = mk(radius * cs.x, z, radius * cs.y);
         res.point = add(mk(sp.center), scale(fabsf(sp.radius), res.normal));
         if(sp.radius < 0) res.normal = neg(res.normal);
@@ -1120,7 +1133,15 @@ __global__ void pack_kernel(const float *sum, uint8_t *rgba, int64_t count, int3
 
 } // namespace
 
+__global__ void trig_values_kernel(float2 *out) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if(k < (1u << 24)) out[k] = device_cos_sin(k);
+}
+
 // ---------------- launchers ----------------
+void launch_trig_values(float2 *out, hipStream_t stream) {
+    hipLaunchKernelGGL(trig_values_kernel, dim3((1u << 24) / 256), dim3(256), 0, stream, out);
+}
 void launch_query(const scene_dev& s, const pt_ray *rays, pt_trace_result *out, int32_t *boxes, int32_t *prims,
                   int32_t count, bool brute, hipStream_t stream) {
     const int block = 256;
