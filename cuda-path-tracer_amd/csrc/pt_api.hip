@@ -569,6 +569,40 @@ int device_cu_count() {
 } // namespace
 
 // ---------------- renderer (cuda_renderer.cu:142-247) ----------------
+// One independent bounce loop over a subset of the pixels, with its own
+// queues, control block and stream.  Pixels never interact (one PCG stream
+// and one writer each), so partitions run concurrently and one partition's
+// launch tail overlaps another's work.
+struct partition {
+    int index = 0;
+    int64_t count = 0;                   // pixels in this partition
+    hipStream_t stream = nullptr;
+    dev_buf<pt::loop_ctl> ctl;
+    pt::loop_ctl *ctl_host = nullptr;    // pinned mirror
+    dev_buf<int32_t> queue0, queue1, shadow_queue, finish_queue;
+    dev_buf<int2> spill;
+    dev_buf<uint32_t> log;
+    pt::paths_dev paths{};
+    std::vector<hipEvent_t> events;      // 5 per iteration of a check interval
+    hipEvent_t checked = nullptr, drain0 = nullptr, drain1 = nullptr, finished = nullptr;
+    int parity = 0;
+    bool done = false, drained = false;
+    ~partition() {
+        if(stream) (void)hipStreamSynchronize(stream);
+        for(auto e : events) (void)hipEventDestroy(e);
+        for(auto e : {checked, drain0, drain1, finished}) if(e) (void)hipEventDestroy(e);
+        if(ctl_host) (void)hipHostFree(ctl_host);
+        if(stream) (void)hipStreamDestroy(stream);
+    }
+    void read_ctl_async() {
+        HIP_CHECK(hipMemcpyAsync(ctl_host, ctl.get(), sizeof(pt::loop_ctl), hipMemcpyDeviceToHost, stream));
+        HIP_CHECK(hipEventRecord(checked, stream));
+    }
+};
+
+constexpr int partition_chunk = 4096;  // pixels per chunk dealt round-robin to partitions
+constexpr uint32_t log_capacity = 1u << 16;
+
 struct pt_renderer {
     std::unique_ptr<device_scene> scene;
     pt::frame_params frame{};
@@ -577,33 +611,28 @@ struct pt_renderer {
     uint64_t rays = 0;
     float milliseconds = 0;
     dim3 block = dim3(16, 16);
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;        // reset, debug views, packing
     hipEvent_t start = nullptr, stop = nullptr;
-    std::vector<hipEvent_t> kernel_events;
     size_t pixels = 0;
     dev_buf<float> sum;
     dev_buf<uint64_t> rng;
     dev_buf<uint8_t> rgba;
-    dev_buf<pt::loop_ctl> ctl;
-    pt::loop_ctl *ctl_host = nullptr;   // pinned mirror
-    // wavefront state (shaded mode only)
+    dev_buf<pt::loop_ctl> ctl;           // debug views' counters
+    pt::loop_ctl *ctl_host = nullptr;
+    // wavefront state (shaded mode only), indexed by pixel and shared by the partitions
     dev_buf<float4> ray_o, ray_d, thr, rad, sh_o, sh_d, sh_c;
     dev_buf<int2> ctr;
     dev_buf<int4> hit;
-    dev_buf<int32_t> queue0, queue1, shadow_queue, finish_queue;
-    dev_buf<int2> spill;
-    dev_buf<uint32_t> iteration_log;
-    static constexpr uint32_t log_capacity = 1u << 16;
-    pt::paths_dev paths{};
-    pt_tuning tuning{128, 256, 16, 8, 0, 0, 0, 262144};
+    std::vector<std::unique_ptr<partition>> parts;
+    pt_tuning tuning{128, 256, 16, 8, 0, 0, 0, 262144, 48, 1};
     int cus = 1;
     pt_launch_stats last{};
 
     ~pt_renderer() {
+        parts.clear();
         if(stream) (void)hipStreamSynchronize(stream);
         if(start) (void)hipEventDestroy(start);
         if(stop) (void)hipEventDestroy(stop);
-        for(auto e : kernel_events) (void)hipEventDestroy(e);
         if(ctl_host) (void)hipHostFree(ctl_host);
         if(stream) (void)hipStreamDestroy(stream);
     }
@@ -611,6 +640,7 @@ struct pt_renderer {
     void reset(const pt_flat_camera& camera) {
         validate_camera(camera);
         pt::launch_init_pixels(sum.get(), rng.get(), int64_t(pixels), seed, ctl.get(), stream);
+        for(auto& part : parts) pt::launch_clear_ctl(part->ctl.get(), stream);
         HIP_CHECK(hipGetLastError());
         HIP_CHECK(hipStreamSynchronize(stream));
         frame.camera = camera; samples = 0; milliseconds = 0; rays = 0;
@@ -632,11 +662,54 @@ struct pt_renderer {
         HIP_CHECK(hipStreamSynchronize(stream));
     }
 
+    // (Re)build the partitions when their number changes.
+    void ensure_partitions(int count) {
+        count = std::max(1, std::min(count, 8));
+        if(int(parts.size()) == count) return;
+        HIP_CHECK(hipDeviceSynchronize());
+        std::vector<uint64_t> rays_before;
+        for(auto& part : parts) {
+            HIP_CHECK(hipMemcpy(part->ctl_host, part->ctl.get(), sizeof(pt::loop_ctl), hipMemcpyDeviceToHost));
+            rays_before.push_back(part->ctl_host->rays);
+        }
+        uint64_t carried = 0;
+        for(uint64_t r : rays_before) carried += r;
+        parts.clear();
+        const int64_t chunks = (int64_t(pixels) + partition_chunk - 1) / partition_chunk;
+        for(int k = 0; k < count; k++) {
+            auto part = std::make_unique<partition>();
+            part->index = k;
+            for(int64_t c = k; c < chunks; c += count)
+                part->count += std::min<int64_t>(partition_chunk, int64_t(pixels) - c * partition_chunk);
+            HIP_CHECK(hipStreamCreateWithFlags(&part->stream, hipStreamNonBlocking));
+            for(auto *e : {&part->checked, &part->drain0, &part->drain1, &part->finished}) HIP_CHECK(hipEventCreate(e));
+            part->ctl.alloc(1);
+            HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&part->ctl_host), sizeof(pt::loop_ctl), hipHostMallocDefault));
+            const size_t n = size_t(std::max<int64_t>(1, part->count));
+            part->queue0.alloc(n); part->queue1.alloc(n); part->shadow_queue.alloc(n); part->finish_queue.alloc(n);
+            part->log.alloc(2 * size_t(log_capacity));
+            pt::loop_ctl init{};
+            init.log = part->log.get();
+            init.log_capacity = log_capacity;
+            if(k == 0) init.rays = carried;
+            HIP_CHECK(hipMemcpy(part->ctl.get(), &init, sizeof init, hipMemcpyHostToDevice));
+            *part->ctl_host = init;
+            auto& p = part->paths;
+            p.ray_o = ray_o.get(); p.ray_d = ray_d.get(); p.thr = thr.get(); p.rad = rad.get();
+            p.ctr = ctr.get(); p.hit = hit.get();
+            p.sh_o = sh_o.get(); p.sh_d = sh_d.get(); p.sh_c = sh_c.get();
+            p.rng = rng.get(); p.sum = sum.get();
+            p.queue[0] = part->queue0.get(); p.queue[1] = part->queue1.get();
+            p.shadow_queue = part->shadow_queue.get(); p.finish_queue = part->finish_queue.get();
+            parts.push_back(std::move(part));
+        }
+    }
+
     void add_samples(int32_t count) {
         if(count <= 0 || count > INT32_MAX - samples) fail(PT_EINVAL, "invalid accumulated sample count");
-        HIP_CHECK(hipEventRecord(start, stream));
         pt_launch_stats st{};
         if(frame.debug != PT_DEBUG_SHADED) {
+            HIP_CHECK(hipEventRecord(start, stream));
             pt::launch_debug(scene->view, frame, sum.get(), rng.get(), count, &ctl.get()->failure, block, stream);
             HIP_CHECK(hipGetLastError());
             pt::launch_add_rays(ctl.get(), (unsigned long long)pixels * (unsigned long long)count, stream);
@@ -644,84 +717,132 @@ struct pt_renderer {
             read_ctl();
             st.iterations = 1;
             st.extend_rays = uint64_t(pixels) * uint64_t(count);
-        } else {
-            pt::launch_start(scene->view, frame, paths, ctl.get(), count, stream);
-            HIP_CHECK(hipGetLastError());
-            int parity = 0;
-            const int trace_threads = trace_block();
-            const int trace_grid = cus * std::max(1, tuning.blocks_per_cu);
-            const int drain_block = 256;
-            const size_t drain_threads = tuning.drain_threshold > 0
-                ? (size_t(tuning.drain_threshold) + drain_block - 1) / drain_block * drain_block : 0;
-            size_t spill_need = std::max(size_t(trace_grid) * size_t(trace_threads), drain_threads) *
-                                size_t(pt::wide_spill_entries(scene->view.wide_stack_capacity));
-            if(spill.size() < spill_need) { spill.alloc(spill_need); paths.spill = spill.get(); }
-            const int shade_grid = int(std::min<size_t>((pixels + size_t(tuning.shade_block) - 1) / size_t(tuning.shade_block),
-                                                        size_t(cus) * 16));
-            const int interval = std::max(1, tuning.check_interval);
-            if(tuning.time_kernels && kernel_events.size() < size_t(5 * interval)) {
-                for(auto e : kernel_events) (void)hipEventDestroy(e);
-                kernel_events.assign(size_t(5 * interval), nullptr);
-                for(auto& e : kernel_events) HIP_CHECK(hipEventCreate(&e));
-            }
-            for(;;) {
-                for(int k = 0; k < interval; k++) {
-                    if(tuning.time_kernels) HIP_CHECK(hipEventRecord(kernel_events[size_t(5 * k)], stream));
-                    pt::launch_trace(scene->view, frame, paths, ctl.get(), parity, trace_grid, trace_threads,
-                                     tuning.count_tests != 0, stream);
-                    if(tuning.time_kernels) HIP_CHECK(hipEventRecord(kernel_events[size_t(5 * k + 1)], stream));
-                    pt::launch_bookkeep(ctl.get(), parity, stream);
-                    if(tuning.time_kernels) HIP_CHECK(hipEventRecord(kernel_events[size_t(5 * k + 2)], stream));
-                    pt::launch_shade(scene->view, frame, paths, ctl.get(), parity, std::max(1, shade_grid), tuning.shade_block, stream);
-                    if(tuning.time_kernels) HIP_CHECK(hipEventRecord(kernel_events[size_t(5 * k + 3)], stream));
-                    pt::launch_regen(frame, paths, ctl.get(), parity, std::max(1, shade_grid), tuning.shade_block, stream);
-                    if(tuning.time_kernels) HIP_CHECK(hipEventRecord(kernel_events[size_t(5 * k + 4)], stream));
-                    parity ^= 1;
-                    st.trace_launches++;
-                }
-                HIP_CHECK(hipGetLastError());
-                read_ctl();
-                if(tuning.time_kernels) {
-                    for(int k = 0; k < interval; k++) {
-                        float a = 0, b = 0, c = 0;
-                        HIP_CHECK(hipEventElapsedTime(&a, kernel_events[size_t(5 * k)], kernel_events[size_t(5 * k + 1)]));
-                        HIP_CHECK(hipEventElapsedTime(&b, kernel_events[size_t(5 * k + 2)], kernel_events[size_t(5 * k + 3)]));
-                        HIP_CHECK(hipEventElapsedTime(&c, kernel_events[size_t(5 * k + 3)], kernel_events[size_t(5 * k + 4)]));
-                        st.trace_ms += a; st.shade_ms += b; st.regen_ms += c;
-                    }
-                }
-                if(ctl_host->n_ext[parity] == 0 || ctl_host->failure) break;
-                if(tuning.drain_threshold > 0 && ctl_host->n_ext[parity] <= uint32_t(tuning.drain_threshold)) {
-                    hipEvent_t d0 = kernel_events.empty() ? nullptr : kernel_events[0];
-                    hipEvent_t d1 = kernel_events.size() > 1 ? kernel_events[1] : nullptr;
-                    if(tuning.time_kernels) HIP_CHECK(hipEventRecord(d0, stream));
-                    pt::launch_drain(scene->view, frame, paths, ctl.get(), parity, ctl_host->n_shadow,
-                                     ctl_host->n_ext[parity], drain_block, stream);
-                    if(tuning.time_kernels) HIP_CHECK(hipEventRecord(d1, stream));
-                    HIP_CHECK(hipGetLastError());
-                    st.drained_paths = ctl_host->n_ext[parity];
-                    read_ctl();
-                    if(tuning.time_kernels) HIP_CHECK(hipEventElapsedTime(&st.drain_ms, d0, d1));
-                    break;
-                }
-            }
-            HIP_CHECK(hipEventRecord(stop, stream));
             HIP_CHECK(hipEventSynchronize(stop));
-            st.iterations = int32_t(ctl_host->iterations);
-            st.extend_rays = ctl_host->ext_rays;
-            st.shadow_rays = ctl_host->shadow_rays;
-            st.box_tests = ctl_host->box_tests;
-            st.triangle_tests = ctl_host->triangle_tests;
-            st.sphere_tests = ctl_host->sphere_tests;
+            if(ctl_host->failure) fail(PT_EOVERFLOW, "HIP traversal stack overflow");
+            rays = ctl_host->rays;
+        } else {
+            shaded_samples(count, st);
         }
-        HIP_CHECK(hipEventSynchronize(stop));
-        if(ctl_host->failure) fail(PT_EOVERFLOW, "HIP traversal stack overflow");
         float elapsed = 0;
         HIP_CHECK(hipEventElapsedTime(&elapsed, start, stop));
         milliseconds += elapsed;
         samples += count;
-        rays = ctl_host->rays;
         last = st;
+    }
+
+    void shaded_samples(int32_t count, pt_launch_stats& st) {
+        ensure_partitions(tuning.partitions);
+        const int nparts = int(parts.size());
+        const int trace_threads = trace_block();
+        const bool use_lanes = tuning.lane_refill > 0 && !frame.brute && !frame.exact_traversal &&
+                               scene->view.wroot_ref != pt::empty_root;
+        const int trace_grid = cus * std::max(1, tuning.blocks_per_cu);
+        const int drain_block = 256;
+        const int64_t drain_threshold = tuning.drain_threshold > 0 ? std::max<int64_t>(1, tuning.drain_threshold / nparts) : 0;
+        const size_t drain_threads = drain_threshold ? size_t((drain_threshold + drain_block - 1) / drain_block * drain_block) : 0;
+        const size_t spill_need = std::max(size_t(trace_grid) * size_t(trace_threads), drain_threads) *
+                                  size_t(pt::wide_spill_entries(scene->view.wide_stack_capacity));
+        const int interval = std::max(1, tuning.check_interval);
+        HIP_CHECK(hipEventRecord(start, stream));
+        for(auto& part : parts) {
+            if(part->spill.size() < spill_need) { part->spill.alloc(spill_need); part->paths.spill = part->spill.get(); }
+            if(tuning.time_kernels && part->events.size() < size_t(5 * interval)) {
+                for(auto e : part->events) (void)hipEventDestroy(e);
+                part->events.assign(size_t(5 * interval), nullptr);
+                for(auto& e : part->events) HIP_CHECK(hipEventCreate(&e));
+            }
+            HIP_CHECK(hipStreamWaitEvent(part->stream, start, 0));
+            part->parity = 0;
+            part->done = part->count == 0;
+            part->drained = false;
+            pt::launch_start(scene->view, frame, part->paths, part->ctl.get(), count, part->index, nparts,
+                             partition_chunk, part->count, part->stream);
+        }
+        HIP_CHECK(hipGetLastError());
+        auto launch_iterations = [&](partition& part) {
+            pt::loop_ctl *c = part.ctl.get();
+            for(int k = 0; k < interval; k++) {
+                const int par = part.parity;
+                const int shade_grid = int(std::max<size_t>(1, std::min<size_t>(
+                    (size_t(part.count) + size_t(tuning.shade_block) - 1) / size_t(tuning.shade_block), size_t(cus) * 16)));
+                if(tuning.time_kernels) HIP_CHECK(hipEventRecord(part.events[size_t(5 * k)], part.stream));
+                if(use_lanes)
+                    pt::launch_trace_lanes(scene->view, frame, part.paths, c, par, trace_grid, trace_threads,
+                                           tuning.lane_refill, tuning.count_tests != 0, part.stream);
+                else
+                    pt::launch_trace(scene->view, frame, part.paths, c, par, trace_grid, trace_threads,
+                                     tuning.count_tests != 0, part.stream);
+                if(tuning.time_kernels) HIP_CHECK(hipEventRecord(part.events[size_t(5 * k + 1)], part.stream));
+                pt::launch_bookkeep(c, par, part.stream);
+                if(tuning.time_kernels) HIP_CHECK(hipEventRecord(part.events[size_t(5 * k + 2)], part.stream));
+                pt::launch_shade(scene->view, frame, part.paths, c, par, shade_grid, tuning.shade_block, part.stream);
+                if(tuning.time_kernels) HIP_CHECK(hipEventRecord(part.events[size_t(5 * k + 3)], part.stream));
+                pt::launch_regen(frame, part.paths, c, par, shade_grid, tuning.shade_block, part.stream);
+                if(tuning.time_kernels) HIP_CHECK(hipEventRecord(part.events[size_t(5 * k + 4)], part.stream));
+                part.parity ^= 1;
+                st.trace_launches++;
+            }
+            HIP_CHECK(hipGetLastError());
+            part.read_ctl_async();
+        };
+        for(;;) {
+            bool any = false;
+            for(auto& part : parts) if(!part->done) { launch_iterations(*part); any = true; }
+            if(!any) break;
+            for(auto& part : parts) {
+                if(part->done) continue;
+                HIP_CHECK(hipEventSynchronize(part->checked));
+                if(tuning.time_kernels) {
+                    for(int k = 0; k < interval; k++) {
+                        float a = 0, b = 0, c = 0;
+                        HIP_CHECK(hipEventElapsedTime(&a, part->events[size_t(5 * k)], part->events[size_t(5 * k + 1)]));
+                        HIP_CHECK(hipEventElapsedTime(&b, part->events[size_t(5 * k + 2)], part->events[size_t(5 * k + 3)]));
+                        HIP_CHECK(hipEventElapsedTime(&c, part->events[size_t(5 * k + 3)], part->events[size_t(5 * k + 4)]));
+                        st.trace_ms += a; st.shade_ms += b; st.regen_ms += c;
+                    }
+                }
+                const pt::loop_ctl& h = *part->ctl_host;
+                const uint32_t live = h.n_ext[part->parity];
+                if(live == 0 || h.failure) { part->done = true; continue; }
+                if(drain_threshold && int64_t(live) <= drain_threshold) {
+                    if(tuning.time_kernels) HIP_CHECK(hipEventRecord(part->drain0, part->stream));
+                    pt::launch_drain(scene->view, frame, part->paths, part->ctl.get(), part->parity, h.n_shadow, live,
+                                     drain_block, part->stream);
+                    HIP_CHECK(hipGetLastError());
+                    if(tuning.time_kernels) HIP_CHECK(hipEventRecord(part->drain1, part->stream));
+                    st.drained_paths += live;
+                    part->read_ctl_async();
+                    part->done = part->drained = true;
+                }
+            }
+        }
+        for(auto& part : parts) {
+            HIP_CHECK(hipEventRecord(part->finished, part->stream));
+            HIP_CHECK(hipStreamWaitEvent(stream, part->finished, 0));
+        }
+        HIP_CHECK(hipEventRecord(stop, stream));
+        HIP_CHECK(hipEventSynchronize(stop));
+        uint64_t total_rays = 0;
+        int failure = 0;
+        for(auto& part : parts) {
+            HIP_CHECK(hipEventSynchronize(part->checked));
+            const pt::loop_ctl& h = *part->ctl_host;
+            st.iterations = std::max(st.iterations, int32_t(h.iterations));
+            st.extend_rays += h.ext_rays;
+            st.shadow_rays += h.shadow_rays;
+            st.box_tests += h.box_tests;
+            st.triangle_tests += h.triangle_tests;
+            st.sphere_tests += h.sphere_tests;
+            total_rays += h.rays;
+            failure |= h.failure;
+            if(tuning.time_kernels && part->drained) {
+                float d = 0;
+                HIP_CHECK(hipEventElapsedTime(&d, part->drain0, part->drain1));
+                st.drain_ms += d;
+            }
+        }
+        if(failure) fail(PT_EOVERFLOW, "HIP traversal stack overflow");
+        rays = total_rays;
     }
 };
 
@@ -737,22 +858,10 @@ void allocate_state(pt_renderer& r, bool wavefront) {
     HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&r.ctl_host), sizeof(pt::loop_ctl), hipHostMallocDefault));
     std::memset(r.ctl_host, 0, sizeof(pt::loop_ctl));
     if(!wavefront) return;
-    r.iteration_log.alloc(2 * size_t(pt_renderer::log_capacity));
-    pt::loop_ctl init{};
-    init.log = r.iteration_log.get();
-    init.log_capacity = pt_renderer::log_capacity;
-    HIP_CHECK(hipMemcpy(r.ctl.get(), &init, sizeof init, hipMemcpyHostToDevice));
     r.ray_o.alloc(n); r.ray_d.alloc(n); r.thr.alloc(n); r.rad.alloc(n);
     r.sh_o.alloc(n); r.sh_d.alloc(n); r.sh_c.alloc(n);
     r.ctr.alloc(n); r.hit.alloc(n);
-    r.queue0.alloc(n); r.queue1.alloc(n); r.shadow_queue.alloc(n); r.finish_queue.alloc(n);
-    auto& p = r.paths;
-    p.ray_o = r.ray_o.get(); p.ray_d = r.ray_d.get(); p.thr = r.thr.get(); p.rad = r.rad.get();
-    p.ctr = r.ctr.get(); p.hit = r.hit.get();
-    p.sh_o = r.sh_o.get(); p.sh_d = r.sh_d.get(); p.sh_c = r.sh_c.get();
-    p.rng = r.rng.get(); p.sum = r.sum.get();
-    p.queue[0] = r.queue0.get(); p.queue[1] = r.queue1.get(); p.shadow_queue = r.shadow_queue.get();
-    p.finish_queue = r.finish_queue.get();
+    r.ensure_partitions(r.tuning.partitions);
 }
 
 void check_grid(const pt_renderer& r, unsigned bx, unsigned by) {
@@ -949,9 +1058,14 @@ pt_status pt_renderer_write_state(pt_renderer *renderer, const float *sum, const
         HIP_CHECK(hipMemcpy(renderer->rng.get(), rng_state, pixels * sizeof(uint64_t), hipMemcpyHostToDevice));
         pt::loop_ctl c{};
         c.rays = rays;
-        c.log = renderer->iteration_log.get();
-        c.log_capacity = renderer->iteration_log.size() ? pt_renderer::log_capacity : 0;
         HIP_CHECK(hipMemcpy(renderer->ctl.get(), &c, sizeof(c), hipMemcpyHostToDevice));
+        for(auto& part : renderer->parts) {
+            pt::loop_ctl pc{};
+            pc.rays = part->index == 0 ? rays : 0;
+            pc.log = part->log.get();
+            pc.log_capacity = log_capacity;
+            HIP_CHECK(hipMemcpy(part->ctl.get(), &pc, sizeof(pc), hipMemcpyHostToDevice));
+        }
         renderer->samples = samples; renderer->rays = rays;
     });
 }
@@ -967,17 +1081,27 @@ pt_status pt_renderer_iteration_log(const pt_renderer *renderer, uint32_t *exten
                                     size_t capacity, int32_t *count) {
     return guarded([&] {
         if(!renderer || !count) fail(PT_EINVAL, "null argument");
-        int32_t n = renderer->last.iterations;
-        *count = n;
-        if(renderer->frame.debug != PT_DEBUG_SHADED || !renderer->iteration_log.size()) { *count = 0; return; }
-        size_t m = std::min<size_t>({capacity, size_t(n), size_t(pt_renderer::log_capacity)});
-        if(!m) return;
-        std::vector<uint32_t> buf(2 * m);
-        HIP_CHECK(hipMemcpy(buf.data(), renderer->iteration_log.get(), buf.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
-        for(size_t i = 0; i < m; i++) {
-            if(extension) extension[i] = buf[2 * i];
-            if(shadow) shadow[i] = buf[2 * i + 1];
+        *count = 0;
+        if(renderer->frame.debug != PT_DEBUG_SHADED) return;
+        // Partitions in order, each with its own iterations.
+        size_t written = 0;
+        int32_t total = 0;
+        for(const auto& part : renderer->parts) {
+            HIP_CHECK(hipMemcpy(part->ctl_host, part->ctl.get(), sizeof(pt::loop_ctl), hipMemcpyDeviceToHost));
+            const size_t n = std::min<size_t>(part->ctl_host->iterations, log_capacity);
+            total += int32_t(n);
+            const size_t m = std::min(n, capacity - std::min(capacity, written));
+            if(m) {
+                std::vector<uint32_t> buf(2 * m);
+                HIP_CHECK(hipMemcpy(buf.data(), part->log.get(), buf.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+                for(size_t i = 0; i < m; i++) {
+                    if(extension) extension[written + i] = buf[2 * i];
+                    if(shadow) shadow[written + i] = buf[2 * i + 1];
+                }
+                written += m;
+            }
         }
+        *count = total;
     });
 }
 
@@ -998,6 +1122,14 @@ pt_status pt_renderer_set_tuning(pt_renderer *renderer, const pt_tuning *tuning)
         t.count_tests = tuning->count_tests ? 1 : 0;
         t.exact_traversal = tuning->exact_traversal ? 1 : 0;
         if(tuning->drain_threshold) t.drain_threshold = tuning->drain_threshold < 0 ? -1 : tuning->drain_threshold;
+        if(tuning->lane_refill) {
+            if(tuning->lane_refill > 64) fail(PT_EINVAL, "lane refill must be 1..64 or -1");
+            t.lane_refill = tuning->lane_refill < 0 ? -1 : tuning->lane_refill;
+        }
+        if(tuning->partitions) {
+            if(tuning->partitions < 1 || tuning->partitions > 8) fail(PT_EINVAL, "partitions must be 1..8");
+            t.partitions = tuning->partitions;
+        }
         renderer->tuning = t;
     });
 }

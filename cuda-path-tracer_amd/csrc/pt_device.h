@@ -123,10 +123,16 @@ void launch_query(const scene_dev& s, const pt_ray *rays, pt_trace_result *out, 
 void launch_init_pixels(float *sum, uint64_t *rng, int64_t count, uint64_t seed, loop_ctl *ctl, hipStream_t stream);
 void launch_debug(const scene_dev& s, const frame_params& f, float *sum, uint64_t *rng, int32_t samples,
                   int32_t *failure, dim3 block, hipStream_t stream);
+// First camera rays of one pixel partition (chunks c with c % parts == part).
 void launch_start(const scene_dev& s, const frame_params& f, const paths_dev& p, loop_ctl *ctl, int32_t samples,
-                  hipStream_t stream);
+                  int part, int parts, int chunk, int64_t count, hipStream_t stream);
+void launch_clear_ctl(loop_ctl *ctl, hipStream_t stream);
 void launch_trace(const scene_dev& s, const frame_params& f, const paths_dev& p, loop_ctl *ctl, int parity,
                   int grid, int block, bool count_tests, hipStream_t stream);
+// Wide-layout trace stage with persistent lanes (refill: idle lanes that
+// trigger a new claim, 1..64).
+void launch_trace_lanes(const scene_dev& s, const frame_params& f, const paths_dev& p, loop_ctl *ctl, int parity,
+                        int grid, int block, int refill, bool count_tests, hipStream_t stream);
 void launch_bookkeep(loop_ctl *ctl, int parity, hipStream_t stream);
 void launch_shade(const scene_dev& s, const frame_params& f, const paths_dev& p, loop_ctl *ctl, int parity,
                   int grid, int block, hipStream_t stream);

@@ -368,6 +368,86 @@ __device__ bool traverse_wide(const scene_dev& s, const ray_in& r, float low, fl
     }
 }
 
+// ---------------- resumable wide traversal (persistent lanes) ----------------
+// The same traversal as traverse_wide, cut into steps (one wide node or one
+// leaf cluster, then the pop) so that a lane whose query has finished can
+// take a new one while the rest of its wave is still traversing.
+struct wide_query {
+    ray_in r;
+    float high, closest;
+    int32_t cur, count, best_rank;
+    best_hit best;
+    bool found, any;
+};
+
+__device__ __forceinline__ void wide_begin(const scene_dev& s, wide_query& q, bool any, float high) {
+    q.high = high; q.closest = high; q.cur = s.wroot_ref; q.count = 0; q.best_rank = -1;
+    q.found = false; q.any = any;
+}
+
+// Returns true when the query has finished.
+template<bool STATS>
+__device__ __forceinline__ bool wide_step(const scene_dev& s, wide_query& q, wide_stack& stk, int& boxes, int& prims,
+                                          int& spheres, int& error) {
+    const float low = 0.001f;
+    const ray_in& r = q.r;
+    if(q.cur >= 0) {
+        const float4 *nd = s.wnodes + 8 * size_t(q.cur);
+        const float4 LX = nd[0], LY = nd[1], LZ = nd[2], HX = nd[3], HY = nd[4], HZ = nd[5], RF = nd[6];
+        const int32_t r0 = __float_as_int(RF.x), r1 = __float_as_int(RF.y), r2 = __float_as_int(RF.z), r3 = __float_as_int(RF.w);
+        const float limit = q.found ? next_toward(q.closest, q.high) : q.high;
+        float e[4];
+        int32_t c[4];
+        bool h0 = r0 != wide_empty && slab_entry(LX.x, LY.x, LZ.x, HX.x, HY.x, HZ.x, r, low, limit, e[0]);
+        bool h1 = r1 != wide_empty && slab_entry(LX.y, LY.y, LZ.y, HX.y, HY.y, HZ.y, r, low, limit, e[1]);
+        bool h2 = r2 != wide_empty && slab_entry(LX.z, LY.z, LZ.z, HX.z, HY.z, HZ.z, r, low, limit, e[2]);
+        bool h3 = r3 != wide_empty && slab_entry(LX.w, LY.w, LZ.w, HX.w, HY.w, HZ.w, r, low, limit, e[3]);
+        if(STATS) boxes += (r0 != wide_empty) + (r1 != wide_empty) + (r2 != wide_empty) + (r3 != wide_empty);
+        if(!h0) e[0] = INFINITY;
+        if(!h1) e[1] = INFINITY;
+        if(!h2) e[2] = INFINITY;
+        if(!h3) e[3] = INFINITY;
+        c[0] = r0; c[1] = r1; c[2] = r2; c[3] = r3;
+        const int nh = int(h0) + int(h1) + int(h2) + int(h3);
+        if(nh) {
+            // nearest first (any-hit queries too: their order does not change the answer)
+#define PT_CSWAP(i, j) if(e[j] < e[i]) { float te = e[i]; e[i] = e[j]; e[j] = te; int32_t tc = c[i]; c[i] = c[j]; c[j] = tc; }
+            PT_CSWAP(0, 1) PT_CSWAP(2, 3) PT_CSWAP(0, 2) PT_CSWAP(1, 3) PT_CSWAP(1, 2)
+#undef PT_CSWAP
+            if(q.count + nh - 1 > s.wide_stack_capacity) { error = 1; return true; }
+            for(int k = nh - 1; k >= 1; k--) { stk.store(q.count, c[k], e[k]); q.count++; }
+            q.cur = c[0];
+            return false;
+        }
+    } else {
+        const int32_t v = ~q.cur;
+        const int32_t first = v >> 3, n = (v & 7) + 1;
+        for(int j = 0; j < n; j++) {
+            const int32_t k = first + j;
+            const float4 A = s.leaves[3 * size_t(k)], B = s.leaves[3 * size_t(k) + 1], C = s.leaves[3 * size_t(k) + 2];
+            const float limit = q.found ? next_toward(q.closest, q.high) : q.high;
+            float t, u = 0, vv = 0;
+            bool hit;
+            if(B.w < 0) { hit = sphere_test(A, r, low, limit, t); if(STATS) spheres++; }
+            else hit = tri_test(A, B, C, r, low, limit, t, u, vv);
+            if(STATS) prims++;
+            if(hit && (!q.found || t < q.closest || (t == q.closest && k > q.best_rank))) {
+                q.found = true; q.closest = t; q.best_rank = k;
+                q.best.t = t; q.best.u = u; q.best.v = vv; q.best.code = __float_as_int(C.w);
+                if(q.any) return true;
+            }
+        }
+    }
+    for(;;) {
+        if(q.count == 0) return true;
+        --q.count;
+        const int2 top = stk.load(q.count);
+        if(q.found && !(__int_as_float(top.y) < next_toward(q.closest, q.high))) continue;
+        q.cur = top.x;
+        return false;
+    }
+}
+
 template<bool ANY_HIT, bool STATS>
 __device__ __forceinline__ bool trace(const scene_dev& s, bool brute, const ray_in& r, float low, float high,
                                       int32_t *stk, int stride, best_hit& best, int& boxes, int& prims, int& error) {
@@ -759,15 +839,20 @@ __global__ void debug_kernel(scene_dev s, frame_params f, float *sum, uint64_t *
     rngs[i] = r.state;
 }
 
-// Start of an add_samples call: the first camera ray of every pixel.
-__global__ void start_kernel(scene_dev s, frame_params f, paths_dev p, loop_ctl *ctl, int32_t samples) {
-    int64_t n = int64_t(f.width) * f.height;
-    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if(i == 0) {
-        ctl->n_ext[0] = uint32_t(n); ctl->n_ext[1] = 0; ctl->n_shadow = 0; ctl->n_finish = 0; ctl->work = 0;
+// Start of an add_samples call: the first camera ray of every pixel of one
+// partition.  Pixels are dealt to `parts` partitions in chunks of
+// `chunk` consecutive pixels: chunk c belongs to partition c % parts.
+__global__ void start_kernel(scene_dev s, frame_params f, paths_dev p, loop_ctl *ctl, int32_t samples,
+                             int part, int parts, int chunk, int64_t count) {
+    const int64_t n = int64_t(f.width) * f.height;
+    const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if(j == 0) {
+        ctl->n_ext[0] = uint32_t(count); ctl->n_ext[1] = 0; ctl->n_shadow = 0; ctl->n_finish = 0; ctl->work = 0;
         ctl->iterations = 0; ctl->ext_rays = 0; ctl->shadow_rays = 0;
         ctl->box_tests = 0; ctl->triangle_tests = 0; ctl->sphere_tests = 0;
     }
+    if(j >= count) return;
+    const int64_t i = ((j / chunk) * parts + part) * chunk + j % chunk;
     if(i >= n) return;
     rng_t r = pixel_rng(p.rng[i], uint64_t(i));
     int x = int(i % f.width), y = int(i / f.width);
@@ -779,12 +864,11 @@ __global__ void start_kernel(scene_dev s, frame_params f, paths_dev p, loop_ctl 
     p.rad[i] = make_float4(0, 0, 0, 0);
     p.ctr[i] = make_int2(0, samples);
     p.rng[i] = r.state;
-    p.queue[0][i] = int32_t(i);
+    p.queue[0][j] = int32_t(i);
 }
 
-// Closest hit for the extension queue and any-hit for the shadow queue.
-// Persistent: each wave claims 64 consecutive items at a time.  COUNT adds the
-// per-query test counts to the control block (untimed counting passes).
+__global__ void clear_ctl_kernel(loop_ctl *ctl) { ctl->failure = 0; ctl->rays = 0; }
+
 template<bool COUNT>
 __global__ __launch_bounds__(256) void trace_kernel(scene_dev s, frame_params f, paths_dev p, loop_ctl *ctl, int parity) {
     extern __shared__ int32_t lds[];
@@ -843,6 +927,81 @@ __global__ __launch_bounds__(256) void trace_kernel(scene_dev s, frame_params f,
             }
         }
         if(COUNT) { n_box += unsigned(boxes); n_leaf += unsigned(prims); n_sph += unsigned(spheres); }
+    }
+    if(error) atomicExch(&ctl->failure, 1);
+    if(COUNT) {
+        n_box = wave_sum(n_box); n_leaf = wave_sum(n_leaf); n_sph = wave_sum(n_sph);
+        if(lane == 0) {
+            atomicAdd(&ctl->box_tests, n_box);
+            atomicAdd(&ctl->triangle_tests, n_leaf - n_sph);
+            atomicAdd(&ctl->sphere_tests, n_sph);
+        }
+    }
+}
+
+// Trace stage on the wide layout with persistent lanes: whenever at least
+// `refill` lanes of a wave are idle, the wave claims that many new queries
+// (one atomic) and hands them to its idle lanes, so a wave does not wait for
+// its slowest query before taking new work.
+template<bool COUNT>
+__global__ __launch_bounds__(256) void trace_lanes_kernel(scene_dev s, frame_params f, paths_dev p, loop_ctl *ctl,
+                                                          int parity, int refill) {
+    extern __shared__ int32_t lds[];
+    const uint32_t n_ext = ctl->n_ext[parity];
+    const uint32_t total = n_ext + ctl->n_shadow;
+    const int32_t *queue = p.queue[parity];
+    const int lane = __lane_id();
+    wide_stack wstk{reinterpret_cast<int2 *>(lds) + threadIdx.x, int(blockDim.x), p.spill,
+                    size_t(gridDim.x) * blockDim.x, size_t(blockIdx.x) * blockDim.x + threadIdx.x};
+    int error = 0;
+    unsigned long long n_box = 0, n_leaf = 0, n_sph = 0;
+    int boxes = 0, prims = 0, spheres = 0;
+    wide_query q;
+    bool active = false, exhausted = false;
+    int32_t pix = 0;
+    for(;;) {
+        const uint64_t idle = __ballot(!active);
+        const int nidle = __popcll(idle);
+        if(!exhausted && nidle >= refill) {
+            uint32_t b = 0;
+            if(lane == 0) b = atomicAdd(&ctl->work, uint32_t(nidle));
+            b = __shfl(b, 0);
+            if(b >= total) exhausted = true;
+            else if(!active) {
+                const uint32_t item = b + uint32_t(__popcll(idle & ((1ull << lane) - 1ull)));
+                if(item < total) {
+                    if(item < n_ext) {
+                        pix = queue[item];
+                        const float4 o = p.ray_o[pix], d = p.ray_d[pix];
+                        q.r = setup_ray(o.x, o.y, o.z, d.x, d.y, d.z);
+                        wide_begin(s, q, false, FLT_MAX);
+                    } else {
+                        pix = p.shadow_queue[item - n_ext];
+                        const float4 o = p.sh_o[pix], d = p.sh_d[pix];
+                        q.r = setup_ray(o.x, o.y, o.z, d.x, d.y, d.z);
+                        wide_begin(s, q, true, o.w);
+                    }
+                    active = true;
+                }
+            }
+        }
+        if(__ballot(active) == 0) {
+            if(exhausted) break;
+            continue;
+        }
+        if(active && wide_step<COUNT>(s, q, wstk, boxes, prims, spheres, error)) {
+            active = false;
+            if(!q.any) {
+                p.hit[pix] = q.found ? make_int4(__float_as_int(q.best.t), __float_as_int(q.best.u),
+                                                 __float_as_int(q.best.v), q.best.code)
+                                     : make_int4(0, 0, 0, -1);
+            } else if(!q.found) {
+                float4 c = p.sh_c[pix], ra = p.rad[pix];
+                ra.x = ra.x + c.x; ra.y = ra.y + c.y; ra.z = ra.z + c.z;
+                p.rad[pix] = ra;
+            }
+            if(COUNT) { n_box += unsigned(boxes); n_leaf += unsigned(prims); n_sph += unsigned(spheres); boxes = prims = spheres = 0; }
+        }
     }
     if(error) atomicExch(&ctl->failure, 1);
     if(COUNT) {
@@ -1160,10 +1319,13 @@ void launch_debug(const scene_dev& s, const frame_params& f, float *sum, uint64_
                        s, f, sum, rng, samples, failure);
 }
 void launch_start(const scene_dev& s, const frame_params& f, const paths_dev& p, loop_ctl *ctl, int32_t samples,
-                  hipStream_t stream) {
-    int64_t n = int64_t(f.width) * f.height;
-    unsigned grid = unsigned((n + 255) / 256);
-    hipLaunchKernelGGL(start_kernel, dim3(grid), dim3(256), 0, stream, s, f, p, ctl, samples);
+                  int part, int parts, int chunk, int64_t count, hipStream_t stream) {
+    unsigned grid = unsigned((count + 255) / 256);
+    hipLaunchKernelGGL(start_kernel, dim3(grid ? grid : 1), dim3(256), 0, stream, s, f, p, ctl, samples, part, parts,
+                       chunk, count);
+}
+void launch_clear_ctl(loop_ctl *ctl, hipStream_t stream) {
+    hipLaunchKernelGGL(clear_ctl_kernel, dim3(1), dim3(1), 0, stream, ctl);
 }
 void launch_trace(const scene_dev& s, const frame_params& f, const paths_dev& p, loop_ctl *ctl, int parity,
                   int grid, int block, bool count_tests, hipStream_t stream) {
@@ -1172,6 +1334,16 @@ void launch_trace(const scene_dev& s, const frame_params& f, const paths_dev& p,
         hipLaunchKernelGGL(trace_kernel<true>, dim3(unsigned(grid)), dim3(unsigned(block)), lds, stream, s, f, p, ctl, parity);
     else
         hipLaunchKernelGGL(trace_kernel<false>, dim3(unsigned(grid)), dim3(unsigned(block)), lds, stream, s, f, p, ctl, parity);
+}
+void launch_trace_lanes(const scene_dev& s, const frame_params& f, const paths_dev& p, loop_ctl *ctl, int parity,
+                        int grid, int block, int refill, bool count_tests, hipStream_t stream) {
+    size_t lds = wide_stack_bytes(s.wide_stack_capacity, block);
+    if(count_tests)
+        hipLaunchKernelGGL(trace_lanes_kernel<true>, dim3(unsigned(grid)), dim3(unsigned(block)), lds, stream, s, f, p, ctl,
+                           parity, refill);
+    else
+        hipLaunchKernelGGL(trace_lanes_kernel<false>, dim3(unsigned(grid)), dim3(unsigned(block)), lds, stream, s, f, p, ctl,
+                           parity, refill);
 }
 void launch_bookkeep(loop_ctl *ctl, int parity, hipStream_t stream) {
     hipLaunchKernelGGL(bookkeep_kernel, dim3(1), dim3(1), 0, stream, ctl, parity);

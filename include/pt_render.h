@@ -171,12 +171,14 @@ pt_status pt_renderer_last_launch_stats(const pt_renderer *renderer, pt_launch_s
 
 /* Queue sizes of each bounce iteration of the last shaded add_samples:
  * extension[i] closest-hit queries and shadow[i] any-hit queries traced by
- * iteration i.  Writes min(capacity, iterations) entries; *count receives the
- * number of iterations. */
+ * iteration i, the pixel partitions' iterations one partition after the
+ * other.  Writes min(capacity, iterations) entries; *count receives the
+ * number of iterations of all partitions. */
 pt_status pt_renderer_iteration_log(const pt_renderer *renderer, uint32_t *extension, uint32_t *shadow,
                                     size_t capacity, int32_t *count);
 
-/* Per-renderer tuning of the wavefront loop (0 keeps the current value). */
+/* Per-renderer tuning of the wavefront loop (0 keeps the current value).
+ * Defaults: 128, 256, 16, 8, 0, 0, 0, 262144, 48, 1. */
 typedef struct pt_tuning {
     int32_t trace_block;     /* threads per block of the trace kernel, multiple of 64 */
     int32_t shade_block;     /* threads per block of the shade kernel, multiple of 64 */
@@ -188,6 +190,10 @@ typedef struct pt_tuning {
                                 4-wide nearest-first one (both return the same closest hit) */
     int32_t drain_threshold; /* live paths at or below which the bounce loop hands every remaining
                                 pixel to one thread (-1: never) */
+    int32_t lane_refill;     /* wide traversal with persistent lanes: idle lanes of a wave that
+                                trigger a claim of new queries (1..64; -1: one batch per wave) */
+    int32_t partitions;      /* pixel partitions whose bounce loops run concurrently on their own
+                                streams (1..8) */
 } pt_tuning;
 pt_status pt_renderer_set_tuning(pt_renderer *renderer, const pt_tuning *tuning);
 

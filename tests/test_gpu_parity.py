@@ -316,3 +316,30 @@ def test_tail_drain_matches_wavefront_loop(threshold):
     assert ref.rays_traced() == r.rays_traced()
     if threshold > 0:
         assert r.last_launch_stats()["drained_paths"] > 0
+
+
+@pytest.mark.parametrize("parts", [2, 3, 5])
+def test_pixel_partitions_match_single_loop(parts):
+    # Concurrent pixel partitions (4096-pixel chunks dealt round-robin, the
+    # last chunk partial) must reproduce the single bounce loop exactly, also
+    # when the partition count changes between add_samples calls.
+    scene, cam = ptrace.icosphere(30, 6, 200, 150)
+    ref = ptrace.Renderer(scene, cam, 200, 150, 11, False, 0, True, 8)
+    ref.set_tuning(partitions=1)
+    ref.add_samples(3)
+    ref.add_samples(2)
+    r = ptrace.Renderer(scene, cam, 200, 150, 11, False, 0, True, 8)
+    r.set_tuning(partitions=parts, drain_threshold=2000)
+    r.add_samples(3)
+    ext, sh = r.iteration_log()
+    st = r.last_launch_stats()
+    assert int(ext.sum()) + int(sh.sum()) <= st["extend_rays"] + st["shadow_rays"]
+    r.set_tuning(partitions=1 if parts != 1 else 2)
+    r.add_samples(2)
+    (a, ga), (b, gb) = ref.read_state(), r.read_state()
+    assert np.array_equal(a, b) and np.array_equal(ga, gb)
+    assert ref.rays_traced() == r.rays_traced()
+    r.reset(cam)
+    r.set_tuning(partitions=parts)
+    r.add_samples(5)
+    assert np.array_equal(ref.image(), r.image()) and ref.rays_traced() == r.rays_traced()
