@@ -1,7 +1,9 @@
 """Build the native libraries of the path-tracing core in-tree.
 
 libptrace.so  -- HIP kernels + C ABI (include/pt_render.h), gfx950 only.
-libptscene.so -- host-only BVH builder and synthetic scenes (include/pt_scene.h).
+libptscene.so -- host-only BVH builders, OBJ/MTL and image import, PPM output and
+                 synthetic scenes (include/pt_scene.h).
+pt_render     -- the command-line renderer (the reference application's flags).
 
 Run as ``python cuda-path-tracer_amd/build.py``.  Every HIP translation unit is
 compiled without FP contraction (the reference builds with --fmad=false,
@@ -56,14 +58,25 @@ def build_ptrace():
 def build_scene():
     out = os.path.join(HERE, "libptscene.so")
     tmp = out + ".tmp"
-    _run(["g++", "-O3", "-DNDEBUG", "-std=c++17", "-fPIC", "-shared", "-Wall", "-Wextra",
-          "-Wno-unused-function", "-I", INCLUDE, os.path.join(CSRC, "pt_scene.cpp"), "-o", tmp])
+    sources = [os.path.join(CSRC, n) for n in ("pt_scene.cpp", "pt_obj.cpp", "pt_image.cpp")]
+    _run(["g++", "-O3", "-DNDEBUG", "-std=c++17", "-ffp-contract=off", "-fPIC", "-shared", "-Wall", "-Wextra",
+          "-Wno-unused-function", "-I", INCLUDE, *sources, "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
+def build_cli():
+    """pt_render: the command-line renderer over both libraries."""
+    out = os.path.join(HERE, "pt_render")
+    tmp = out + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-Wextra", "-I", INCLUDE, os.path.join(CSRC, "pt_cli.cpp"),
+          "-L", HERE, "-lptscene", "-lptrace", "-Wl,-rpath,$ORIGIN", "-o", tmp])
     os.replace(tmp, out)
     return out
 
 
 def build_all():
-    return [build_scene(), build_ptrace()]
+    return [build_scene(), build_ptrace(), build_cli()]
 
 
 if __name__ == "__main__":

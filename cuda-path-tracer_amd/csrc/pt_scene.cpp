@@ -2,6 +2,7 @@
 // basis, and seeded synthetic scenes (Cornell room, displaced icosphere,
 // clustered triangle soup).  Host-only C++; no device code.
 #include "pt_scene.h"
+#include "pt_scene_impl.h"
 
 #include <algorithm>
 #include <cmath>
@@ -13,70 +14,31 @@
 #include <unordered_map>
 #include <vector>
 
-struct pts_scene {
-    std::vector<pt_flat_triangle> triangles;
-    std::vector<pt_flat_sphere> spheres;
-    std::vector<pt_flat_node> nodes;
-    std::vector<pt_flat_material> materials;
-    std::vector<pt_flat_texture> textures;
-    std::vector<uint8_t> pixels;
-    int32_t root = -1, max_depth = 0;
-};
-
 namespace {
 
 thread_local std::string last_error;
 
-pt_vec3 v3(float x, float y, float z) { return pt_vec3{x, y, z}; }
-pt_vec3 operator+(pt_vec3 a, pt_vec3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
-pt_vec3 operator-(pt_vec3 a, pt_vec3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
-pt_vec3 operator*(float t, pt_vec3 v) { return v3(t * v.x, t * v.y, t * v.z); }
-float dot(pt_vec3 a, pt_vec3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-pt_vec3 cross(pt_vec3 a, pt_vec3 b) {
-    return v3(a.y * b.z - a.z * b.y, -(a.x * b.z - a.z * b.x), a.x * b.y - a.y * b.x);
-}
-float length(pt_vec3 v) { return std::sqrt(v.x * v.x + v.y * v.y + v.z * v.z); }
-pt_vec3 unit(pt_vec3 v) { float l = length(v); return v3(v.x / l, v.y / l, v.z / l); }
-float axis_of(pt_vec3 v, int a) { return a == 0 ? v.x : a == 1 ? v.y : v.z; }
+} // namespace
 
-struct box3 {
-    float lo[3], hi[3];
-    void empty() {
-        for(int a = 0; a < 3; a++) { lo[a] = std::numeric_limits<float>::infinity(); hi[a] = -lo[a]; }
-    }
-    void grow(const box3& b) {
-        for(int a = 0; a < 3; a++) { lo[a] = std::min(lo[a], b.lo[a]); hi[a] = std::max(hi[a], b.hi[a]); }
-    }
-    void grow(const float p[3]) {
-        for(int a = 0; a < 3; a++) { lo[a] = std::min(lo[a], p[a]); hi[a] = std::max(hi[a], p[a]); }
-    }
-    double area() const {
-        double dx = std::max(0.0f, hi[0] - lo[0]), dy = std::max(0.0f, hi[1] - lo[1]), dz = std::max(0.0f, hi[2] - lo[2]);
-        return 2.0 * (dx * dy + dy * dz + dz * dx);
-    }
-};
+void pts::set_error(const std::string& message) { last_error = message; }
 
-// Leaf bounds exactly as the reference computes them.
-box3 triangle_bounds(const pt_flat_triangle& t) {
-    // src/geometry/triangle.h:96-109: min/max per axis, padded by 1e-4 and
-    // pushed one ulp further outwards.
-    box3 b;
-    const float inf = std::numeric_limits<float>::infinity();
-    for(int a = 0; a < 3; a++) {
-        float x0 = axis_of(t.v0, a), x1 = axis_of(t.v1, a), x2 = axis_of(t.v2, a);
-        float mn = x0 < (x1 < x2 ? x1 : x2) ? x0 : (x1 < x2 ? x1 : x2);
-        float mx = x0 > (x1 > x2 ? x1 : x2) ? x0 : (x1 > x2 ? x1 : x2);
-        b.lo[a] = std::nextafter(mn - 0.0001f, -inf);
-        b.hi[a] = std::nextafter(mx + 0.0001f, inf);
-    }
-    return b;
+pt_flat_texture pts::constant_texture(pt_vec3 color) {
+    // flat_export.h:30: constant record defaults.
+    pt_flat_texture t{};
+    t.color = color; t.scale = v3(1, 1, 0); t.offset = v3(0, 0, 0);
+    t.type = PT_CONSTANT; t.image_id = -1; t.clamp = 1;
+    return t;
 }
-box3 sphere_bounds(const pt_flat_sphere& s) {
-    // src/geometry/sphere.h:69-75
-    box3 b;
-    for(int a = 0; a < 3; a++) { b.lo[a] = axis_of(s.center, a) - s.radius; b.hi[a] = axis_of(s.center, a) + s.radius; }
-    return b;
+
+pt_flat_material pts::flat_material(int32_t type, pt_vec3 albedo, float fuzz, float ref_idx, int32_t texture) {
+    pt_flat_material m{};
+    m.albedo = albedo; m.fuzz = fuzz; m.ref_idx = ref_idx; m.type = type; m.texture_id = texture;
+    return m;
 }
+
+namespace {
+
+using namespace pts;
 
 struct prim_ref {
     box3 bounds;
@@ -225,16 +187,11 @@ void build_into(pts_scene& s, const std::vector<pt_flat_triangle>& tris, const s
 }
 
 int32_t add_constant(pts_scene& s, pt_vec3 color) {
-    pt_flat_texture t{};
-    t.color = color; t.scale = v3(1, 1, 0); t.offset = v3(0, 0, 0);
-    t.type = PT_CONSTANT; t.image_id = -1; t.clamp = 1;
-    s.textures.push_back(t);
+    s.textures.push_back(constant_texture(color));
     return int32_t(s.textures.size() - 1);
 }
 int32_t add_material(pts_scene& s, int32_t type, pt_vec3 albedo, float fuzz, float ref_idx, int32_t texture) {
-    pt_flat_material m{};
-    m.albedo = albedo; m.fuzz = fuzz; m.ref_idx = ref_idx; m.type = type; m.texture_id = texture;
-    s.materials.push_back(m);
+    s.materials.push_back(flat_material(type, albedo, fuzz, ref_idx, texture));
     return int32_t(s.materials.size() - 1);
 }
 int32_t diffuse(pts_scene& s, pt_vec3 c) { return add_material(s, PT_DIFFUSE, v3(0, 0, 0), 0, 1, add_constant(s, c)); }
@@ -297,9 +254,16 @@ pt_flat_camera make_camera(pt_vec3 from, pt_vec3 at, pt_vec3 up, float vfov, flo
 
 } // namespace
 
+void pts::build_scene(pts_scene& s, const std::vector<pt_flat_triangle>& tris, const std::vector<pt_flat_sphere>& sph,
+                      int split) {
+    build_into(s, tris, sph, split);
+}
+
 extern "C" {
 
 const char *pt_scene_last_error(void) { return last_error.c_str(); }
+
+const char *pts_warnings(const pts_scene *scene) { return scene ? scene->warnings.c_str() : ""; }
 
 void pts_camera(const float lookfrom[3], const float lookat[3], const float vup[3], float vfov, float aspect,
                 float aperture, float focus_dist, float time0, float time1, pt_flat_camera *out) {

@@ -186,6 +186,17 @@ def scene_lib():
         h.pts_free.argtypes = [ctypes.c_void_p]
         h.pts_camera.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_float] * 6 + [ctypes.c_void_p]
         h.pt_scene_last_error.restype = ctypes.c_char_p
+        h.pts_build_seeded.restype = ctypes.c_void_p
+        h.pts_build_seeded.argtypes = h.pts_build.argtypes + [ctypes.c_int64]
+        h.pts_load_obj.restype = ctypes.c_void_p
+        h.pts_load_obj.argtypes = [ctypes.c_char_p, ctypes.c_void_p, ctypes.c_void_p]
+        h.pts_warnings.restype = ctypes.c_char_p
+        h.pts_warnings.argtypes = [ctypes.c_void_p]
+        h.pts_load_image.argtypes = [ctypes.c_char_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        h.pts_load_image.restype = ctypes.c_int32
+        h.pts_free_image.argtypes = [ctypes.c_void_p]
+        h.pts_write_ppm.argtypes = [ctypes.c_char_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]
+        h.pts_write_ppm.restype = ctypes.c_int32
         _SCENE_LIB = h
     return _SCENE_LIB
 
@@ -335,10 +346,12 @@ def query_rays(scene, rays, brute=False, stats=False):
 
 
 # ---------------------------------------------------------------- scenes
-def _take_scene(handle):
+def _take_scene(handle, warnings=None):
     L = scene_lib()
     if not handle:
         raise InvalidArgument(L.pt_scene_last_error().decode())
+    if warnings is not None:
+        warnings.append(L.pts_warnings(ctypes.c_void_p(handle)).decode())
     counts = (ctypes.c_int64 * 6)()
     rd = (ctypes.c_int32 * 2)()
     L.pts_counts(ctypes.c_void_p(handle), counts, rd)
@@ -371,16 +384,73 @@ def soup(triangles, seed=7, width=640, height=480):
     return _take_scene(h), cam
 
 
-def build_scene(triangles=None, spheres=None, materials=None, textures=None, pixels=None, split="sah"):
-    """BVH over the given primitives in the flat format (leaf-ordered)."""
+SPLITS = {"sah": 0, "median": 1, "reference": 2}
+
+OBJ_OPTIONS = np.dtype([("scale", "<f4", 3), ("rotate", "<f4", 3), ("translate", "<f4", 3), ("aspect", "<f4"),
+                        ("split", "<i4"), ("seed", "<i8")], align=True)
+OBJ_INFO = np.dtype([("lookfrom", "<f4", 3), ("lookat", "<f4", 3), ("focus_dist", "<f4"),
+                     ("source_vertices", "<i8"), ("shape_count", "<i8"), ("mesh_triangles", "<i8")], align=True)
+
+
+def build_scene(triangles=None, spheres=None, materials=None, textures=None, pixels=None, split="sah", seed=0):
+    """BVH over the given primitives in the flat format (leaf-ordered).
+
+    split "reference" rebuilds the reference's own hierarchy (random-axis
+    median split driven by drand48 after srand48(seed), src/bvh/bvh.h:132-171)
+    and numbers materials and textures as its exporter does."""
     t = np.ascontiguousarray(triangles if triangles is not None else np.zeros(0, TRIANGLE), TRIANGLE)
     s = np.ascontiguousarray(spheres if spheres is not None else np.zeros(0, SPHERE), SPHERE)
     m = np.ascontiguousarray(materials if materials is not None else np.zeros(0, MATERIAL), MATERIAL)
     x = np.ascontiguousarray(textures if textures is not None else np.zeros(0, TEXTURE), TEXTURE)
     p = np.ascontiguousarray(pixels if pixels is not None else np.zeros(0, np.uint8), np.uint8)
-    h = scene_lib().pts_build(_ptr(t), len(t), _ptr(s), len(s), _ptr(m), len(m), _ptr(x), len(x), _ptr(p), len(p),
-                              0 if split == "sah" else 1)
+    h = scene_lib().pts_build_seeded(_ptr(t), len(t), _ptr(s), len(s), _ptr(m), len(m), _ptr(x), len(x), _ptr(p),
+                                     len(p), SPLITS[split], int(seed))
     return _take_scene(h)
+
+
+def load_obj(path, aspect, scale=(1, 1, 1), rotate=(0, 0, 0), translate=(0, 0, 0), split="reference", seed=7,
+             vfov=40.0, aperture=0.0, time0=0.0, time1=1.0):
+    """Import an OBJ/MTL asset staged as the reference's obj_scene
+    (src/scene/obj_scene.h:10-31): returns (scene, camera, info) with info a
+    dict of lookfrom, lookat, focus_dist, source_vertices, shape_count,
+    mesh_triangles and the loader's warnings.  With split "reference" the
+    arrays equal the reference's srand48(seed); obj_scene; flatten_scene."""
+    opt = np.zeros((), OBJ_OPTIONS)
+    opt["scale"], opt["rotate"], opt["translate"] = scale, rotate, translate
+    opt["aspect"], opt["split"], opt["seed"] = aspect, SPLITS[split], int(seed)
+    info = np.zeros((), OBJ_INFO)
+    h = scene_lib().pts_load_obj(os.fsencode(path), opt.ctypes.data, info.ctypes.data)
+    warnings = []
+    scene = _take_scene(h, warnings)
+    camera = make_camera(info["lookfrom"], info["lookat"], (0, 1, 0), vfov, aspect, aperture, float(info["focus_dist"]),
+                         time0, time1)
+    out = {k: (info[k].copy() if info[k].ndim else info[k].item()) for k in OBJ_INFO.names}
+    out["warnings"] = warnings[0]
+    return scene, camera, out
+
+
+def load_image(path):
+    """Decode a PNG or binary PNM texture to an (height, width, 3) uint8 array."""
+    L = scene_lib()
+    w, h, buf = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_void_p()
+    if L.pts_load_image(os.fsencode(path), ctypes.byref(w), ctypes.byref(h), ctypes.byref(buf)) != 0:
+        raise RuntimeError(L.pt_scene_last_error().decode())
+    try:
+        data = np.ctypeslib.as_array(ctypes.cast(buf, ctypes.POINTER(ctypes.c_uint8)), (h.value, w.value, 3)).copy()
+    finally:
+        L.pts_free_image(buf)
+    return data
+
+
+def write_ppm(path, image, width, height):
+    """Write a linear RGB image (row 0 at the bottom) as the reference
+    application's ASCII PPM; path "-" is stdout."""
+    rgb = np.ascontiguousarray(image, np.float32).reshape(-1)
+    if rgb.size != 3 * width * height:
+        raise InvalidArgument("image size does not match width*height")
+    L = scene_lib()
+    if L.pts_write_ppm(os.fsencode(path), rgb.ctypes.data, int(width), int(height)) != 0:
+        raise RuntimeError(L.pt_scene_last_error().decode())
 
 
 def make_camera(lookfrom, lookat, vup, vfov, aspect, aperture, focus_dist, time0=0.0, time1=1.0):

@@ -153,6 +153,9 @@ def ref():
                                       ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                       ctypes.c_void_p, ctypes.c_void_p]
         L.ref_display_pixels.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]
+        L.ref_build_flat.restype = ctypes.c_void_p
+        L.ref_build_flat.argtypes = [ctypes.c_void_p, ctypes.c_long]
+        L.ref_load_image.argtypes = [ctypes.c_char_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long]
         L.ref_load_obj.restype = ctypes.c_void_p
         L.ref_load_obj.argtypes = [ctypes.c_char_p, ctypes.c_float, ctypes.c_long, ctypes.c_void_p, ctypes.c_void_p,
                                    ctypes.c_void_p, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float]

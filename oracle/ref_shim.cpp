@@ -5,7 +5,9 @@
 // oracle/_ref/libptref.so.  It is used to pin the C oracle bit-for-bit, to
 // export golden fixtures, and optionally as the "reference" CPU baseline.
 // Nothing here is part of the product.
+#include <cstdio>
 #include <cstring>
+#include <memory>
 #include <stdexcept>
 #include <thread>
 #include <vector>
@@ -249,5 +251,83 @@ void ref_scene_copy(void *h, flat_triangle *t, flat_sphere *s, flat_node *n, fla
 }
 
 void ref_scene_free(void *h) { delete static_cast<ref_scene_out*>(h); }
+
+// The reference's random-axis BVH (bvh_node, srand48-seeded) over
+// [triangles..., spheres...] of a flat input, flattened by its exporter.
+// Materials become the reference's material objects over constant, image
+// and OBJ diffuse-map textures.
+void *ref_build_flat(const ref_scene_in *in, long srand_seed) {
+    try {
+        auto *out = new ref_scene_out;
+        static std::vector<std::unique_ptr<texture>> textures;
+        static std::vector<std::unique_ptr<material>> materials;
+        static std::vector<std::unique_ptr<hitable>> objects;
+        static std::vector<std::vector<unsigned char>> images;
+        textures.clear(); materials.clear(); objects.clear(); images.clear();
+        std::vector<texture*> tex(size_t(in->texture_count), nullptr);
+        for(int pass = 0; pass < 2; pass++) {
+            for(int64_t i = 0; i < in->texture_count; i++) {
+                const flat_texture& t = in->textures[i];
+                if((pass == 0) != (t.type != FLAT_MAP)) continue;
+                if(t.type == FLAT_CONSTANT) textures.push_back(std::make_unique<constant_texture>(t.color));
+                else if(t.type == FLAT_IMAGE) {
+                    images.emplace_back(in->pixels + t.byte_offset, in->pixels + t.byte_offset + size_t(t.width) * t.height * 3);
+                    textures.push_back(std::make_unique<image_texture>(images.back().data(), t.width, t.height));
+                } else {
+                    tinyobj::texture_option_t opt{};
+                    opt.scale[0] = t.scale.x(); opt.scale[1] = t.scale.y(); opt.scale[2] = 1;
+                    opt.origin_offset[0] = t.offset.x(); opt.origin_offset[1] = t.offset.y(); opt.origin_offset[2] = 0;
+                    opt.clamp = t.clamp != 0;
+                    textures.push_back(std::make_unique<obj_detail::diffuse_map>(tex[size_t(t.image_id)], t.color, opt));
+                }
+                tex[size_t(i)] = textures.back().get();
+            }
+        }
+        std::vector<material*> mats;
+        for(int64_t i = 0; i < in->material_count; i++) {
+            const flat_material& m = in->materials[i];
+            if(m.type == FLAT_DIFFUSE) materials.push_back(std::make_unique<lambertian>(tex[size_t(m.texture_id)]));
+            else if(m.type == FLAT_METAL) materials.push_back(std::make_unique<metal>(m.albedo, m.fuzz));
+            else if(m.type == FLAT_GLASS) materials.push_back(std::make_unique<dielectric>(m.ref_idx));
+            else materials.push_back(std::make_unique<diffuse_light>(tex[size_t(m.texture_id)]));
+            mats.push_back(materials.back().get());
+        }
+        std::vector<hitable*> list;
+        for(int64_t i = 0; i < in->triangle_count; i++) {
+            const flat_triangle& f = in->triangles[i];
+            auto t = std::make_unique<triangle>(f.v0, f.v1, f.v2, mats[size_t(f.material_id)]);
+            t->n0 = f.n0; t->n1 = f.n1; t->n2 = f.n2; t->smooth = f.smooth != 0;
+            t->uv0 = f.uv0; t->uv1 = f.uv1; t->uv2 = f.uv2;
+            list.push_back(t.get());
+            objects.push_back(std::move(t));
+        }
+        for(int64_t i = 0; i < in->sphere_count; i++) {
+            const flat_sphere& s = in->spheres[i];
+            auto o = std::make_unique<sphere>(s.center, s.radius, mats[size_t(s.material_id)]);
+            list.push_back(o.get());
+            objects.push_back(std::move(o));
+        }
+        srand48(srand_seed);
+        bvh_node root(list.data(), int(list.size()), 0, 1);
+        out->scene = flatten_scene(&root);
+        out->camera = flat_camera{};
+        return out;
+    } catch(const std::exception& e) {
+        fprintf(stderr, "ref_build_flat: %s\n", e.what());
+        return nullptr;
+    }
+}
+
+// stb_image as the reference loads textures (texture.h:65-72): 8-bit RGB.
+int ref_load_image(const char *path, int *w, int *h, unsigned char *out, long capacity) {
+    int channels = 0;
+    unsigned char *data = stbi_load(path, w, h, &channels, STBI_rgb);
+    if(!data) return -1;
+    long n = long(*w) * long(*h) * 3;
+    if(n <= capacity) std::memcpy(out, data, size_t(n));
+    stbi_image_free(data);
+    return n <= capacity ? 0 : 1;
+}
+
 
 }

@@ -1,0 +1,127 @@
+"""The command-line renderer (pt_render), a drop-in for the reference's
+application (src/app/main.cpp, src/app/render_options.h:49-107): flags,
+messages, exit codes, scene staging and PPM output."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+import ptrace
+from conftest import GOLDEN, ROOT
+
+CLI = os.path.join(ROOT, "cuda-path-tracer_amd", "pt_render")
+ASSETS = os.path.join(GOLDEN, "assets", "models")
+
+
+def run(*args, check=False):
+    p = subprocess.run([CLI, *map(str, args)], capture_output=True, text=True, timeout=300)
+    if check and p.returncode != 0:
+        raise AssertionError(p.stderr)
+    return p
+
+
+def test_help_lists_the_reference_flags():
+    p = run("--help")
+    assert p.returncode == 0
+    for flag in ("--obj FILE", "--width N", "--height N", "--samples N", "--batch-samples N", "--seed N",
+                 "--obj-scale X Y Z", "--obj-rotate X Y Z", "--obj-translate X Y Z", "--normals",
+                 "shaded|normals|triangle-id|material-id|bvh|intersections", "--lighting classic|mis"):
+        assert flag in p.stdout
+
+
+@pytest.mark.parametrize("args,message", [
+    (["--width", "0"], "invalid integer: 0"),
+    (["--samples", "12x"], "invalid integer: 12x"),
+    (["--seed", "-1"], "invalid integer: -1"),
+    (["--obj-scale", "1", "1", "1"], "OBJ transforms require --obj FILE"),
+    (["--obj", "a.obj", "--obj-scale", "1", "0", "1"], "mesh transform requires finite values and nonzero scale"),
+    (["--obj", "a.obj", "--obj-rotate", "1", "inf", "1"], "invalid finite number: inf"),
+    (["--obj-translate", "1", "2"], "expected three values for --obj-translate"),
+    (["--lighting", "bright"], "lighting must be classic or mis"),
+    (["--debug", "uv"], "unknown debug mode: uv"),
+    (["--backend", "opencl"], "HIP is the only rendering backend"),
+    (["--obj", ""], "OBJ filename must not be empty"),
+    (["--width"], "missing value for --width"),
+    (["--frobnicate"], "unknown option: --frobnicate"),
+])
+def test_argument_errors_exit_1_with_the_reference_message(args, message):
+    p = run(*args)
+    assert p.returncode == 1
+    assert p.stderr == message + "\nUse --help for usage.\n"
+
+
+def test_missing_device_exits_77():
+    ok, _ = ptrace.device_available()
+    if ok:
+        pytest.skip("a HIP device is present")
+    p = run("--samples", "1")
+    assert p.returncode == 77 and p.stderr.startswith("HIP device unavailable: ")
+
+
+def _ppm(text):
+    lines = text.split("\n")
+    w, h = map(int, lines[1].split())
+    vals = np.array([int(x) for l in lines[3:] if l for x in l.split()], np.int32)
+    return w, h, vals.reshape(h, w, 3)
+
+
+def _oracle_ppm(scene, cam, w, h, seed, samples, debug, direct):
+    o = oracle.Renderer(scene, cam, w, h, seed, False, debug, direct, 50)
+    o.add_samples(samples, threads=min(8, os.cpu_count() or 1))
+    img = o.image().reshape(h, w, 3)[::-1]
+    ch = np.vectorize(lambda v: 0 if not v > 0 else 255 if v >= 1 else int(255.99 * float(np.sqrt(np.float32(v)))))
+    return ch(img.astype(np.float32)).astype(np.int32)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("debug,name", [(2, "triangle-id"), (3, "material-id"), (4, "bvh"), (5, "intersections"),
+                                        (1, "normals"), (0, "shaded")])
+def test_cli_render_matches_oracle(debug, name):
+    # The application's pipeline end to end on the gallery asset: import,
+    # reference-order BVH, render, PPM; against the oracle on the same arrays.
+    obj = os.path.join(ASSETS, "gallery", "gallery.obj")
+    w, h, samples, seed = 48, 32, 4 if debug == 0 else 1, 5
+    p = run("--obj", obj, "--width", w, "--height", h, "--samples", samples, "--seed", seed, "--debug", name,
+            check=True)
+    assert "Loaded 472 triangles" in p.stderr and f"Rendered {w}x{h}, {samples} samples/pixel" in p.stderr
+    _, _, got = _ppm(p.stdout)
+    scene, cam, _ = ptrace.load_obj(obj, w / h, seed=seed)
+    want = _oracle_ppm(scene, cam, w, h, seed, samples, debug, True)
+    if debug != 0:
+        assert np.array_equal(got, want)
+    else:
+        assert (got == want).all(axis=2).mean() >= 0.99
+
+
+@pytest.mark.gpu
+def test_cli_batch_split_is_byte_identical(tmp_path):
+    # The reference's tests/render_tests.py:24 property: the PPM does not
+    # depend on how the samples are split into batches.
+    obj = os.path.join(ASSETS, "cube", "cube.obj")
+    outs = []
+    for batch in (1, 3, 7):
+        path = tmp_path / f"b{batch}.ppm"
+        run("--obj", obj, "--width", 40, "--height", 30, "--samples", 7, "--batch-samples", batch, "--output", path,
+            check=True)
+        outs.append(path.read_bytes())
+    assert outs[0] == outs[1] == outs[2]
+    p = run("--obj", obj, "--width", 40, "--height", 30, "--samples", 7, "--batch-samples", 7, check=True)
+    assert p.stdout.encode() == outs[0]
+
+
+@pytest.mark.gpu
+def test_cli_builtin_scene_and_sah_hierarchy():
+    p = run("--scene", "cornell", "--width", 32, "--height", 20, "--samples", 2, check=True)
+    w, h, img = _ppm(p.stdout)
+    assert (w, h) == (32, 20) and img.max() > 0
+    obj = os.path.join(ASSETS, "gallery", "gallery.obj")
+    a = run("--obj", obj, "--width", 24, "--height", 16, "--samples", 1, "--debug", "material-id", check=True)
+    b = run("--obj", obj, "--width", 24, "--height", 16, "--samples", 1, "--debug", "normals", "--bvh", "sah",
+            check=True)
+    c = run("--obj", obj, "--width", 24, "--height", 16, "--samples", 1, "--debug", "normals", check=True)
+    assert a.returncode == 0
+    # Same closest hits up to exact ties between triangles, which the two
+    # hierarchies may break differently.
+    assert (_ppm(b.stdout)[2] == _ppm(c.stdout)[2]).all(axis=2).mean() >= 0.99
