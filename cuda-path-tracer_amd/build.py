@@ -1,6 +1,6 @@
 """Build the native libraries of the path-tracing core in-tree.
 
-libptrace.so  -- HIP kernels + C ABI (include/pt_render.h), gfx950 only.
+libptrace.so  -- HIP kernels + C ABI (include/pt_render.h, include/pt_denoise.h), gfx950 only.
 libptscene.so -- host-only BVH builders, OBJ/MTL and image import, PPM output and
                  synthetic scenes (include/pt_scene.h).
 pt_render     -- the command-line renderer (the reference application's flags).
@@ -44,7 +44,7 @@ def _run(cmd, cwd=None):
 def build_ptrace():
     out = os.path.join(HERE, "libptrace.so")
     objs = []
-    for name in ("pt_kernels.hip", "pt_api.hip"):
+    for name in ("pt_kernels.hip", "pt_api.hip", "pt_denoise.hip"):
         obj = os.path.join(HERE, "_build", name + ".o")
         os.makedirs(os.path.dirname(obj), exist_ok=True)
         _run([HIPCC, *HIP_FLAGS, "-I", INCLUDE, "-I", CSRC, "-c", os.path.join(CSRC, name), "-o", obj])

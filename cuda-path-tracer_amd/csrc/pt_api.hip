@@ -624,6 +624,7 @@ struct pt_renderer {
     dev_buf<int2> ctr;
     dev_buf<int4> hit;
     std::vector<std::unique_ptr<partition>> parts;
+    dev_buf<float> aov;                  // first-hit albedo and normal sums (6 per pixel), when enabled
     pt_tuning tuning{128, 256, 16, 8, 0, 0, 0, 262144, 48, 1};
     int cus = 1;
     pt_launch_stats last{};
@@ -641,6 +642,7 @@ struct pt_renderer {
         validate_camera(camera);
         pt::launch_init_pixels(sum.get(), rng.get(), int64_t(pixels), seed, ctl.get(), stream);
         for(auto& part : parts) pt::launch_clear_ctl(part->ctl.get(), stream);
+        if(aov.size()) HIP_CHECK(hipMemsetAsync(aov.get(), 0, aov.size() * sizeof(float), stream));
         HIP_CHECK(hipGetLastError());
         HIP_CHECK(hipStreamSynchronize(stream));
         frame.camera = camera; samples = 0; milliseconds = 0; rays = 0;
@@ -698,7 +700,7 @@ struct pt_renderer {
             p.ray_o = ray_o.get(); p.ray_d = ray_d.get(); p.thr = thr.get(); p.rad = rad.get();
             p.ctr = ctr.get(); p.hit = hit.get();
             p.sh_o = sh_o.get(); p.sh_d = sh_d.get(); p.sh_c = sh_c.get();
-            p.rng = rng.get(); p.sum = sum.get();
+            p.rng = rng.get(); p.sum = sum.get(); p.aov = aov.get();
             p.queue[0] = part->queue0.get(); p.queue[1] = part->queue1.get();
             p.shadow_queue = part->shadow_queue.get(); p.finish_queue = part->finish_queue.get();
             parts.push_back(std::move(part));
@@ -845,6 +847,17 @@ struct pt_renderer {
         rays = total_rays;
     }
 };
+
+bool pt::renderer_frame(const pt_renderer *r, pt::frame_view *out) {
+    if(!r || !out) return false;
+    out->sum = r->sum.get();
+    out->aov = r->aov.size() ? r->aov.get() : nullptr;
+    out->width = r->frame.width;
+    out->height = r->frame.height;
+    out->samples = r->samples;
+    out->stream = r->stream;
+    return true;
+}
 
 namespace {
 
@@ -1102,6 +1115,40 @@ pt_status pt_renderer_iteration_log(const pt_renderer *renderer, uint32_t *exten
             }
         }
         *count = total;
+    });
+}
+
+pt_status pt_renderer_enable_aovs(pt_renderer *renderer, int32_t enable) {
+    return guarded([&] {
+        if(!renderer) fail(PT_EINVAL, "null renderer");
+        if(renderer->frame.debug != PT_DEBUG_SHADED) fail(PT_EINVAL, "feature buffers need the shaded mode");
+        if(renderer->samples != 0) fail(PT_EINVAL, "feature buffers must be enabled before samples are added");
+        HIP_CHECK(hipStreamSynchronize(renderer->stream));
+        if(enable) {
+            if(!renderer->aov.size()) renderer->aov.alloc(6 * renderer->pixels);
+            HIP_CHECK(hipMemset(renderer->aov.get(), 0, renderer->aov.size() * sizeof(float)));
+        } else {
+            renderer->aov.release();
+        }
+        for(auto& part : renderer->parts) part->paths.aov = renderer->aov.get();
+    });
+}
+
+pt_status pt_renderer_read_aovs(const pt_renderer *renderer, float *albedo, float *normal, size_t count) {
+    return guarded([&] {
+        if(!renderer) fail(PT_EINVAL, "null renderer");
+        if(!renderer->aov.size()) fail(PT_EINVAL, "feature buffers are not enabled");
+        if(count != 3 * renderer->pixels) fail(PT_EINVAL, "feature buffer size mismatch");
+        if(renderer->samples <= 0) fail(PT_EINVAL, "no samples accumulated");
+        std::vector<float> host(renderer->aov.size());
+        HIP_CHECK(hipMemcpy(host.data(), renderer->aov.get(), host.size() * sizeof(float), hipMemcpyDeviceToHost));
+        const float k = float(1.0 / double(renderer->samples));
+        for(size_t i = 0; i < renderer->pixels; i++) {
+            for(int c = 0; c < 3; c++) {
+                if(albedo) albedo[3 * i + size_t(c)] = host[6 * i + size_t(c)] * k;
+                if(normal) normal[3 * i + size_t(c)] = host[6 * i + 3 + size_t(c)] * k;
+            }
+        }
     });
 }
 

@@ -86,6 +86,7 @@ struct paths_dev {
     int32_t *shadow_queue;
     int32_t *finish_queue; // paths whose sample ended this bounce
     int2 *spill;        // wide traversal stack entries beyond wide_lds_entries
+    float *aov;         // optional: 6 floats per pixel, first-hit albedo.xyz and normal.xyz summed over samples
 };
 
 // Device control block of the bounce loop.
@@ -158,5 +159,14 @@ inline size_t trace_lds_bytes(const scene_dev& s, const frame_params& f, int thr
     bool wide = !f.brute && s.wroot_ref != empty_root && !f.exact_traversal;
     return wide ? std::max(exact, wide_stack_bytes(s.wide_stack_capacity, threads)) : exact;
 }
+
+// Device view of a renderer's accumulation for the denoiser (same library).
+struct frame_view {
+    const float *sum;   // 3 floats per pixel
+    const float *aov;   // 6 floats per pixel or nullptr
+    int32_t width, height, samples;
+    hipStream_t stream;
+};
+bool renderer_frame(const pt_renderer *renderer, frame_view *out);
 
 } // namespace pt

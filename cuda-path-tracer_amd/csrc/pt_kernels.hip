@@ -1060,17 +1060,34 @@ __device__ __forceinline__ void store_path(const paths_dev& p, int32_t pix, cons
     p.ctr[pix] = make_int2(st.depth, st.left);
 }
 
+// First-hit feature buffers for the denoiser (no counterpart in the
+// reference): the surface albedo and shading normal of the camera ray's hit,
+// summed per pixel.  Reads no random numbers, so radiance is unchanged.
+__device__ void record_aov(const scene_dev& s, const hit_rec& rec, const pt_flat_material& mat, float *aov) {
+    v3 albedo;
+    if(mat.type == PT_DIFFUSE) albedo = trace_texture(s, mat.texture_id, rec.u, rec.v);
+    else if(mat.type == PT_METAL) albedo = mk(mat.albedo);
+    else if(mat.type == PT_GLASS) albedo = mk(1, 1, 1);
+    else {
+        v3 e = trace_texture(s, mat.texture_id, rec.u, rec.v);
+        albedo = mk(fminf(e.x, 1.0f), fminf(e.y, 1.0f), fminf(e.z, 1.0f));
+    }
+    aov[0] = aov[0] + albedo.x; aov[1] = aov[1] + albedo.y; aov[2] = aov[2] + albedo.z;
+    aov[3] = aov[3] + rec.normal.x; aov[4] = aov[4] + rec.normal.y; aov[5] = aov[5] + rec.normal.z;
+}
+
 // One iteration of trace_color's loop after its closest-hit query
 // (transport.h:184-227).  Returns true when the path continues with st.o/st.d
 // as the next ray, false when the sample has ended with st.radiance.  A next
 // event estimation query is returned in `sq` (has_shadow); its contribution
 // must be added to st.radiance if unblocked, before the next bounce.
 __device__ bool shade_bounce(const scene_dev& s, const frame_params& f, path_state& st, bool hit, const best_hit& b,
-                             rng_t& r, shadow_query& sq, bool& has_shadow) {
+                             rng_t& r, shadow_query& sq, bool& has_shadow, float *aov) {
     has_shadow = false;
     if(!hit) return false;
     hit_rec rec = make_hit(s, st.o, st.d, b);
     const pt_flat_material mat = s.materials[rec.material_id];
+    if(aov && st.depth == 0) record_aov(s, rec, mat, aov);
     if(mat.type == PT_LIGHT) {
         float weight = st.previous_pdf > 0 ? mis_weight(st.previous_pdf, hit_light_pdf(s, st.o, rec)) : 1;
         st.radiance = add(st.radiance, mul(scale(weight, st.throughput), trace_texture(s, mat.texture_id, rec.u, rec.v)));
@@ -1131,7 +1148,7 @@ __global__ __launch_bounds__(1024) void shade_kernel(scene_dev s, frame_params f
             rng_t r = pixel_rng(p.rng[pix], uint64_t(pix));
             const best_hit b{__int_as_float(hb.x), __int_as_float(hb.y), __int_as_float(hb.z), hb.w};
             shadow_query sq;
-            cont = shade_bounce(s, f, st, hb.w >= 0, b, r, sq, shadow);
+            cont = shade_bounce(s, f, st, hb.w >= 0, b, r, sq, shadow, p.aov ? p.aov + 6 * size_t(pix) : nullptr);
             if(shadow) {
                 p.sh_o[pix] = make_float4(sq.o.x, sq.o.y, sq.o.z, sq.t_max);
                 p.sh_d[pix] = make_float4(sq.d.x, sq.d.y, sq.d.z, 0);
@@ -1215,7 +1232,7 @@ __global__ __launch_bounds__(256) void drain_kernel(scene_dev s, frame_params f,
             ext++;
             shadow_query sq;
             bool has_shadow;
-            const bool cont = shade_bounce(s, f, st, hit, b, r, sq, has_shadow);
+            const bool cont = shade_bounce(s, f, st, hit, b, r, sq, has_shadow, p.aov ? p.aov + 6 * size_t(pix) : nullptr);
             if(has_shadow) {
                 shadow++;
                 best_hit bb;

@@ -1,0 +1,296 @@
+"""Learned denoiser over the progressive render (include/pt_denoise.h).
+
+Python side of the BF16 MFMA encoder-decoder in libptrace.so: the network
+handle, training-set generation with this renderer (low-spp radiance plus
+first-hit albedo and normal, against a high-spp target of the same view),
+training, and denoising a renderer's accumulation on the device.
+
+    python cuda-path-tracer_amd/denoise.py train --out weights.npz
+    python cuda-path-tracer_amd/denoise.py evaluate --weights weights.npz
+
+The weights trained by the project ship as ``denoiser_weights.npz`` beside
+this file.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+if HERE not in sys.path:
+    sys.path.insert(0, HERE)
+
+import ptrace  # noqa: E402
+
+WEIGHTS_PATH = os.path.join(HERE, "denoiser_weights.npz")
+
+PTD_FUNCTIONS = {
+    "ptd_last_error": (ctypes.c_char_p, []),
+    "ptd_parameter_count": (ctypes.c_int64, []),
+    "ptd_create": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_void_p]),
+    "ptd_destroy": (None, [ctypes.c_void_p]),
+    "ptd_get_parameters": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]),
+    "ptd_set_parameters": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]),
+    "ptd_forward": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                   ctypes.c_void_p]),
+    "ptd_loss_and_gradients": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
+                                              ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p,
+                                              ctypes.c_int64]),
+    "ptd_set_training_data": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
+                                             ctypes.c_int32, ctypes.c_int32]),
+    "ptd_train": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_float,
+                                 ctypes.c_void_p, ctypes.c_void_p]),
+    "ptd_denoise_renderer": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+}
+
+_READY = False
+
+
+def lib():
+    global _READY
+    L = ptrace.lib()
+    if not _READY:
+        for name, (res, args) in PTD_FUNCTIONS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _READY = True
+    return L
+
+
+def _check(status):
+    if status == 0:
+        return
+    msg = lib().ptd_last_error().decode()
+    if status == ptrace.PT_EINVAL:
+        raise ptrace.InvalidArgument(msg)
+    raise RuntimeError(msg)
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, np.float32)
+
+
+class Denoiser:
+    """The network on the device (FP32 master weights, bf16 compute)."""
+
+    def __init__(self, seed=1, weights=None):
+        h = ctypes.c_void_p()
+        _check(lib().ptd_create(int(seed), ctypes.byref(h)))
+        self._h = h
+        if weights is not None:
+            self.set_parameters(np.load(weights)["parameters"] if isinstance(weights, str) else weights)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().ptd_destroy(self._h)
+            self._h = None
+
+    @staticmethod
+    def parameter_count():
+        return int(lib().ptd_parameter_count())
+
+    def parameters(self):
+        p = np.zeros(self.parameter_count(), np.float32)
+        _check(lib().ptd_get_parameters(self._h, p.ctypes.data, p.size))
+        return p
+
+    def set_parameters(self, params):
+        p = _f32(params)
+        _check(lib().ptd_set_parameters(self._h, p.ctypes.data, p.size))
+
+    def save(self, path, **meta):
+        np.savez_compressed(path, parameters=self.parameters(), **meta)
+
+    def forward(self, features):
+        """Denoised radiance (n,h,w,3) of features (n,h,w,9)."""
+        f = _f32(features)
+        n, h, w, _ = f.shape
+        out = np.zeros((n, h, w, 3), np.float32)
+        _check(lib().ptd_forward(self._h, f.ctypes.data, n, h, w, out.ctypes.data))
+        return out
+
+    def loss_and_gradients(self, features, targets):
+        f, t = _f32(features), _f32(targets)
+        n, h, w, _ = f.shape
+        loss = ctypes.c_float()
+        g = np.zeros(self.parameter_count(), np.float32)
+        _check(lib().ptd_loss_and_gradients(self._h, f.ctypes.data, t.ctypes.data, n, h, w, ctypes.byref(loss),
+                                            g.ctypes.data, g.size))
+        return float(loss.value), g
+
+    def set_training_data(self, features, targets):
+        f, t = _f32(features), _f32(targets)
+        crops, h, w, _ = f.shape
+        _check(lib().ptd_set_training_data(self._h, f.ctypes.data, t.ctypes.data, crops, h, w))
+
+    def train(self, crop_ids, batch, learning_rate):
+        ids = np.ascontiguousarray(crop_ids, np.int32).reshape(-1)
+        steps = ids.size // batch
+        losses = np.zeros(steps, np.float32)
+        ms = ctypes.c_float()
+        _check(lib().ptd_train(self._h, ids.ctypes.data, batch, steps, float(learning_rate), losses.ctypes.data,
+                               ctypes.byref(ms)))
+        return losses, float(ms.value)
+
+    def denoise_renderer(self, renderer):
+        """(W*H, 3) denoised radiance of a renderer's accumulation and the device milliseconds."""
+        out = np.zeros((renderer.pixels, 3), np.float32)
+        ms = ctypes.c_float()
+        _check(lib().ptd_denoise_renderer(self._h, renderer._h, out.ctypes.data, ctypes.byref(ms)))
+        return out, float(ms.value)
+
+
+def relmse(x, ref, eps=0.01):
+    """mean over pixels and channels of (x - ref)^2 / (ref^2 + eps)."""
+    x = np.asarray(x, np.float64)
+    ref = np.asarray(ref, np.float64)
+    return float(np.mean((x - ref) ** 2 / (ref ** 2 + eps)))
+
+
+# ---------------------------------------------------------------- training data
+def training_views(count, seed, size):
+    """Seeded variants of the small room and the displaced icosphere scenes:
+    (scene, camera) pairs at size x size pixels."""
+    rng = np.random.default_rng(seed)
+    views = []
+    for i in range(count):
+        if i % 2 == 0:
+            scene, cam = ptrace.cornell(int(rng.integers(1, 1 << 30)), size, size)
+        else:
+            freq = int(rng.choice([24, 40, 64]))
+            scene, cam = ptrace.icosphere(freq, int(rng.integers(1, 1 << 30)), size, size)
+        views.append((scene, cam))
+    return views
+
+
+def render_features(scene, camera, width, height, seed, spp, depth=8):
+    """(radiance, albedo, normal) of an spp-sample render with feature buffers, each (H*W, 3)."""
+    r = ptrace.Renderer(scene, camera, width, height, seed, False, ptrace.DEBUG_SHADED, True, depth)
+    r.enable_aovs()
+    r.add_samples(spp)
+    albedo, normal = r.aovs()
+    return r.image(), albedo, normal
+
+
+def make_training_set(views=16, frame=512, crop=128, spp_in=4, spp_target=1024, seed=11, log=print):
+    """Crops of rendered views: features (crops, crop, crop, 9) and targets (crops, crop, crop, 3)."""
+    feats, targets = [], []
+    t0 = time.perf_counter()
+    for i, (scene, cam) in enumerate(training_views(views, seed, frame)):
+        rad, alb, nrm = render_features(scene, cam, frame, frame, 1000 + i, spp_in)
+        ref, _, _ = render_features(scene, cam, frame, frame, 5000 + i, spp_target)
+        f = np.concatenate([rad, alb, nrm], axis=1).reshape(frame, frame, 9)
+        t = ref.reshape(frame, frame, 3)
+        for y in range(0, frame, crop):
+            for x in range(0, frame, crop):
+                feats.append(f[y:y + crop, x:x + crop])
+                targets.append(t[y:y + crop, x:x + crop])
+        log(f"view {i}: {time.perf_counter() - t0:.1f}s")
+    return np.stack(feats).astype(np.float32), np.stack(targets).astype(np.float32)
+
+
+def train(net, feats, targets, steps=2000, batch=16, lr=3e-4, seed=3, chunk=100, log=print):
+    net.set_training_data(feats, targets)
+    rng = np.random.default_rng(seed)
+    history, device_ms = [], 0.0
+    for s in range(0, steps, chunk):
+        n = min(chunk, steps - s)
+        ids = rng.integers(0, len(feats), n * batch)
+        rate = lr * (0.1 ** (s / steps))  # exponential decay to lr/10
+        losses, ms = net.train(ids, batch, rate)
+        device_ms += ms
+        history += losses.tolist()
+        log(f"steps {s + n}: loss {np.mean(losses):.4f}, {ms / n:.2f} ms/step")
+    return np.array(history), device_ms
+
+
+# ---------------------------------------------------------------- evaluation
+def time_to_relmse(scene, camera, width, height, net, target=0.01, reference_spp=16384, plain_spp=(64, 256, 1024, 4096),
+                   denoised_spp=(1, 4, 16), seed=7, log=print):
+    """Wall time for plain progressive accumulation and for render + denoise to
+    reach relMSE <= target against a high-spp render of the same view."""
+    ref = ptrace.Renderer(scene, camera, width, height, 99991, False, ptrace.DEBUG_SHADED, True, 8)
+    t0 = time.perf_counter()
+    done = 0
+    while done < reference_spp:
+        step = min(1024, reference_spp - done)
+        ref.add_samples(step)
+        done += step
+    reference = ref.image()
+    log(f"reference {reference_spp} spp: {time.perf_counter() - t0:.1f}s")
+    del ref
+    plain = []
+    r = ptrace.Renderer(scene, camera, width, height, seed, False, ptrace.DEBUG_SHADED, True, 8)
+    have, spent = 0, 0.0
+    for spp in sorted(plain_spp):
+        t0 = time.perf_counter()
+        r.add_samples(spp - have)
+        spent += time.perf_counter() - t0
+        have = spp
+        plain.append({"spp": spp, "seconds": spent, "relmse": relmse(r.image(), reference)})
+        log("plain", plain[-1])
+    den = []
+    for spp in sorted(denoised_spp):
+        r = ptrace.Renderer(scene, camera, width, height, seed, False, ptrace.DEBUG_SHADED, True, 8)
+        r.enable_aovs()
+        t0 = time.perf_counter()
+        r.add_samples(spp)
+        out, ms = net.denoise_renderer(r)
+        secs = time.perf_counter() - t0
+        den.append({"spp": spp, "seconds": secs, "denoise_ms": ms, "relmse": relmse(out, reference),
+                    "noisy_relmse": relmse(r.image(), reference)})
+        log("denoised", den[-1])
+
+    def first(rows):
+        ok = [row for row in rows if row["relmse"] <= target]
+        return ok[0] if ok else None
+    p, d = first(plain), first(den)
+    return {"plain": plain, "denoised": den, "plain_hit": p, "denoised_hit": d,
+            "speedup": (p["seconds"] / d["seconds"]) if p and d else None}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    t = sub.add_parser("train")
+    t.add_argument("--out", default=WEIGHTS_PATH)
+    t.add_argument("--views", type=int, default=16)
+    t.add_argument("--frame", type=int, default=512)
+    t.add_argument("--crop", type=int, default=128)
+    t.add_argument("--spp-in", type=int, default=4)
+    t.add_argument("--spp-target", type=int, default=1024)
+    t.add_argument("--steps", type=int, default=2000)
+    t.add_argument("--batch", type=int, default=16)
+    t.add_argument("--lr", type=float, default=3e-4)
+    e = sub.add_parser("evaluate")
+    e.add_argument("--weights", default=WEIGHTS_PATH)
+    e.add_argument("--width", type=int, default=960)
+    e.add_argument("--height", type=int, default=540)
+    e.add_argument("--reference-spp", type=int, default=16384)
+    e.add_argument("--out", default=None)
+    args = ap.parse_args()
+    log = lambda *a: print(*a, flush=True)  # noqa: E731
+    if args.cmd == "train":
+        feats, targets = make_training_set(args.views, args.frame, args.crop, args.spp_in, args.spp_target, log=log)
+        net = Denoiser(seed=1)
+        history, ms = train(net, feats, targets, args.steps, args.batch, args.lr, log=log)
+        net.save(args.out, loss=history, device_ms=ms, config=json.dumps(vars(args)))
+        log(f"saved {args.out}: final loss {history[-50:].mean():.4f}, {ms / 1e3:.1f}s on the device")
+    else:
+        net = Denoiser(weights=args.weights)
+        scene, cam = ptrace.icosphere(224, 7, args.width, args.height)
+        res = time_to_relmse(scene, cam, args.width, args.height, net, reference_spp=args.reference_spp, log=log)
+        log(json.dumps(res))
+        if args.out:
+            with open(args.out, "w") as f:
+                json.dump(res, f, indent=1)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -149,6 +149,8 @@ PT_FUNCTIONS = {
     "pt_renderer_set_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "pt_renderer_iteration_log": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
                                                  ctypes.c_void_p]),
+    "pt_renderer_enable_aovs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "pt_renderer_read_aovs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]),
 }
 
 
@@ -309,6 +311,17 @@ class Renderer:
         st = LaunchStats()
         _check(lib().pt_renderer_last_launch_stats(self._h, ctypes.byref(st)))
         return {k: getattr(st, k) for k, _ in LaunchStats._fields_}
+
+    def enable_aovs(self, enable=True):
+        """Accumulate first-hit albedo and shading normal (denoiser features)."""
+        _check(lib().pt_renderer_enable_aovs(self._h, int(bool(enable))))
+
+    def aovs(self):
+        """(albedo, normal) averaged over the samples, each (W*H, 3) float32."""
+        albedo = np.zeros((self.pixels, 3), np.float32)
+        normal = np.zeros((self.pixels, 3), np.float32)
+        _check(lib().pt_renderer_read_aovs(self._h, albedo.ctypes.data, normal.ctypes.data, albedo.size))
+        return albedo, normal
 
     def iteration_log(self):
         """(extension, shadow) query counts per bounce iteration of the last add_samples."""

@@ -177,6 +177,16 @@ pt_status pt_renderer_last_launch_stats(const pt_renderer *renderer, pt_launch_s
 pt_status pt_renderer_iteration_log(const pt_renderer *renderer, uint32_t *extension, uint32_t *shadow,
                                     size_t capacity, int32_t *count);
 
+/* First-hit feature buffers for the denoiser (no counterpart in the
+ * reference): per pixel, the albedo (diffuse texture value, metal albedo,
+ * 1 for glass, emission clamped to 1 for lights) and shading normal of each
+ * camera ray's first hit, averaged over the samples like the image (0 where
+ * the ray escapes).  Shaded mode only; enable right after create or reset
+ * (no samples yet).  Random-number use and radiance are unchanged. */
+pt_status pt_renderer_enable_aovs(pt_renderer *renderer, int32_t enable);
+/* count = 3 * width * height floats per buffer; either pointer may be NULL. */
+pt_status pt_renderer_read_aovs(const pt_renderer *renderer, float *albedo, float *normal, size_t count);
+
 /* Per-renderer tuning of the wavefront loop (0 keeps the current value).
  * Defaults: 128, 256, 16, 8, 0, 0, 0, 262144, 48, 1. */
 typedef struct pt_tuning {
