@@ -65,6 +65,71 @@ def cpu_baseline(scene, camera, threads, spp=1):
                       f"({rays} rays, {dt:.1f} s, {threads} threads)"}
 
 
+BATCH_SIZES = [(256, 256), (512, 384), (640, 480), (800, 600), (1024, 576), (1280, 720)]
+BATCH_SPP, BATCH_DEPTH = 32, 6
+
+
+def batch_workload(ptrace, count, seed=SEED):
+    """Seeded clustered-triangle soups, log-uniform 10k..2M triangles, each with
+    its own camera and a resolution from 256x256 to 1280x720."""
+    import numpy as np
+    from concurrent.futures import ThreadPoolExecutor
+    rng = np.random.default_rng(seed)
+    tris = np.exp(rng.uniform(np.log(1e4), np.log(2e6), count)).astype(np.int64)
+    sizes = [BATCH_SIZES[int(i)] for i in rng.integers(0, len(BATCH_SIZES), count)]
+    def make(k):
+        w, h = sizes[k]
+        return ptrace.soup(int(tris[k]), seed * 1000 + k, w, h)
+    workers = min(16, len(os.sched_getaffinity(0)))
+    with ThreadPoolExecutor(workers) as ex:
+        built = list(ex.map(make, range(count)))
+    scenes = [s for s, _ in built]
+    views = [(k, built[k][1], sizes[k][0], sizes[k][1], seed + k) for k in range(count)]
+    return scenes, views, int(tris.sum())
+
+
+def bench_batch(args, ptrace):
+    t0 = time.perf_counter()
+    scenes, views, total_tris = batch_workload(ptrace, args.scenes)
+    pixels = sum(w * h for _, _, w, h, _ in views)
+    log(f"batch: {len(scenes)} scenes, {total_tris} triangles, {pixels} pixels, built in {time.perf_counter() - t0:.1f}s")
+    t0 = time.perf_counter()
+    b = ptrace.Batch(scenes, views, direct_light=True, max_depth=BATCH_DEPTH)
+    del scenes
+    log(f"batch upload {time.perf_counter() - t0:.1f}s")
+    tuned = None
+    if args.autotune:
+        t0 = time.perf_counter()
+        best, rate, trials = b.autotune(probe_samples=1, passes=1)
+        tuned = {"best": best, "probe_mrays": round(rate, 1), "trials": len(trials),
+                 "seconds": round(time.perf_counter() - t0, 1)}
+        log("autotune:", tuned)
+    for _ in range(args.warmup):
+        b.reset()
+        b.add_samples(BATCH_SPP)
+    rays = 0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        b.reset()
+        b.add_samples(BATCH_SPP)
+        rays += b.stats()["rays"]
+    elapsed = time.perf_counter() - t0
+    mrays = rays / elapsed / 1e6
+    st = b.last_launch_stats()
+    line = {"metric": "Mrays/s, 64-scene batch", "value": round(mrays, 2), "unit": "Mrays/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(elapsed / args.steps * 1e3, 2),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic (seeded clustered triangle soups)",
+            "config": {"workload": f"batched persistent rendering, {len(views)} scenes, {total_tris} triangles, "
+                                   f"{pixels} pixels, {BATCH_SPP} spp, depth {BATCH_DEPTH}, MIS",
+                       "scenes": len(views), "triangles": total_tris, "pixels": pixels, "spp": BATCH_SPP,
+                       "max_depth": BATCH_DEPTH, "rays_per_step": rays // args.steps,
+                       "bounce_iterations": st["iterations"], "graph_replays": st["trace_launches"],
+                       "drained_paths": st["drained_paths"], "tuning": b.tuning(), "autotune": tuned}}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -74,6 +139,10 @@ def main():
     ap.add_argument("--spp", type=int, default=SPP)
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a plain 1 GiB device copy (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--workload", choices=["frame", "batch"], default="frame",
+                    help="frame: the 1080p 1M-triangle frame (default); batch: 64 scenes in one batched loop")
+    ap.add_argument("--scenes", type=int, default=64, help="batch workload: number of scenes")
+    ap.add_argument("--autotune", action="store_true", help="batch workload: run the autotuner before timing")
     args = ap.parse_args()
     if args.copy_probe:
         import torch
@@ -95,6 +164,8 @@ def main():
     if not ok:
         raise SystemExit(f"no HIP device: {device}")
     log("device:", device)
+    if args.workload == "batch":
+        return bench_batch(args, ptrace)
     t0 = time.perf_counter()
     scene, camera = ptrace.icosphere(FREQ, SEED, WIDTH, HEIGHT)
     log(f"scene: {len(scene.triangles)} triangles, {len(scene.nodes)} nodes, depth {scene.max_depth}, "

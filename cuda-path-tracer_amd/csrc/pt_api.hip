@@ -335,6 +335,23 @@ struct wide_layout {
     int depth = 0;
 };
 
+// Triangle test data: v0, e1 = v1-v0, e2 = v2-v0 and |e1 x e2| exactly as
+// flat_intersect.h:9-12 computes them, plus 1e-7f*|e1 x e2| (:16).
+std::vector<float4> triangle_test_records(const pt_scene_desc& s) {
+    std::vector<float4> ti(size_t(3 * s.triangle_count));
+    for(int64_t i = 0; i < s.triangle_count; i++) {
+        const auto& t = s.triangles[i];
+        float e1x = t.v1.x - t.v0.x, e1y = t.v1.y - t.v0.y, e1z = t.v1.z - t.v0.z;
+        float e2x = t.v2.x - t.v0.x, e2y = t.v2.y - t.v0.y, e2z = t.v2.z - t.v0.z;
+        float cx = e1y * e2z - e1z * e2y, cy = -(e1x * e2z - e1z * e2x), cz = e1x * e2y - e1y * e2x;
+        float nl = std::sqrt(cx * cx + cy * cy + cz * cz);
+        ti[size_t(3 * i)] = make_float4(t.v0.x, t.v0.y, t.v0.z, 0.0000001f * nl);
+        ti[size_t(3 * i + 1)] = make_float4(e1x, e1y, e1z, nl);
+        ti[size_t(3 * i + 2)] = make_float4(e2x, e2y, e2z, 0);
+    }
+    return ti;
+}
+
 bool build_wide(const pt_scene_desc& s, const std::vector<float4>& ti, wide_layout& out) {
     if(s.root < 0) return false;
     const int64_t n = s.node_count;
@@ -382,7 +399,10 @@ bool build_wide(const pt_scene_desc& s, const std::vector<float4>& ti, wide_layo
         double dx = double(b.maximum.x) - b.minimum.x, dy = double(b.maximum.y) - b.minimum.y, dz = double(b.maximum.z) - b.minimum.z;
         return dx * dy + dy * dz + dz * dx;
     };
+    // A cluster of count <= 4 leaves from depth-first position first; note that
+    // the first leaf alone encodes as -1, a valid reference.
     auto cluster_ref = [&](int32_t node) { return ~((first[size_t(node)] << 3) | (count[size_t(node)] - 1)); };
+    bool failed = false;
     auto expandable = [&](int32_t node) { return s.nodes[node].type == PT_BRANCH && count[size_t(node)] > pt::wide_leaf_max; };
     std::function<int32_t(int32_t, int)> make = [&](int32_t b, int depth) -> int32_t {
         if(!expandable(b)) return cluster_ref(b);
@@ -402,7 +422,8 @@ bool build_wide(const pt_scene_desc& s, const std::vector<float4>& ti, wide_layo
             kids.erase(kids.begin() + best);
             kids.insert(kids.begin() + best, repl.begin(), repl.end());
             if(kids.size() > 4) {  // cannot happen: one slot becomes at most two
-                return -1;
+                failed = true;
+                return 0;
             }
         }
         size_t w = out.nodes.size() / 8;
@@ -418,7 +439,7 @@ bool build_wide(const pt_scene_desc& s, const std::vector<float4>& ti, wide_layo
             lo[0][k] = c.minimum.x; lo[1][k] = c.minimum.y; lo[2][k] = c.minimum.z;
             hi[0][k] = c.maximum.x; hi[1][k] = c.maximum.y; hi[2][k] = c.maximum.z;
             ref[k] = make(kids[k], depth + 1);
-            if(ref[k] == -1) return -1;
+            if(failed) return 0;
         }
         float4 *dst = &out.nodes[8 * w];
         for(int a = 0; a < 3; a++) {
@@ -433,7 +454,7 @@ bool build_wide(const pt_scene_desc& s, const std::vector<float4>& ti, wide_layo
     };
     out.depth = 1;
     out.root_ref = make(s.root, 1);
-    return out.root_ref != -1;
+    return !failed;
 }
 
 // ---------------- uploaded scene (cuda_renderer.cu:36-54) ----------------
@@ -451,7 +472,7 @@ struct device_scene {
     pt::scene_dev view{};
     int64_t branch_count = 0;
 
-    device_scene(const pt_scene_desc& s, bool with_tables, bool with_wide = false) {
+    device_scene(const pt_scene_desc& s, bool with_tables, bool with_wide = false, bool with_log2 = true) {
         // Traversal layout: branch nodes renumbered in array order, children
         // encoded as branch indices or leaf references.
         std::vector<int32_t> branch_id(size_t(s.node_count), -1);
@@ -474,19 +495,7 @@ struct device_scene {
             bn[2 * b] = make_float4(n.minimum.x, n.minimum.y, n.minimum.z, n.maximum.x);
             bn[2 * b + 1] = make_float4(n.maximum.y, n.maximum.z, lf, rf);
         }
-        // Triangle test data: v0, e1 = v1-v0, e2 = v2-v0 and |e1 x e2| exactly
-        // as flat_intersect.h:9-12 computes them, plus 1e-7f*|e1 x e2| (:16).
-        std::vector<float4> ti(size_t(3 * s.triangle_count));
-        for(int64_t i = 0; i < s.triangle_count; i++) {
-            const auto& t = s.triangles[i];
-            float e1x = t.v1.x - t.v0.x, e1y = t.v1.y - t.v0.y, e1z = t.v1.z - t.v0.z;
-            float e2x = t.v2.x - t.v0.x, e2y = t.v2.y - t.v0.y, e2z = t.v2.z - t.v0.z;
-            float cx = e1y * e2z - e1z * e2y, cy = -(e1x * e2z - e1z * e2x), cz = e1x * e2y - e1y * e2x;
-            float nl = std::sqrt(cx * cx + cy * cy + cz * cz);
-            ti[size_t(3 * i)] = make_float4(t.v0.x, t.v0.y, t.v0.z, 0.0000001f * nl);
-            ti[size_t(3 * i + 1)] = make_float4(e1x, e1y, e1z, nl);
-            ti[size_t(3 * i + 2)] = make_float4(e2x, e2y, e2z, 0);
-        }
+        std::vector<float4> ti = triangle_test_records(s);
         // Shading records (see scene_dev::shade_tri).  The outward normal is
         // cross(e1,e2) scaled by float(1.0/|e1 x e2|) as vec3::operator/=
         // does (src/core/vec3.h:152-158, flat_intersect.h:28).
@@ -547,6 +556,8 @@ struct device_scene {
         if(with_tables) {
             const trig_tables t = device_trig_tables();
             view.trig_bits = t.bits; view.trig_rank = t.rank; view.trig_exceptions = t.exceptions;
+        }
+        if(with_tables && with_log2) {
             // log2f(1+n) of the host libm for every count a traversal can reach.
             size_t n = size_t(std::min<int64_t>(s.node_count + 2, int64_t(1) << 26));
             std::vector<float> lg(n);
@@ -594,9 +605,10 @@ struct partition {
         if(ctl_host) (void)hipHostFree(ctl_host);
         if(stream) (void)hipStreamDestroy(stream);
     }
-    void read_ctl_async() {
-        HIP_CHECK(hipMemcpyAsync(ctl_host, ctl.get(), sizeof(pt::loop_ctl), hipMemcpyDeviceToHost, stream));
-        HIP_CHECK(hipEventRecord(checked, stream));
+    void read_ctl_async() { read_ctl_async(stream); }
+    void read_ctl_async(hipStream_t s) {
+        HIP_CHECK(hipMemcpyAsync(ctl_host, ctl.get(), sizeof(pt::loop_ctl), hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipEventRecord(checked, s));
     }
 };
 
@@ -909,6 +921,363 @@ void query(const pt_scene_desc *scene, const pt_ray *rays, size_t count, int32_t
 
 } // namespace
 
+// ---------------- batched rendering (no counterpart in the reference) ----------------
+// Many views of many scenes rendered by one wavefront bounce loop: the
+// scenes are merged into one scene set (arrays concatenated, indices
+// rebased), each keeping its own 4-wide hierarchy and light range; the loop
+// is captured once into a HIP graph and replayed until the tail kernel takes
+// over.  Every view's image equals the one a single renderer makes of it.
+namespace {
+
+struct merged_scenes {
+    std::vector<pt_flat_triangle> triangles;
+    std::vector<pt_flat_sphere> spheres;
+    std::vector<pt_flat_node> nodes;
+    std::vector<pt_flat_material> materials;
+    std::vector<pt_flat_texture> textures;
+    std::vector<uint8_t> pixels;
+    std::vector<int64_t> tri_offset, sph_offset;
+    pt_scene_desc desc() const {
+        pt_scene_desc d{};
+        d.triangles = triangles.data(); d.triangle_count = int64_t(triangles.size());
+        d.spheres = spheres.data(); d.sphere_count = int64_t(spheres.size());
+        d.nodes = nodes.data(); d.node_count = int64_t(nodes.size());
+        d.materials = materials.data(); d.material_count = int64_t(materials.size());
+        d.textures = textures.data(); d.texture_count = int64_t(textures.size());
+        d.pixels = pixels.data(); d.pixel_bytes = int64_t(pixels.size());
+        d.root = -1; d.max_depth = 1;
+        return d;
+    }
+};
+
+merged_scenes merge_scenes(const pt_scene_desc *scenes, int32_t count) {
+    merged_scenes m;
+    for(int32_t k = 0; k < count; k++) {
+        const pt_scene_desc& s = scenes[k];
+        const int64_t to = int64_t(m.triangles.size()), so = int64_t(m.spheres.size()), no = int64_t(m.nodes.size());
+        const int64_t mo = int64_t(m.materials.size()), xo = int64_t(m.textures.size()), po = int64_t(m.pixels.size());
+        if(to + s.triangle_count > INT32_MAX / 2 || so + s.sphere_count > INT32_MAX / 2 || no + s.node_count > INT32_MAX ||
+           po + s.pixel_bytes > INT32_MAX)
+            fail(PT_EINVAL, "batch exceeds 32-bit indices");
+        m.tri_offset.push_back(to);
+        m.sph_offset.push_back(so);
+        for(int64_t i = 0; i < s.triangle_count; i++) {
+            pt_flat_triangle t = s.triangles[i];
+            t.material_id += int32_t(mo);
+            m.triangles.push_back(t);
+        }
+        for(int64_t i = 0; i < s.sphere_count; i++) {
+            pt_flat_sphere p = s.spheres[i];
+            p.material_id += int32_t(mo);
+            m.spheres.push_back(p);
+        }
+        for(int64_t i = 0; i < s.node_count; i++) {
+            pt_flat_node n = s.nodes[i];
+            if(n.type == PT_BRANCH) { n.left += int32_t(no); n.right += int32_t(no); }
+            else n.primitive += int32_t(n.type == PT_SPHERE ? so : to);
+            m.nodes.push_back(n);
+        }
+        for(int64_t i = 0; i < s.material_count; i++) {
+            pt_flat_material x = s.materials[i];
+            if(x.texture_id >= 0) x.texture_id += int32_t(xo);
+            m.materials.push_back(x);
+        }
+        for(int64_t i = 0; i < s.texture_count; i++) {
+            pt_flat_texture x = s.textures[i];
+            if(x.image_id >= 0) x.image_id += int32_t(xo);
+            if(x.type == PT_IMAGE) x.byte_offset += int32_t(po);
+            m.textures.push_back(x);
+        }
+        m.pixels.insert(m.pixels.end(), s.pixels, s.pixels + s.pixel_bytes);
+    }
+    return m;
+}
+
+// A wide-hierarchy reference moved into the concatenated layout.
+int32_t rebase_ref(int32_t ref, int32_t node_offset, int32_t leaf_offset) {
+    if(ref >= 0) return ref + node_offset;
+    if(ref == pt::wide_empty) return ref;
+    const int32_t v = ~ref;
+    return ~((((v >> 3) + leaf_offset) << 3) | (v & 7));
+}
+
+} // namespace
+
+struct pt_batch {
+    std::unique_ptr<device_scene> scene;
+    dev_buf<float4> wnodes, leaves;
+    dev_buf<pt::light_rec> lights;
+    dev_buf<pt::view_rec> views;
+    dev_buf<uint16_t> pixel_view;
+    std::vector<pt::view_rec> host_views;
+    pt::batch_dev bd{};
+    pt::frame_params frame{};
+    size_t pixels = 0;
+    dev_buf<float> sum;
+    dev_buf<uint64_t> rng;
+    dev_buf<float4> ray_o, ray_d, thr, rad, sh_o, sh_d, sh_c;
+    dev_buf<int2> ctr;
+    dev_buf<int4> hit;
+    partition part;
+    hipStream_t stream = nullptr;
+    hipEvent_t start = nullptr, stop = nullptr;
+    pt_tuning tuning{128, 256, 16, 8, 0, 0, 0, 0, 48, 1};
+    int cus = 1;
+    int32_t samples = 0;
+    uint64_t rays = 0;
+    float milliseconds = 0;
+    hipGraphExec_t graph = nullptr;
+    int graph_iterations = 0, graph_trace_block = 0, graph_blocks = 0, graph_shade_block = 0, graph_refill = 0;
+    pt_launch_stats last{};
+
+    ~pt_batch() {
+        if(stream) (void)hipStreamSynchronize(stream);
+        if(graph) (void)hipGraphExecDestroy(graph);
+        if(start) (void)hipEventDestroy(start);
+        if(stop) (void)hipEventDestroy(stop);
+        if(stream) (void)hipStreamDestroy(stream);
+    }
+
+    int trace_block() {
+        int block = tuning.trace_block;
+        while(block > 64 && pt::trace_lds_bytes(scene->view, frame, block) > 65536) block -= 64;
+        if(pt::trace_lds_bytes(scene->view, frame, block) > 65536) fail(PT_EINVAL, "batch hierarchy too deep for the LDS stack");
+        return block;
+    }
+
+    void reset() {
+        HIP_CHECK(hipStreamSynchronize(stream));
+        pt::launch_batch_init(bd, sum.get(), rng.get(), int64_t(pixels), part.ctl.get(), stream);
+        pt::launch_clear_ctl(part.ctl.get(), stream);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipStreamSynchronize(stream));
+        samples = 0; rays = 0; milliseconds = 0;
+    }
+
+    // The bounce loop as one graph: trace, bookkeep, shade, regenerate,
+    // `iterations` times (even, so each replay ends on parity 0).
+    void capture(int iterations, int block, int blocks, hipStream_t s) {
+        if(graph && graph_iterations == iterations && graph_trace_block == block && graph_blocks == blocks &&
+           graph_shade_block == tuning.shade_block && graph_refill == tuning.lane_refill)
+            return;
+        if(graph) { HIP_CHECK(hipGraphExecDestroy(graph)); graph = nullptr; }
+        const int shade_grid = int(std::max<size_t>(1, std::min<size_t>(
+            (pixels + size_t(tuning.shade_block) - 1) / size_t(tuning.shade_block), size_t(cus) * 16)));
+        hipGraph_t g = nullptr;
+        HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        for(int k = 0; k < iterations; k++) {
+            const int par = k & 1;
+            pt::launch_trace_lanes(scene->view, frame, part.paths, part.ctl.get(), par, blocks, block, tuning.lane_refill,
+                                   false, s, &bd);
+            pt::launch_bookkeep(part.ctl.get(), par, s);
+            pt::launch_shade(scene->view, frame, part.paths, part.ctl.get(), par, shade_grid, tuning.shade_block, s, &bd);
+            pt::launch_regen(frame, part.paths, part.ctl.get(), par, shade_grid, tuning.shade_block, s, &bd);
+        }
+        HIP_CHECK(hipStreamEndCapture(s, &g));
+        hipError_t e = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIP_CHECK(e);
+        graph_iterations = iterations; graph_trace_block = block; graph_blocks = blocks;
+        graph_shade_block = tuning.shade_block; graph_refill = tuning.lane_refill;
+    }
+
+    void add_samples(int32_t count) {
+        if(count <= 0 || count > INT32_MAX - samples) fail(PT_EINVAL, "invalid accumulated sample count");
+        pt_launch_stats st{};
+        const int block = trace_block();
+        const int blocks = cus * std::max(1, tuning.blocks_per_cu);
+        const int drain_block = 256;
+        const int64_t threshold = tuning.drain_threshold > 0 ? tuning.drain_threshold
+                                : tuning.drain_threshold < 0 ? 0 : std::max<int64_t>(4096, int64_t(pixels) / 8);
+        const size_t drain_threads = threshold ? size_t((threshold + drain_block - 1) / drain_block * drain_block) : 0;
+        const size_t spill_need = std::max(size_t(blocks) * size_t(block), drain_threads) *
+                                  size_t(pt::wide_spill_entries(scene->view.wide_stack_capacity));
+        if(part.spill.size() < spill_need) { part.spill.alloc(spill_need); part.paths.spill = part.spill.get(); }
+        const int iterations = 2 * std::max(1, tuning.check_interval / 2);
+        capture(iterations, block, blocks, stream);
+        HIP_CHECK(hipEventRecord(start, stream));
+        pt::launch_start(scene->view, frame, part.paths, part.ctl.get(), count, 0, 1, partition_chunk, int64_t(pixels),
+                         stream, &bd);
+        HIP_CHECK(hipGetLastError());
+        bool drained = false;
+        for(;;) {
+            HIP_CHECK(hipGraphLaunch(graph, stream));
+            part.read_ctl_async(stream);
+            st.trace_launches += iterations;
+            HIP_CHECK(hipEventSynchronize(part.checked));
+            const pt::loop_ctl& h = *part.ctl_host;
+            if(h.failure) break;
+            const uint32_t live = h.n_ext[0];
+            if(live == 0) break;
+            if(threshold && int64_t(live) <= threshold) {
+                pt::launch_drain(scene->view, frame, part.paths, part.ctl.get(), 0, h.n_shadow, live, drain_block, stream, &bd);
+                HIP_CHECK(hipGetLastError());
+                st.drained_paths = live;
+                drained = true;
+                break;
+            }
+        }
+        HIP_CHECK(hipEventRecord(stop, stream));
+        part.read_ctl_async(stream);
+        HIP_CHECK(hipEventSynchronize(part.checked));
+        (void)drained;
+        const pt::loop_ctl& h = *part.ctl_host;
+        if(h.failure) fail(PT_EOVERFLOW, "HIP traversal stack overflow");
+        st.iterations = int32_t(h.iterations);
+        st.extend_rays = h.ext_rays;
+        st.shadow_rays = h.shadow_rays;
+        float elapsed = 0;
+        HIP_CHECK(hipEventElapsedTime(&elapsed, start, stop));
+        milliseconds += elapsed;
+        rays = h.rays;
+        samples += count;
+        last = st;
+    }
+};
+
+namespace {
+
+pt_batch *create_batch(const pt_scene_desc *scenes, int32_t scene_count, const pt_batch_view *views, int32_t view_count,
+                       int32_t direct_light, int32_t max_path_depth) {
+    if(!scenes || !views || scene_count <= 0 || view_count <= 0) fail(PT_EINVAL, "empty batch");
+    if(view_count > 65535) fail(PT_EINVAL, "too many views in one batch");
+    if(max_path_depth < 0) fail(PT_EINVAL, "invalid path depth");
+    for(int32_t k = 0; k < scene_count; k++) {
+        validate_scene(scenes[k]);
+        if(scenes[k].root < 0) fail(PT_EINVAL, "batch scenes must not be empty");
+    }
+    size_t total = 0;
+    for(int32_t v = 0; v < view_count; v++) {
+        if(views[v].scene < 0 || views[v].scene >= scene_count) fail(PT_EINVAL, "view scene index out of range");
+        validate_camera(views[v].camera);
+        total += checked_pixel_count(views[v].width, views[v].height);
+        if(total > size_t(INT32_MAX)) fail(PT_EINVAL, "batch exceeds 2^31 pixels");
+    }
+    require_device();
+    auto b = std::make_unique<pt_batch>();
+    b->pixels = total;
+    HIP_CHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    HIP_CHECK(hipEventCreate(&b->start));
+    HIP_CHECK(hipEventCreate(&b->stop));
+    b->cus = device_cu_count();
+    b->frame.max_depth = max_path_depth ? max_path_depth : 50;
+    b->frame.direct_light = direct_light ? 1 : 0;
+    b->frame.debug = PT_DEBUG_SHADED;
+    b->frame.width = views[0].width; b->frame.height = views[0].height; b->frame.camera = views[0].camera;
+
+    merged_scenes m = merge_scenes(scenes, scene_count);
+    const pt_scene_desc md = m.desc();
+    b->scene = std::make_unique<device_scene>(md, true, false, false);
+
+    // Per-scene 4-wide hierarchies and light ranges, concatenated.
+    std::vector<float4> wn, lv;
+    std::vector<pt::light_rec> lights;
+    const size_t nscenes = static_cast<size_t>(scene_count);
+    std::vector<int32_t> roots(nscenes, pt::empty_root), light_offset(nscenes, 0), light_count(nscenes, 0);
+    std::vector<float> light_area(nscenes, 0.0f);
+    int depth = 1;
+    for(int32_t k = 0; k < scene_count; k++) {
+        const pt_scene_desc& s = scenes[k];
+        std::vector<float4> ti = triangle_test_records(s);
+        wide_layout w;
+        if(s.root >= 0) {
+            if(!build_wide(s, ti, w)) fail(PT_EINVAL, "batch scenes need tree hierarchies (no shared subtrees)");
+            const int32_t node_offset = int32_t(wn.size() / 8), leaf_offset = int32_t(lv.size() / 3);
+            if(lv.size() / 3 + w.leaves.size() / 3 >= (size_t(1) << 27)) fail(PT_EINVAL, "batch exceeds 2^27 leaves");
+            for(size_t i = 0; i < w.nodes.size() / 8; i++) {
+                float4 *r = &w.nodes[8 * i + 6];
+                int32_t ref[4];
+                std::memcpy(ref, r, sizeof ref);
+                for(int j = 0; j < 4; j++) ref[j] = rebase_ref(ref[j], node_offset, leaf_offset);
+                std::memcpy(r, ref, sizeof ref);
+            }
+            for(size_t i = 0; i < w.leaves.size() / 3; i++) {
+                float *cf = &w.leaves[3 * i + 2].w;
+                int32_t code;
+                std::memcpy(&code, cf, 4);
+                const int32_t prim = code >> 1, sphere = code & 1;
+                code = int32_t((prim + (sphere ? m.sph_offset[size_t(k)] : m.tri_offset[size_t(k)])) * 2 + sphere);
+                std::memcpy(cf, &code, 4);
+            }
+            roots[size_t(k)] = rebase_ref(w.root_ref, node_offset, leaf_offset);
+            wn.insert(wn.end(), w.nodes.begin(), w.nodes.end());
+            lv.insert(lv.end(), w.leaves.begin(), w.leaves.end());
+            depth = std::max(depth, w.depth);
+        }
+        std::vector<pt::light_rec> lk = scene_lights(s);
+        light_offset[size_t(k)] = int32_t(lights.size());
+        light_count[size_t(k)] = int32_t(lk.size());
+        light_area[size_t(k)] = lk.empty() ? 0.0f : lk.back().cumulative_area;
+        for(auto l : lk) {
+            l.primitive += int32_t(l.type == PT_SPHERE ? m.sph_offset[size_t(k)] : m.tri_offset[size_t(k)]);
+            lights.push_back(l);
+        }
+    }
+    b->wnodes.upload(wn.data(), wn.size());
+    b->leaves.upload(lv.data(), lv.size());
+    b->lights.upload(lights.data(), lights.size());
+    pt::scene_dev& sv = b->scene->view;
+    sv.wnodes = b->wnodes.get(); sv.leaves = b->leaves.get();
+    // Marks the wide layout as present; each query starts at its view's root.
+    sv.wroot_ref = roots[0];
+    sv.wide_stack_capacity = 3 * depth + 1;
+    sv.lights = b->lights.get(); sv.light_count = int32_t(lights.size());
+    sv.light_area = 0;
+
+    // Views and the pixel -> view map.
+    std::vector<uint16_t> pv(total);
+    int64_t offset = 0;
+    for(int32_t v = 0; v < view_count; v++) {
+        pt::view_rec r{};
+        const pt_batch_view& in = views[v];
+        r.camera = in.camera; r.width = in.width; r.height = in.height;
+        r.pixel_offset = offset; r.seed = in.seed;
+        r.wroot_ref = roots[size_t(in.scene)];
+        r.light_offset = light_offset[size_t(in.scene)]; r.light_count = light_count[size_t(in.scene)];
+        r.light_area = light_area[size_t(in.scene)];
+        b->host_views.push_back(r);
+        const int64_t n = int64_t(in.width) * in.height;
+        std::fill(pv.begin() + offset, pv.begin() + offset + n, uint16_t(v));
+        offset += n;
+    }
+    b->views.upload(b->host_views.data(), b->host_views.size());
+    b->pixel_view.upload(pv.data(), pv.size());
+    b->bd.views = b->views.get();
+    b->bd.pixel_view = b->pixel_view.get();
+
+    // Path state for every pixel of every view; one set of queues.
+    const size_t n = total;
+    b->sum.alloc(3 * n); b->rng.alloc(n);
+    b->ray_o.alloc(n); b->ray_d.alloc(n); b->thr.alloc(n); b->rad.alloc(n);
+    b->sh_o.alloc(n); b->sh_d.alloc(n); b->sh_c.alloc(n); b->ctr.alloc(n); b->hit.alloc(n);
+    partition& part = b->part;
+    part.count = int64_t(n);
+    part.stream = nullptr;
+    for(auto *e : {&part.checked, &part.drain0, &part.drain1, &part.finished}) HIP_CHECK(hipEventCreate(e));
+    part.ctl.alloc(1);
+    HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&part.ctl_host), sizeof(pt::loop_ctl), hipHostMallocDefault));
+    part.queue0.alloc(n); part.queue1.alloc(n); part.shadow_queue.alloc(n); part.finish_queue.alloc(n);
+    part.log.alloc(2 * size_t(log_capacity));
+    pt::loop_ctl init{};
+    init.log = part.log.get();
+    init.log_capacity = log_capacity;
+    HIP_CHECK(hipMemcpy(part.ctl.get(), &init, sizeof init, hipMemcpyHostToDevice));
+    auto& p = part.paths;
+    p.ray_o = b->ray_o.get(); p.ray_d = b->ray_d.get(); p.thr = b->thr.get(); p.rad = b->rad.get();
+    p.ctr = b->ctr.get(); p.hit = b->hit.get();
+    p.sh_o = b->sh_o.get(); p.sh_d = b->sh_d.get(); p.sh_c = b->sh_c.get();
+    p.rng = b->rng.get(); p.sum = b->sum.get(); p.aov = nullptr;
+    p.queue[0] = part.queue0.get(); p.queue[1] = part.queue1.get();
+    p.shadow_queue = part.shadow_queue.get(); p.finish_queue = part.finish_queue.get();
+    b->reset();
+    return b.release();
+}
+
+// Tuning dimensions the autotuner sweeps, in order.
+struct tune_axis { int32_t pt_tuning::*field; std::vector<int32_t> values; };
+
+} // namespace
+
 extern "C" {
 
 const char *pt_last_error(void) { return g_last_error.c_str(); }
@@ -1178,6 +1547,133 @@ pt_status pt_renderer_set_tuning(pt_renderer *renderer, const pt_tuning *tuning)
             t.partitions = tuning->partitions;
         }
         renderer->tuning = t;
+    });
+}
+
+
+pt_status pt_batch_create(const pt_scene_desc *scenes, int32_t scene_count, const pt_batch_view *views,
+                          int32_t view_count, int32_t direct_light, int32_t max_path_depth, pt_batch **out) {
+    return guarded([&] {
+        if(!out) fail(PT_EINVAL, "null output handle");
+        *out = nullptr;
+        *out = create_batch(scenes, scene_count, views, view_count, direct_light, max_path_depth);
+    });
+}
+
+pt_status pt_batch_destroy(pt_batch *batch) {
+    return guarded([&] { delete batch; });
+}
+
+pt_status pt_batch_reset(pt_batch *batch) {
+    return guarded([&] {
+        if(!batch) fail(PT_EINVAL, "null batch");
+        batch->reset();
+    });
+}
+
+pt_status pt_batch_add_samples(pt_batch *batch, int32_t count) {
+    return guarded([&] {
+        if(!batch) fail(PT_EINVAL, "null batch");
+        batch->add_samples(count);
+    });
+}
+
+pt_status pt_batch_read_image(const pt_batch *batch, int32_t view, float *rgb, size_t count) {
+    return guarded([&] {
+        if(!batch || !rgb) fail(PT_EINVAL, "null argument");
+        if(view < 0 || size_t(view) >= batch->host_views.size()) fail(PT_EINVAL, "view index out of range");
+        const pt::view_rec& v = batch->host_views[size_t(view)];
+        const size_t n = size_t(v.width) * size_t(v.height);
+        if(count != 3 * n) fail(PT_EINVAL, "image buffer size mismatch");
+        if(batch->samples <= 0) fail(PT_EINVAL, "no samples accumulated");
+        HIP_CHECK(hipMemcpy(rgb, batch->sum.get() + 3 * size_t(v.pixel_offset), 3 * n * sizeof(float), hipMemcpyDeviceToHost));
+        // image(): sum times float(1.0 / samples) (cuda_renderer.cu:230-234).
+        const float k = float(1.0 / double(batch->samples));
+        for(size_t i = 0; i < 3 * n; i++) rgb[i] *= k;
+    });
+}
+
+pt_status pt_batch_stats(const pt_batch *batch, int32_t *samples, float *milliseconds, uint64_t *rays) {
+    return guarded([&] {
+        if(!batch) fail(PT_EINVAL, "null batch");
+        if(samples) *samples = batch->samples;
+        if(milliseconds) *milliseconds = batch->milliseconds;
+        if(rays) *rays = batch->rays;
+    });
+}
+
+pt_status pt_batch_last_launch_stats(const pt_batch *batch, pt_launch_stats *stats) {
+    return guarded([&] {
+        if(!batch || !stats) fail(PT_EINVAL, "null argument");
+        *stats = batch->last;
+    });
+}
+
+pt_status pt_batch_get_tuning(const pt_batch *batch, pt_tuning *tuning) {
+    return guarded([&] {
+        if(!batch || !tuning) fail(PT_EINVAL, "null argument");
+        *tuning = batch->tuning;
+    });
+}
+
+pt_status pt_batch_set_tuning(pt_batch *batch, const pt_tuning *tuning) {
+    return guarded([&] {
+        if(!batch || !tuning) fail(PT_EINVAL, "null argument");
+        pt_tuning t = batch->tuning;
+        auto block_ok = [](int32_t b, int32_t hi) { return b >= 64 && b <= hi && b % 64 == 0; };
+        if(tuning->trace_block) { if(!block_ok(tuning->trace_block, 256)) fail(PT_EINVAL, "trace block must be 64..256, multiple of 64"); t.trace_block = tuning->trace_block; }
+        if(tuning->shade_block) { if(!block_ok(tuning->shade_block, 1024)) fail(PT_EINVAL, "shade block must be 64..1024, multiple of 64"); t.shade_block = tuning->shade_block; }
+        if(tuning->blocks_per_cu) { if(tuning->blocks_per_cu < 1 || tuning->blocks_per_cu > 64) fail(PT_EINVAL, "blocks per CU must be 1..64"); t.blocks_per_cu = tuning->blocks_per_cu; }
+        if(tuning->check_interval) { if(tuning->check_interval < 1 || tuning->check_interval > 64) fail(PT_EINVAL, "check interval must be 1..64"); t.check_interval = tuning->check_interval; }
+        if(tuning->drain_threshold) t.drain_threshold = tuning->drain_threshold < 0 ? -1 : tuning->drain_threshold;
+        if(tuning->lane_refill) { if(tuning->lane_refill < 1 || tuning->lane_refill > 64) fail(PT_EINVAL, "lane refill must be 1..64"); t.lane_refill = tuning->lane_refill; }
+        batch->tuning = t;
+    });
+}
+
+pt_status pt_batch_autotune(pt_batch *batch, int32_t probe_samples, int32_t passes, pt_tuning *best, float *best_mrays,
+                            pt_batch_trial *trials, int32_t trial_capacity, int32_t *trial_count) {
+    return guarded([&] {
+        if(!batch) fail(PT_EINVAL, "null batch");
+        if(probe_samples <= 0 || passes <= 0) fail(PT_EINVAL, "invalid probe");
+        const int32_t px = int32_t(std::min<size_t>(batch->pixels, size_t(INT32_MAX)));
+        const std::vector<tune_axis> axes = {
+            {&pt_tuning::trace_block, {64, 128, 192, 256}},
+            {&pt_tuning::blocks_per_cu, {8, 16, 24, 32}},
+            {&pt_tuning::lane_refill, {16, 32, 48, 64}},
+            {&pt_tuning::shade_block, {256, 512, 1024}},
+            {&pt_tuning::drain_threshold, {std::max(1024, px / 32), std::max(2048, px / 16), std::max(4096, px / 8), std::max(8192, px / 4)}},
+            {&pt_tuning::check_interval, {2, 4, 8, 16}},
+        };
+        int32_t count = 0;
+        auto measure = [&](const pt_tuning& t) {
+            batch->tuning = t;
+            batch->reset();
+            batch->add_samples(probe_samples);
+            const float rate = float(double(batch->rays) / (double(batch->milliseconds) * 1e3));
+            if(trials && count < trial_capacity) trials[count] = pt_batch_trial{t, rate};
+            count++;
+            return rate;
+        };
+        pt_tuning current = batch->tuning;
+        if(current.drain_threshold == 0) current.drain_threshold = std::max(4096, px / 8);
+        float current_rate = measure(current);
+        for(int32_t pass = 0; pass < passes; pass++) {
+            for(const tune_axis& axis : axes) {
+                for(int32_t value : axis.values) {
+                    if(current.*axis.field == value) continue;
+                    pt_tuning t = current;
+                    t.*axis.field = value;
+                    const float rate = measure(t);
+                    if(rate > current_rate) { current = t; current_rate = rate; }
+                }
+            }
+        }
+        batch->tuning = current;
+        batch->reset();
+        if(best) *best = current;
+        if(best_mrays) *best_mrays = current_rate;
+        if(trial_count) *trial_count = count;
     });
 }
 

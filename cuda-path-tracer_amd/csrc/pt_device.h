@@ -121,28 +121,47 @@ struct frame_params {
 void launch_trig_values(float2 *out, hipStream_t stream);
 void launch_query(const scene_dev& s, const pt_ray *rays, pt_trace_result *out, int32_t *boxes, int32_t *prims,
                   int32_t count, bool brute, hipStream_t stream);
+// Batched rendering: many views, each of one scene of a merged scene set
+// (nodes, primitives, materials and lights of all scenes concatenated; each
+// scene keeps its own hierarchy root and light range).  Pixels of all views
+// form one index space; each path's view gives its camera, root, lights
+// and its pixel index within the view (for the PCG stream).
+struct view_rec {
+    pt_flat_camera camera;
+    int32_t width, height;
+    int64_t pixel_offset;      // first pixel of the view in the batch
+    uint64_t seed;
+    int32_t wroot_ref;         // the view's scene's hierarchy root
+    int32_t light_offset, light_count;
+    float light_area;
+};
+struct batch_dev {
+    const view_rec *views;
+    const uint16_t *pixel_view;   // view of every batch pixel
+};
 void launch_init_pixels(float *sum, uint64_t *rng, int64_t count, uint64_t seed, loop_ctl *ctl, hipStream_t stream);
 void launch_debug(const scene_dev& s, const frame_params& f, float *sum, uint64_t *rng, int32_t samples,
                   int32_t *failure, dim3 block, hipStream_t stream);
 // First camera rays of one pixel partition (chunks c with c % parts == part).
 void launch_start(const scene_dev& s, const frame_params& f, const paths_dev& p, loop_ctl *ctl, int32_t samples,
-                  int part, int parts, int chunk, int64_t count, hipStream_t stream);
+                  int part, int parts, int chunk, int64_t count, hipStream_t stream, const batch_dev *batch = nullptr);
 void launch_clear_ctl(loop_ctl *ctl, hipStream_t stream);
 void launch_trace(const scene_dev& s, const frame_params& f, const paths_dev& p, loop_ctl *ctl, int parity,
                   int grid, int block, bool count_tests, hipStream_t stream);
 // Wide-layout trace stage with persistent lanes (refill: idle lanes that
 // trigger a new claim, 1..64).
 void launch_trace_lanes(const scene_dev& s, const frame_params& f, const paths_dev& p, loop_ctl *ctl, int parity,
-                        int grid, int block, int refill, bool count_tests, hipStream_t stream);
+                        int grid, int block, int refill, bool count_tests, hipStream_t stream,
+                        const batch_dev *batch = nullptr);
 void launch_bookkeep(loop_ctl *ctl, int parity, hipStream_t stream);
 void launch_shade(const scene_dev& s, const frame_params& f, const paths_dev& p, loop_ctl *ctl, int parity,
-                  int grid, int block, hipStream_t stream);
+                  int grid, int block, hipStream_t stream, const batch_dev *batch = nullptr);
 void launch_regen(const frame_params& f, const paths_dev& p, loop_ctl *ctl, int parity, int grid, int block,
-                  hipStream_t stream);
+                  hipStream_t stream, const batch_dev *batch = nullptr);
 // Finishes the pending shadow queries and then every queued path in one
 // thread each; `shadows`/`paths` are the host's copies of the queue sizes.
 void launch_drain(const scene_dev& s, const frame_params& f, const paths_dev& p, loop_ctl *ctl, int parity,
-                  uint32_t shadows, uint32_t paths, int block, hipStream_t stream);
+                  uint32_t shadows, uint32_t paths, int block, hipStream_t stream, const batch_dev *batch = nullptr);
 void launch_pack(const float *sum, uint8_t *rgba, int64_t count, int32_t samples, hipStream_t stream);
 void launch_add_rays(loop_ctl *ctl, unsigned long long rays, hipStream_t stream);
 
@@ -159,6 +178,8 @@ inline size_t trace_lds_bytes(const scene_dev& s, const frame_params& f, int thr
     bool wide = !f.brute && s.wroot_ref != empty_root && !f.exact_traversal;
     return wide ? std::max(exact, wide_stack_bytes(s.wide_stack_capacity, threads)) : exact;
 }
+
+void launch_batch_init(const batch_dev& b, float *sum, uint64_t *rng, int64_t count, loop_ctl *ctl, hipStream_t stream);
 
 // Device view of a renderer's accumulation for the denoiser (same library).
 struct frame_view {

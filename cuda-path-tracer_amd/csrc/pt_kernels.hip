@@ -291,12 +291,12 @@ struct wide_stack {
 };
 
 template<bool ANY_HIT, bool STATS>
-__device__ bool traverse_wide(const scene_dev& s, const ray_in& r, float low, float high, wide_stack& stk,
+__device__ bool traverse_wide(const scene_dev& s, int32_t root, const ray_in& r, float low, float high, wide_stack& stk,
                               best_hit& best, int& boxes, int& prims, int& spheres, int& error) {
     bool found = false;
     float closest = high;
     int32_t best_rank = -1;
-    int32_t cur = s.wroot_ref;
+    int32_t cur = root;
     int count = 0;
     const int cap = s.wide_stack_capacity;
     for(;;) {
@@ -380,8 +380,8 @@ struct wide_query {
     bool found, any;
 };
 
-__device__ __forceinline__ void wide_begin(const scene_dev& s, wide_query& q, bool any, float high) {
-    q.high = high; q.closest = high; q.cur = s.wroot_ref; q.count = 0; q.best_rank = -1;
+__device__ __forceinline__ void wide_begin(int32_t root, wide_query& q, bool any, float high) {
+    q.high = high; q.closest = high; q.cur = root; q.count = 0; q.best_rank = -1;
     q.found = false; q.any = any;
 }
 
@@ -584,16 +584,19 @@ __device__ __forceinline__ v3 cosine_direction(const scene_dev& s, v3 normal, rn
     return add(add(scale(radius * cs.x, tangent), scale(radius * cs.y, bitangent)), scale(sqrtf(1 - u), normal));
 }
 struct light_sample { v3 point, normal; float u, v; int32_t material; };
+// The emitters a path samples: the scene's, or one view's range of a batch.
+struct light_set { const light_rec *lights; int32_t count; float area; };
+__device__ __forceinline__ light_set scene_lights(const scene_dev& s) { return light_set{s.lights, s.light_count, s.light_area}; }
 // light_sampling.h:27-54
-__device__ light_sample sample_light(const scene_dev& s, rng_t& r) {
-    float choice = r.uniform() * s.light_area;
-    int low = 0, high = s.light_count - 1;
+__device__ light_sample sample_light(const scene_dev& s, const light_set& ls, rng_t& r) {
+    float choice = r.uniform() * ls.area;
+    int low = 0, high = ls.count - 1;
     while(low < high) {
         int middle = low + (high - low) / 2;
-        if(choice < s.lights[middle].cumulative_area) high = middle;
+        if(choice < ls.lights[middle].cumulative_area) high = middle;
         else low = middle + 1;
     }
-    const light_rec light = s.lights[low];
+    const light_rec light = ls.lights[low];
     float u = r.uniform();
     uint32_t kv = r.next24();
     float v = float(kv) * 0x1p-24f;
@@ -628,13 +631,13 @@ __device__ __forceinline__ float light_direction_pdf(float area, v3 from, v3 poi
     return isfinite(pdf) ? pdf : 0;
 }
 // light_sampling.h:66-73
-__device__ __forceinline__ float hit_light_pdf(const scene_dev& s, v3 from, const hit_rec& h) {
+__device__ __forceinline__ float hit_light_pdf(const scene_dev& s, float light_area, v3 from, const hit_rec& h) {
     v3 normal = h.normal;
     if(h.primitive_type == PT_TRIANGLE) {
         const pt_flat_triangle& t = s.triangles[h.primitive_id];
         normal = unit_vector(cross(sub(mk(t.v1), mk(t.v0)), sub(mk(t.v2), mk(t.v0))));
     }
-    return light_direction_pdf(s.light_area, from, h.p, normal);
+    return light_direction_pdf(light_area, from, h.p, normal);
 }
 
 // ---------------- BSDF sampling (src/renderer/transport.h:113-155) ----------------
@@ -842,9 +845,54 @@ __global__ void debug_kernel(scene_dev s, frame_params f, float *sum, uint64_t *
 // Start of an add_samples call: the first camera ray of every pixel of one
 // partition.  Pixels are dealt to `parts` partitions in chunks of
 // `chunk` consecutive pixels: chunk c belongs to partition c % parts.
+// ---------------- per-pixel view (single frame or batch) ----------------
+// A single render reads its frame parameters and the scene's root and lights;
+// a batch looks up the pixel's view.
+template<bool BATCH>
+__device__ __forceinline__ uint64_t local_index(const batch_dev& b, int32_t pix) {
+    if constexpr(BATCH) return uint64_t(int64_t(pix) - b.views[b.pixel_view[pix]].pixel_offset);
+    else return uint64_t(pix);
+}
+template<bool BATCH>
+__device__ __forceinline__ int32_t query_root(const scene_dev& s, const batch_dev& b, int32_t pix) {
+    if constexpr(BATCH) return b.views[b.pixel_view[pix]].wroot_ref;
+    else return s.wroot_ref;
+}
+template<bool BATCH>
+__device__ __forceinline__ light_set pixel_lights(const scene_dev& s, const batch_dev& b, int32_t pix) {
+    if constexpr(BATCH) {
+        const view_rec& v = b.views[b.pixel_view[pix]];
+        return light_set{s.lights + v.light_offset, v.light_count, v.light_area};
+    } else {
+        return scene_lights(s);
+    }
+}
+template<bool BATCH>
+__device__ __forceinline__ void pixel_camera_ray(const frame_params& f, const batch_dev& b, int32_t pix, rng_t& r, v3& o,
+                                                 v3& d, float& time) {
+    if constexpr(BATCH) {
+        const view_rec& v = b.views[b.pixel_view[pix]];
+        const int64_t local = int64_t(pix) - v.pixel_offset;
+        camera_ray(v.camera, int(local % v.width), int(local / v.width), v.width, v.height, r, o, d, time);
+    } else {
+        camera_ray(f.camera, int(pix % f.width), int(pix / f.width), f.width, f.height, r, o, d, time);
+    }
+}
+
+// Batch set-up: zero sums, PCG state of each pixel from its view's seed and
+// its index within the view (cuda_renderer.cu:64-68 per view).
+__global__ void batch_init_kernel(batch_dev b, float *sum, uint64_t *rng, int64_t count, loop_ctl *ctl) {
+    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if(i == 0) { ctl->failure = 0; ctl->rays = 0; }
+    if(i >= count) return;
+    const view_rec& v = b.views[b.pixel_view[i]];
+    sum[3 * i] = 0; sum[3 * i + 1] = 0; sum[3 * i + 2] = 0;
+    rng[i] = seed_state(v.seed, uint64_t(i - v.pixel_offset));
+}
+
+template<bool BATCH>
 __global__ void start_kernel(scene_dev s, frame_params f, paths_dev p, loop_ctl *ctl, int32_t samples,
-                             int part, int parts, int chunk, int64_t count) {
-    const int64_t n = int64_t(f.width) * f.height;
+                             int part, int parts, int chunk, int64_t count, int64_t n, batch_dev bd) {
     const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if(j == 0) {
         ctl->n_ext[0] = uint32_t(count); ctl->n_ext[1] = 0; ctl->n_shadow = 0; ctl->n_finish = 0; ctl->work = 0;
@@ -854,10 +902,9 @@ __global__ void start_kernel(scene_dev s, frame_params f, paths_dev p, loop_ctl 
     if(j >= count) return;
     const int64_t i = ((j / chunk) * parts + part) * chunk + j % chunk;
     if(i >= n) return;
-    rng_t r = pixel_rng(p.rng[i], uint64_t(i));
-    int x = int(i % f.width), y = int(i / f.width);
+    rng_t r = pixel_rng(p.rng[i], local_index<BATCH>(bd, int32_t(i)));
     v3 o, d; float time;
-    camera_ray(f.camera, x, y, f.width, f.height, r, o, d, time);
+    pixel_camera_ray<BATCH>(f, bd, int32_t(i), r, o, d, time);
     p.ray_o[i] = make_float4(o.x, o.y, o.z, time);
     p.ray_d[i] = make_float4(d.x, d.y, d.z, 0);
     p.thr[i] = make_float4(1, 1, 1, 0);
@@ -906,7 +953,7 @@ __global__ __launch_bounds__(256) void trace_kernel(scene_dev s, frame_params f,
             ray_in r = setup_ray(o.x, o.y, o.z, d.x, d.y, d.z);
             best_hit b;
             bool hit;
-            if(wide) hit = traverse_wide<false, COUNT>(s, r, 0.001f, FLT_MAX, wstk, b, boxes, prims, spheres, error);
+            if(wide) hit = traverse_wide<false, COUNT>(s, s.wroot_ref, r, 0.001f, FLT_MAX, wstk, b, boxes, prims, spheres, error);
             else hit = COUNT ? trace_counted<false>(s, brute, r, 0.001f, FLT_MAX, stk, stride, b, boxes, prims, spheres, error)
                              : trace<false, false>(s, brute, r, 0.001f, FLT_MAX, stk, stride, b, boxes, prims, error);
             p.hit[pix] = hit ? make_int4(__float_as_int(b.t), __float_as_int(b.u), __float_as_int(b.v), b.code)
@@ -917,7 +964,7 @@ __global__ __launch_bounds__(256) void trace_kernel(scene_dev s, frame_params f,
             ray_in r = setup_ray(o.x, o.y, o.z, d.x, d.y, d.z);
             best_hit b;
             bool blocked;
-            if(wide) blocked = traverse_wide<true, COUNT>(s, r, 0.001f, o.w, wstk, b, boxes, prims, spheres, error);
+            if(wide) blocked = traverse_wide<true, COUNT>(s, s.wroot_ref, r, 0.001f, o.w, wstk, b, boxes, prims, spheres, error);
             else blocked = COUNT ? trace_counted<true>(s, brute, r, 0.001f, o.w, stk, stride, b, boxes, prims, spheres, error)
                                  : trace<true, false>(s, brute, r, 0.001f, o.w, stk, stride, b, boxes, prims, error);
             if(!blocked) {
@@ -943,9 +990,9 @@ __global__ __launch_bounds__(256) void trace_kernel(scene_dev s, frame_params f,
 // `refill` lanes of a wave are idle, the wave claims that many new queries
 // (one atomic) and hands them to its idle lanes, so a wave does not wait for
 // its slowest query before taking new work.
-template<bool COUNT>
+template<bool COUNT, bool BATCH>
 __global__ __launch_bounds__(256) void trace_lanes_kernel(scene_dev s, frame_params f, paths_dev p, loop_ctl *ctl,
-                                                          int parity, int refill) {
+                                                          int parity, int refill, batch_dev bd) {
     extern __shared__ int32_t lds[];
     const uint32_t n_ext = ctl->n_ext[parity];
     const uint32_t total = n_ext + ctl->n_shadow;
@@ -974,12 +1021,12 @@ __global__ __launch_bounds__(256) void trace_lanes_kernel(scene_dev s, frame_par
                         pix = queue[item];
                         const float4 o = p.ray_o[pix], d = p.ray_d[pix];
                         q.r = setup_ray(o.x, o.y, o.z, d.x, d.y, d.z);
-                        wide_begin(s, q, false, FLT_MAX);
+                        wide_begin(query_root<BATCH>(s, bd, pix), q, false, FLT_MAX);
                     } else {
                         pix = p.shadow_queue[item - n_ext];
                         const float4 o = p.sh_o[pix], d = p.sh_d[pix];
                         q.r = setup_ray(o.x, o.y, o.z, d.x, d.y, d.z);
-                        wide_begin(s, q, true, o.w);
+                        wide_begin(query_root<BATCH>(s, bd, pix), q, true, o.w);
                     }
                     active = true;
                 }
@@ -1081,29 +1128,29 @@ __device__ void record_aov(const scene_dev& s, const hit_rec& rec, const pt_flat
 // as the next ray, false when the sample has ended with st.radiance.  A next
 // event estimation query is returned in `sq` (has_shadow); its contribution
 // must be added to st.radiance if unblocked, before the next bounce.
-__device__ bool shade_bounce(const scene_dev& s, const frame_params& f, path_state& st, bool hit, const best_hit& b,
-                             rng_t& r, shadow_query& sq, bool& has_shadow, float *aov) {
+__device__ bool shade_bounce(const scene_dev& s, const frame_params& f, const light_set& ls, path_state& st, bool hit,
+                             const best_hit& b, rng_t& r, shadow_query& sq, bool& has_shadow, float *aov) {
     has_shadow = false;
     if(!hit) return false;
     hit_rec rec = make_hit(s, st.o, st.d, b);
     const pt_flat_material mat = s.materials[rec.material_id];
     if(aov && st.depth == 0) record_aov(s, rec, mat, aov);
     if(mat.type == PT_LIGHT) {
-        float weight = st.previous_pdf > 0 ? mis_weight(st.previous_pdf, hit_light_pdf(s, st.o, rec)) : 1;
+        float weight = st.previous_pdf > 0 ? mis_weight(st.previous_pdf, hit_light_pdf(s, ls.area, st.o, rec)) : 1;
         st.radiance = add(st.radiance, mul(scale(weight, st.throughput), trace_texture(s, mat.texture_id, rec.u, rec.v)));
         return false;
     }
     if(st.depth == f.max_depth) return false;
     if(f.direct_light && mat.type == PT_DIFFUSE) {
         v3 attenuation = trace_texture(s, mat.texture_id, rec.u, rec.v);
-        if(s.light_count) {
-            light_sample light = sample_light(s, r);
+        if(ls.count) {
+            light_sample light = sample_light(s, ls, r);
             v3 delta = sub(light.point, rec.p);
             float distance = length(delta);
             if(distance > 0.002f && isfinite(distance)) {
                 v3 direction = divs(delta, distance);
                 float cosine = fmaxf(0, dot(rec.normal, direction));
-                float pdf = light_direction_pdf(s.light_area, rec.p, light.point, light.normal);
+                float pdf = light_direction_pdf(ls.area, rec.p, light.point, light.normal);
                 if(cosine > 0 && pdf > 0) {
                     const pt_flat_material emitter = s.materials[light.material];
                     v3 emission = trace_texture(s, emitter.texture_id, light.u, light.v);
@@ -1132,7 +1179,9 @@ __device__ bool shade_bounce(const scene_dev& s, const frame_params& f, path_sta
 }
 
 // One bounce for every path in the extension queue.
-__global__ __launch_bounds__(1024) void shade_kernel(scene_dev s, frame_params f, paths_dev p, loop_ctl *ctl, int parity) {
+template<bool BATCH>
+__global__ __launch_bounds__(1024) void shade_kernel(scene_dev s, frame_params f, paths_dev p, loop_ctl *ctl, int parity,
+                                                     batch_dev bd) {
     const uint32_t n = ctl->n_ext[parity];
     const int32_t *q = p.queue[parity];
     int32_t *qn = p.queue[parity ^ 1];
@@ -1145,10 +1194,11 @@ __global__ __launch_bounds__(1024) void shade_kernel(scene_dev s, frame_params f
             pix = q[item];
             const int4 hb = p.hit[pix];
             path_state st = load_path(p, pix);
-            rng_t r = pixel_rng(p.rng[pix], uint64_t(pix));
+            rng_t r = pixel_rng(p.rng[pix], local_index<BATCH>(bd, pix));
             const best_hit b{__int_as_float(hb.x), __int_as_float(hb.y), __int_as_float(hb.z), hb.w};
             shadow_query sq;
-            cont = shade_bounce(s, f, st, hb.w >= 0, b, r, sq, shadow, p.aov ? p.aov + 6 * size_t(pix) : nullptr);
+            cont = shade_bounce(s, f, pixel_lights<BATCH>(s, bd, pix), st, hb.w >= 0, b, r, sq, shadow,
+                                p.aov ? p.aov + 6 * size_t(pix) : nullptr);
             if(shadow) {
                 p.sh_o[pix] = make_float4(sq.o.x, sq.o.y, sq.o.z, sq.t_max);
                 p.sh_d[pix] = make_float4(sq.d.x, sq.d.y, sq.d.z, 0);
@@ -1165,16 +1215,19 @@ __global__ __launch_bounds__(1024) void shade_kernel(scene_dev s, frame_params f
 }
 
 template<bool ANY_HIT>
-__device__ __forceinline__ bool trace_query(const scene_dev& s, bool brute, bool wide, const ray_in& r, float high,
-                                            int32_t *stk, int stride, wide_stack& wstk, best_hit& b, int& error) {
+__device__ __forceinline__ bool trace_query(const scene_dev& s, int32_t root, bool brute, bool wide, const ray_in& r,
+                                            float high, int32_t *stk, int stride, wide_stack& wstk, best_hit& b,
+                                            int& error) {
     int boxes = 0, prims = 0, spheres = 0;
-    if(wide) return traverse_wide<ANY_HIT, false>(s, r, 0.001f, high, wstk, b, boxes, prims, spheres, error);
+    if(wide) return traverse_wide<ANY_HIT, false>(s, root, r, 0.001f, high, wstk, b, boxes, prims, spheres, error);
     return trace<ANY_HIT, false>(s, brute, r, 0.001f, high, stk, stride, b, boxes, prims, error);
 }
 
 // Tail of an add_samples call, part 1: the shadow queries still pending
 // from the last wavefront bounce.
-__global__ __launch_bounds__(256) void drain_shadow_kernel(scene_dev s, frame_params f, paths_dev p, loop_ctl *ctl) {
+template<bool BATCH>
+__global__ __launch_bounds__(256) void drain_shadow_kernel(scene_dev s, frame_params f, paths_dev p, loop_ctl *ctl,
+                                                           batch_dev bd) {
     extern __shared__ int32_t lds[];
     const uint32_t n = ctl->n_shadow;
     const bool brute = f.brute != 0;
@@ -1189,7 +1242,8 @@ __global__ __launch_bounds__(256) void drain_shadow_kernel(scene_dev s, frame_pa
         const int32_t pix = p.shadow_queue[item];
         const float4 o = p.sh_o[pix], d = p.sh_d[pix];
         best_hit b;
-        if(!trace_query<true>(s, brute, wide, setup_ray(o.x, o.y, o.z, d.x, d.y, d.z), o.w, stk, stride, wstk, b, error)) {
+        if(!trace_query<true>(s, query_root<BATCH>(s, bd, pix), brute, wide, setup_ray(o.x, o.y, o.z, d.x, d.y, d.z), o.w,
+                              stk, stride, wstk, b, error)) {
             float4 c = p.sh_c[pix], ra = p.rad[pix];
             ra.x = ra.x + c.x; ra.y = ra.y + c.y; ra.z = ra.z + c.z;
             p.rad[pix] = ra;
@@ -1207,7 +1261,9 @@ __global__ __launch_bounds__(256) void drain_shadow_kernel(scene_dev s, frame_pa
 // consumes its PCG stream and accumulates exactly as in the wavefront loop.
 // Used once few paths are live, where a bounce iteration costs as much as
 // its slowest query whatever the queue length.
-__global__ __launch_bounds__(256) void drain_kernel(scene_dev s, frame_params f, paths_dev p, loop_ctl *ctl, int parity) {
+template<bool BATCH>
+__global__ __launch_bounds__(256) void drain_kernel(scene_dev s, frame_params f, paths_dev p, loop_ctl *ctl, int parity,
+                                                    batch_dev bd) {
     extern __shared__ int32_t lds[];
     const uint32_t n = ctl->n_ext[parity];
     const int32_t *q = p.queue[parity];
@@ -1223,28 +1279,29 @@ __global__ __launch_bounds__(256) void drain_kernel(scene_dev s, frame_params f,
     for(uint32_t item = blockIdx.x * blockDim.x + threadIdx.x; item < n; item += step) {
         const int32_t pix = q[item];
         path_state st = load_path(p, pix);
-        rng_t r = pixel_rng(p.rng[pix], uint64_t(pix));
+        rng_t r = pixel_rng(p.rng[pix], local_index<BATCH>(bd, pix));
+        const int32_t root = query_root<BATCH>(s, bd, pix);
+        const light_set ls = pixel_lights<BATCH>(s, bd, pix);
         float *sum = p.sum + 3 * size_t(pix);
         for(;;) {
             best_hit b;
-            const bool hit = trace_query<false>(s, brute, wide, setup_ray(st.o.x, st.o.y, st.o.z, st.d.x, st.d.y, st.d.z),
+            const bool hit = trace_query<false>(s, root, brute, wide, setup_ray(st.o.x, st.o.y, st.o.z, st.d.x, st.d.y, st.d.z),
                                                 FLT_MAX, stk, stride, wstk, b, error);
             ext++;
             shadow_query sq;
             bool has_shadow;
-            const bool cont = shade_bounce(s, f, st, hit, b, r, sq, has_shadow, p.aov ? p.aov + 6 * size_t(pix) : nullptr);
+            const bool cont = shade_bounce(s, f, ls, st, hit, b, r, sq, has_shadow, p.aov ? p.aov + 6 * size_t(pix) : nullptr);
             if(has_shadow) {
                 shadow++;
                 best_hit bb;
-                if(!trace_query<true>(s, brute, wide, setup_ray(sq.o.x, sq.o.y, sq.o.z, sq.d.x, sq.d.y, sq.d.z), sq.t_max,
-                                      stk, stride, wstk, bb, error))
+                if(!trace_query<true>(s, root, brute, wide, setup_ray(sq.o.x, sq.o.y, sq.o.z, sq.d.x, sq.d.y, sq.d.z),
+                                      sq.t_max, stk, stride, wstk, bb, error))
                     st.radiance = add(st.radiance, sq.contribution);
             }
             if(cont) continue;
             sum[0] = sum[0] + st.radiance.x; sum[1] = sum[1] + st.radiance.y; sum[2] = sum[2] + st.radiance.z;
             if(--st.left <= 0) break;
-            const int x = int(pix % f.width), y = int(pix / f.width);
-            camera_ray(f.camera, x, y, f.width, f.height, r, st.o, st.d, st.time);
+            pixel_camera_ray<BATCH>(f, bd, pix, r, st.o, st.d, st.time);
             st.throughput = mk(1, 1, 1); st.radiance = mk(0, 0, 0); st.previous_pdf = 0; st.depth = 0;
         }
         p.rng[pix] = r.state;
@@ -1261,7 +1318,8 @@ __global__ __launch_bounds__(256) void drain_kernel(scene_dev s, frame_params f,
 
 // End of a sample (cuda_renderer.cu:81): accumulated += radiance in sample
 // order, then the pixel's next camera ray while samples remain.
-__global__ __launch_bounds__(1024) void regen_kernel(frame_params f, paths_dev p, loop_ctl *ctl, int parity) {
+template<bool BATCH>
+__global__ __launch_bounds__(1024) void regen_kernel(frame_params f, paths_dev p, loop_ctl *ctl, int parity, batch_dev bd) {
     const uint32_t n = ctl->n_finish;
     int32_t *qn = p.queue[parity ^ 1];
     const uint32_t step = gridDim.x * blockDim.x;
@@ -1276,10 +1334,9 @@ __global__ __launch_bounds__(1024) void regen_kernel(frame_params f, paths_dev p
             sp[0] = sp[0] + ra.x; sp[1] = sp[1] + ra.y; sp[2] = sp[2] + ra.z;
             int left = p.ctr[pix].y - 1;
             if(left > 0) {
-                rng_t r = pixel_rng(p.rng[pix], uint64_t(pix));
-                int x = int(pix % f.width), y = int(pix / f.width);
+                rng_t r = pixel_rng(p.rng[pix], local_index<BATCH>(bd, pix));
                 v3 o, d; float time;
-                camera_ray(f.camera, x, y, f.width, f.height, r, o, d, time);
+                pixel_camera_ray<BATCH>(f, bd, pix, r, o, d, time);
                 p.rng[pix] = r.state;
                 p.ray_o[pix] = make_float4(o.x, o.y, o.z, time);
                 p.ray_d[pix] = make_float4(d.x, d.y, d.z, 0);
@@ -1336,10 +1393,19 @@ void launch_debug(const scene_dev& s, const frame_params& f, float *sum, uint64_
                        s, f, sum, rng, samples, failure);
 }
 void launch_start(const scene_dev& s, const frame_params& f, const paths_dev& p, loop_ctl *ctl, int32_t samples,
-                  int part, int parts, int chunk, int64_t count, hipStream_t stream) {
+                  int part, int parts, int chunk, int64_t count, hipStream_t stream, const batch_dev *batch) {
     unsigned grid = unsigned((count + 255) / 256);
-    hipLaunchKernelGGL(start_kernel, dim3(grid ? grid : 1), dim3(256), 0, stream, s, f, p, ctl, samples, part, parts,
-                       chunk, count);
+    const int64_t n = batch ? count : int64_t(f.width) * f.height;
+    if(batch)
+        hipLaunchKernelGGL(start_kernel<true>, dim3(grid ? grid : 1), dim3(256), 0, stream, s, f, p, ctl, samples, part,
+                           parts, chunk, count, n, *batch);
+    else
+        hipLaunchKernelGGL(start_kernel<false>, dim3(grid ? grid : 1), dim3(256), 0, stream, s, f, p, ctl, samples, part,
+                           parts, chunk, count, n, batch_dev{});
+}
+void launch_batch_init(const batch_dev& b, float *sum, uint64_t *rng, int64_t count, loop_ctl *ctl, hipStream_t stream) {
+    unsigned grid = unsigned((count + 255) / 256);
+    hipLaunchKernelGGL(batch_init_kernel, dim3(grid ? grid : 1), dim3(256), 0, stream, b, sum, rng, count, ctl);
 }
 void launch_clear_ctl(loop_ctl *ctl, hipStream_t stream) {
     hipLaunchKernelGGL(clear_ctl_kernel, dim3(1), dim3(1), 0, stream, ctl);
@@ -1353,36 +1419,55 @@ void launch_trace(const scene_dev& s, const frame_params& f, const paths_dev& p,
         hipLaunchKernelGGL(trace_kernel<false>, dim3(unsigned(grid)), dim3(unsigned(block)), lds, stream, s, f, p, ctl, parity);
 }
 void launch_trace_lanes(const scene_dev& s, const frame_params& f, const paths_dev& p, loop_ctl *ctl, int parity,
-                        int grid, int block, int refill, bool count_tests, hipStream_t stream) {
+                        int grid, int block, int refill, bool count_tests, hipStream_t stream, const batch_dev *batch) {
     size_t lds = wide_stack_bytes(s.wide_stack_capacity, block);
-    if(count_tests)
-        hipLaunchKernelGGL(trace_lanes_kernel<true>, dim3(unsigned(grid)), dim3(unsigned(block)), lds, stream, s, f, p, ctl,
-                           parity, refill);
-    else
-        hipLaunchKernelGGL(trace_lanes_kernel<false>, dim3(unsigned(grid)), dim3(unsigned(block)), lds, stream, s, f, p, ctl,
-                           parity, refill);
+    const dim3 g{unsigned(grid)}, b{unsigned(block)};
+    if(batch) {
+        if(count_tests) hipLaunchKernelGGL((trace_lanes_kernel<true, true>), g, b, lds, stream, s, f, p, ctl, parity, refill, *batch);
+        else hipLaunchKernelGGL((trace_lanes_kernel<false, true>), g, b, lds, stream, s, f, p, ctl, parity, refill, *batch);
+    } else {
+        if(count_tests)
+            hipLaunchKernelGGL((trace_lanes_kernel<true, false>), g, b, lds, stream, s, f, p, ctl, parity, refill, batch_dev{});
+        else
+            hipLaunchKernelGGL((trace_lanes_kernel<false, false>), g, b, lds, stream, s, f, p, ctl, parity, refill, batch_dev{});
+    }
 }
 void launch_bookkeep(loop_ctl *ctl, int parity, hipStream_t stream) {
     hipLaunchKernelGGL(bookkeep_kernel, dim3(1), dim3(1), 0, stream, ctl, parity);
 }
 void launch_shade(const scene_dev& s, const frame_params& f, const paths_dev& p, loop_ctl *ctl, int parity,
-                  int grid, int block, hipStream_t stream) {
-    hipLaunchKernelGGL(shade_kernel, dim3(unsigned(grid)), dim3(unsigned(block)), 0, stream, s, f, p, ctl, parity);
+                  int grid, int block, hipStream_t stream, const batch_dev *batch) {
+    if(batch)
+        hipLaunchKernelGGL(shade_kernel<true>, dim3(unsigned(grid)), dim3(unsigned(block)), 0, stream, s, f, p, ctl, parity,
+                           *batch);
+    else
+        hipLaunchKernelGGL(shade_kernel<false>, dim3(unsigned(grid)), dim3(unsigned(block)), 0, stream, s, f, p, ctl, parity,
+                           batch_dev{});
 }
 void launch_regen(const frame_params& f, const paths_dev& p, loop_ctl *ctl, int parity, int grid, int block,
-                  hipStream_t stream) {
-    hipLaunchKernelGGL(regen_kernel, dim3(unsigned(grid)), dim3(unsigned(block)), 0, stream, f, p, ctl, parity);
+                  hipStream_t stream, const batch_dev *batch) {
+    if(batch)
+        hipLaunchKernelGGL(regen_kernel<true>, dim3(unsigned(grid)), dim3(unsigned(block)), 0, stream, f, p, ctl, parity,
+                           *batch);
+    else
+        hipLaunchKernelGGL(regen_kernel<false>, dim3(unsigned(grid)), dim3(unsigned(block)), 0, stream, f, p, ctl, parity,
+                           batch_dev{});
 }
 void launch_drain(const scene_dev& s, const frame_params& f, const paths_dev& p, loop_ctl *ctl, int parity,
-                  uint32_t shadows, uint32_t paths, int block, hipStream_t stream) {
+                  uint32_t shadows, uint32_t paths, int block, hipStream_t stream, const batch_dev *batch) {
     size_t lds = trace_lds_bytes(s, f, block);
+    const batch_dev bd = batch ? *batch : batch_dev{};
     if(shadows) {
         unsigned grid = unsigned((shadows + uint32_t(block) - 1) / uint32_t(block));
-        hipLaunchKernelGGL(drain_shadow_kernel, dim3(grid), dim3(unsigned(block)), lds, stream, s, f, p, ctl);
+        if(batch) hipLaunchKernelGGL(drain_shadow_kernel<true>, dim3(grid), dim3(unsigned(block)), lds, stream, s, f, p, ctl, bd);
+        else hipLaunchKernelGGL(drain_shadow_kernel<false>, dim3(grid), dim3(unsigned(block)), lds, stream, s, f, p, ctl, bd);
     }
     if(paths) {
         unsigned grid = unsigned((paths + uint32_t(block) - 1) / uint32_t(block));
-        hipLaunchKernelGGL(drain_kernel, dim3(grid), dim3(unsigned(block)), lds, stream, s, f, p, ctl, parity);
+        if(batch)
+            hipLaunchKernelGGL(drain_kernel<true>, dim3(grid), dim3(unsigned(block)), lds, stream, s, f, p, ctl, parity, bd);
+        else
+            hipLaunchKernelGGL(drain_kernel<false>, dim3(grid), dim3(unsigned(block)), lds, stream, s, f, p, ctl, parity, bd);
     }
 }
 void launch_pack(const float *sum, uint8_t *rgba, int64_t count, int32_t samples, hipStream_t stream) {

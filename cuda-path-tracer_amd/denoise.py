@@ -177,19 +177,22 @@ def render_features(scene, camera, width, height, seed, spp, depth=8):
     return r.image(), albedo, normal
 
 
-def make_training_set(views=16, frame=512, crop=128, spp_in=4, spp_target=1024, seed=11, log=print):
-    """Crops of rendered views: features (crops, crop, crop, 9) and targets (crops, crop, crop, 3)."""
+def make_training_set(views=16, frame=512, crop=128, spp_in=(1, 4, 16), spp_target=1024, seed=11, log=print):
+    """Crops of rendered views: features (crops, crop, crop, 9) and targets
+    (crops, crop, crop, 3).  Each view gives one input per sample count in
+    spp_in (independent seeds), all against the same high-spp target."""
     feats, targets = [], []
     t0 = time.perf_counter()
     for i, (scene, cam) in enumerate(training_views(views, seed, frame)):
-        rad, alb, nrm = render_features(scene, cam, frame, frame, 1000 + i, spp_in)
         ref, _, _ = render_features(scene, cam, frame, frame, 5000 + i, spp_target)
-        f = np.concatenate([rad, alb, nrm], axis=1).reshape(frame, frame, 9)
         t = ref.reshape(frame, frame, 3)
-        for y in range(0, frame, crop):
-            for x in range(0, frame, crop):
-                feats.append(f[y:y + crop, x:x + crop])
-                targets.append(t[y:y + crop, x:x + crop])
+        for k, spp in enumerate(spp_in):
+            rad, alb, nrm = render_features(scene, cam, frame, frame, 1000 + 31 * i + k, spp)
+            f = np.concatenate([rad, alb, nrm], axis=1).reshape(frame, frame, 9)
+            for y in range(0, frame, crop):
+                for x in range(0, frame, crop):
+                    feats.append(f[y:y + crop, x:x + crop])
+                    targets.append(t[y:y + crop, x:x + crop])
         log(f"view {i}: {time.perf_counter() - t0:.1f}s")
     return np.stack(feats).astype(np.float32), np.stack(targets).astype(np.float32)
 
@@ -259,12 +262,12 @@ def main():
     sub = ap.add_subparsers(dest="cmd", required=True)
     t = sub.add_parser("train")
     t.add_argument("--out", default=WEIGHTS_PATH)
-    t.add_argument("--views", type=int, default=16)
+    t.add_argument("--views", type=int, default=48)
     t.add_argument("--frame", type=int, default=512)
-    t.add_argument("--crop", type=int, default=128)
-    t.add_argument("--spp-in", type=int, default=4)
-    t.add_argument("--spp-target", type=int, default=1024)
-    t.add_argument("--steps", type=int, default=2000)
+    t.add_argument("--crop", type=int, default=256)
+    t.add_argument("--spp-in", type=int, nargs="+", default=[1, 4, 16])
+    t.add_argument("--spp-target", type=int, default=2048)
+    t.add_argument("--steps", type=int, default=3000)
     t.add_argument("--batch", type=int, default=16)
     t.add_argument("--lr", type=float, default=3e-4)
     e = sub.add_parser("evaluate")
@@ -276,7 +279,8 @@ def main():
     args = ap.parse_args()
     log = lambda *a: print(*a, flush=True)  # noqa: E731
     if args.cmd == "train":
-        feats, targets = make_training_set(args.views, args.frame, args.crop, args.spp_in, args.spp_target, log=log)
+        feats, targets = make_training_set(args.views, args.frame, args.crop, tuple(args.spp_in), args.spp_target, log=log)
+        log(f"training set: {len(feats)} crops of {args.crop}x{args.crop}")
         net = Denoiser(seed=1)
         history, ms = train(net, feats, targets, args.steps, args.batch, args.lr, log=log)
         net.save(args.out, loss=history, device_ms=ms, config=json.dumps(vars(args)))

@@ -109,6 +109,10 @@ class Tuning(ctypes.Structure):
                 ("partitions", ctypes.c_int32)]
 
 
+class BatchTrial(ctypes.Structure):
+    _fields_ = [("tuning", Tuning), ("mrays", ctypes.c_float)]
+
+
 def _ptr(a):
     return ctypes.c_void_p(a.ctypes.data) if a.size else ctypes.c_void_p(0)
 
@@ -150,6 +154,18 @@ PT_FUNCTIONS = {
     "pt_renderer_iteration_log": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
                                                  ctypes.c_void_p]),
     "pt_renderer_enable_aovs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "pt_batch_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
+                                       ctypes.c_int32, ctypes.c_void_p]),
+    "pt_batch_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "pt_batch_reset": (ctypes.c_int, [ctypes.c_void_p]),
+    "pt_batch_add_samples": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "pt_batch_read_image": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, ctypes.c_size_t]),
+    "pt_batch_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "pt_batch_last_launch_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "pt_batch_get_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "pt_batch_set_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "pt_batch_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p,
+                                         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p]),
     "pt_renderer_read_aovs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]),
 }
 
@@ -339,6 +355,78 @@ class Renderer:
                    int(bool(count_tests)), int(bool(exact_traversal)), int(drain_threshold), int(lane_refill),
                    int(partitions))
         _check(lib().pt_renderer_set_tuning(self._h, ctypes.byref(t)))
+
+
+
+BATCH_VIEW = np.dtype([("seed", "<u8"), ("scene", "<i4"), ("width", "<i4"), ("height", "<i4"), ("camera", CAMERA)])
+assert BATCH_VIEW.itemsize == 104
+
+
+class Batch:
+    """Batched rendering of many views of many scenes in one graph-captured
+    bounce loop (pt_batch_* in include/pt_render.h).
+
+    scenes: list of FlatScene; views: list of (scene index, camera, width,
+    height, seed).  Each view's image equals a single Renderer's."""
+
+    def __init__(self, scenes, views, direct_light=True, max_depth=0):
+        self._scenes = [s.normalized() for s in scenes]
+        descs = (_SceneDesc * len(scenes))(*[scene_desc(s) for s in self._scenes])
+        v = np.zeros(len(views), BATCH_VIEW)
+        for i, (scene, cam, w, h, seed) in enumerate(views):
+            v[i]["scene"], v[i]["width"], v[i]["height"], v[i]["seed"] = scene, w, h, seed
+            v[i]["camera"] = np.asarray(cam, CAMERA)
+        self.views = v
+        h = ctypes.c_void_p()
+        _check(lib().pt_batch_create(descs, len(scenes), v.ctypes.data, len(v), int(bool(direct_light)), int(max_depth),
+                                     ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().pt_batch_destroy(self._h)
+            self._h = None
+
+    def reset(self):
+        _check(lib().pt_batch_reset(self._h))
+
+    def add_samples(self, count):
+        _check(lib().pt_batch_add_samples(self._h, int(count)))
+
+    def image(self, view):
+        w, h = int(self.views[view]["width"]), int(self.views[view]["height"])
+        out = np.zeros((w * h, 3), np.float32)
+        _check(lib().pt_batch_read_image(self._h, int(view), out.ctypes.data, out.size))
+        return out
+
+    def stats(self):
+        n, ms, rays = ctypes.c_int32(), ctypes.c_float(), ctypes.c_uint64()
+        _check(lib().pt_batch_stats(self._h, ctypes.byref(n), ctypes.byref(ms), ctypes.byref(rays)))
+        return {"samples": n.value, "milliseconds": ms.value, "rays": rays.value}
+
+    def last_launch_stats(self):
+        st = LaunchStats()
+        _check(lib().pt_batch_last_launch_stats(self._h, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in LaunchStats._fields_}
+
+    def tuning(self):
+        t = Tuning()
+        _check(lib().pt_batch_get_tuning(self._h, ctypes.byref(t)))
+        return {k: getattr(t, k) for k, _ in Tuning._fields_}
+
+    def set_tuning(self, **fields):
+        t = Tuning(**{k: int(v) for k, v in fields.items()})
+        _check(lib().pt_batch_set_tuning(self._h, ctypes.byref(t)))
+
+    def autotune(self, probe_samples=1, passes=1, capacity=256):
+        """Coordinate-descent autotuner; returns (best tuning, best Mrays/s, trials)."""
+        best, rate, count = Tuning(), ctypes.c_float(), ctypes.c_int32()
+        trials = (BatchTrial * capacity)()
+        _check(lib().pt_batch_autotune(self._h, int(probe_samples), int(passes), ctypes.byref(best), ctypes.byref(rate),
+                                       trials, capacity, ctypes.byref(count)))
+        rows = [dict({k: getattr(t.tuning, k) for k, _ in Tuning._fields_}, mrays=t.mrays)
+                for t in trials[:min(count.value, capacity)]]
+        return {k: getattr(best, k) for k, _ in Tuning._fields_}, rate.value, rows
 
 
 def query_rays(scene, rays, brute=False, stats=False):

@@ -207,6 +207,46 @@ typedef struct pt_tuning {
 } pt_tuning;
 pt_status pt_renderer_set_tuning(pt_renderer *renderer, const pt_tuning *tuning);
 
+/* ---------------- batched rendering (no counterpart in the reference) ----------------
+ * Many views of many scenes rendered by one persistent wavefront bounce
+ * loop, captured once into a HIP graph and replayed: the scenes' arrays are
+ * merged on the device, each scene keeping its own hierarchy and lights, and
+ * the pixels of all views share the path queues.  Shaded mode, 4-wide
+ * traversal.  Each view's image equals a single renderer's image of it
+ * (same scene, camera, size and seed). */
+typedef struct pt_batch pt_batch;
+typedef struct pt_batch_view {
+    uint64_t seed;           /* the view's render seed (pt_settings.seed of a single render) */
+    int32_t scene;           /* index into the scene array */
+    int32_t width, height;
+    pt_flat_camera camera;
+} pt_batch_view;             /* 104 B */
+typedef struct pt_batch_trial { pt_tuning tuning; float mrays; } pt_batch_trial;
+
+pt_status pt_batch_create(const pt_scene_desc *scenes, int32_t scene_count, const pt_batch_view *views,
+                          int32_t view_count, int32_t direct_light, int32_t max_path_depth, pt_batch **out);
+pt_status pt_batch_destroy(pt_batch *batch);
+/* Zero every view's accumulation and reseed its PCG streams. */
+pt_status pt_batch_reset(pt_batch *batch);
+/* `count` more samples for every pixel of every view; synchronous. */
+pt_status pt_batch_add_samples(pt_batch *batch, int32_t count);
+/* Averaged linear RGB of one view (3 * width * height floats, row 0 at the bottom). */
+pt_status pt_batch_read_image(const pt_batch *batch, int32_t view, float *rgb, size_t count);
+pt_status pt_batch_stats(const pt_batch *batch, int32_t *samples, float *milliseconds, uint64_t *rays);
+pt_status pt_batch_last_launch_stats(const pt_batch *batch, pt_launch_stats *stats);
+/* Fields used: trace_block, shade_block, blocks_per_cu, check_interval (bounce
+ * iterations per graph replay, rounded down to even), drain_threshold (live
+ * paths at which the tail kernel takes over; -1 never), lane_refill. */
+pt_status pt_batch_get_tuning(const pt_batch *batch, pt_tuning *tuning);
+pt_status pt_batch_set_tuning(pt_batch *batch, const pt_tuning *tuning);
+/* Autotuner: coordinate descent over workgroup sizes, persistent blocks per
+ * CU, lane refill, the drain (compaction) threshold and the graph length,
+ * each candidate timed on probe_samples samples of the whole batch; `passes`
+ * sweeps.  Adopts the best tuning, resets the accumulation, and reports the
+ * best rate in Mrays/s and up to trial_capacity trials. */
+pt_status pt_batch_autotune(pt_batch *batch, int32_t probe_samples, int32_t passes, pt_tuning *best, float *best_mrays,
+                            pt_batch_trial *trials, int32_t trial_capacity, int32_t *trial_count);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,64 @@
+"""Batched rendering (pt_batch_*): many views of many scenes in one
+graph-captured bounce loop.  Every view must equal what a single renderer
+makes of the same scene, camera, size and seed, bit for bit."""
+import numpy as np
+import pytest
+
+import ptrace
+from conftest import golden
+
+
+def _scenes():
+    a, ca = ptrace.cornell(3, 40, 30)
+    b, cb = ptrace.soup(6000, 2, 48, 32)
+    c, cc = ptrace.icosphere(20, 5, 36, 36)
+    d, cd, _ = golden("asset_sphere.npz")
+    return [a, b, c, d], [ca, cb, cc, cd]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("depth,direct", [(6, True), (8, False)])
+def test_batch_views_match_single_renderers(depth, direct):
+    scenes, cams = _scenes()
+    views = [(0, cams[0], 40, 30, 7), (1, cams[1], 48, 32, 9), (2, cams[2], 36, 36, 3), (3, cams[3], 20, 14, 5),
+             (0, cams[0], 24, 18, 11)]
+    batch = ptrace.Batch(scenes, views, direct_light=direct, max_depth=depth)
+    batch.add_samples(3)
+    batch.add_samples(2)
+    total = 0
+    for k, (si, cam, w, h, seed) in enumerate(views):
+        r = ptrace.Renderer(scenes[si], cam, w, h, seed, False, ptrace.DEBUG_SHADED, direct, depth)
+        r.add_samples(3)
+        r.add_samples(2)
+        assert np.array_equal(batch.image(k), r.image()), k
+        total += r.rays_traced()
+    st = batch.stats()
+    assert st["samples"] == 5 and st["rays"] == total
+
+
+@pytest.mark.gpu
+def test_batch_tuning_and_autotuner_keep_images():
+    scenes, cams = _scenes()
+    views = [(0, cams[0], 40, 30, 7), (1, cams[1], 48, 32, 9), (2, cams[2], 36, 36, 3)]
+    batch = ptrace.Batch(scenes[:3], views, direct_light=True, max_depth=6)
+    batch.add_samples(4)
+    ref = [batch.image(k) for k in range(3)]
+    best, rate, trials = batch.autotune(probe_samples=1, passes=1)
+    assert rate > 0 and len(trials) >= 10 and best["trace_block"] in (64, 128, 192, 256)
+    assert batch.stats()["samples"] == 0
+    batch.add_samples(4)
+    for k in range(3):
+        assert np.array_equal(batch.image(k), ref[k]), k
+    batch.reset()
+    batch.set_tuning(trace_block=64, shade_block=128, check_interval=2, drain_threshold=-1, lane_refill=8)
+    batch.add_samples(4)
+    for k in range(3):
+        assert np.array_equal(batch.image(k), ref[k]), k
+
+
+def test_batch_argument_checks():
+    scenes, cams = _scenes()
+    with pytest.raises(ptrace.InvalidArgument):
+        ptrace.Batch(scenes[:1], [(1, cams[0], 8, 8, 1)])
+    with pytest.raises(ptrace.InvalidArgument):
+        ptrace.Batch(scenes[:1], [(0, cams[0], 0, 8, 1)])
