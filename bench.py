@@ -11,8 +11,8 @@ inputs resident in HBM.  ``value`` is rays per second over the timed steps,
 counting every closest-hit and shadow query (the reference's rays_traced).
 
 Also reported:
-  roofline     -- trace kernel (closest-hit + any-hit traversal of the bounce
-                  loop), HBM-bound: algorithmic bytes of its launches /
+  roofline     -- trace kernel (trace_lanes_kernel: closest-hit + any-hit
+                  traversal of the bounce loop on persistent lanes), HBM-bound: algorithmic bytes of its launches /
                   HIP-event time of those launches on the renderer's stream,
                   against 8 TB/s.  Bytes per query = 32*box tests +
                   36*triangle tests + 16*sphere tests + 48 (closest hit) or
@@ -21,8 +21,8 @@ Also reported:
                   (committed summary under profiles/), scaled by a plain-copy
                   calibration.
   cpu_baseline -- the CPU oracle (a C restatement of the reference's host
-                  transport path) on the host's cores over a 1-spp frame of
-                  the same scene and camera.
+                  transport path) on the host's cores over a 40-spp frame of
+                  the same scene and camera (about 10 s on 16 threads).
 """
 import argparse
 import json
@@ -216,7 +216,7 @@ def main():
     traffic = traffic_from_profiles()
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "trace_kernel<false>", "avg_launch_ms": round(launch_s * 1e3, 4),
+                "kernel": "trace_lanes_kernel<false, false>", "avg_launch_ms": round(launch_s * 1e3, 4),
                 "alg_bytes_per_launch": int(bytes_per_launch),
                 "box_tests_per_query": round(counts["box_tests"] / (traced_ext + traced_sh), 2),
                 "triangle_tests_per_query": round(counts["triangle_tests"] / (traced_ext + traced_sh), 2)}
@@ -225,7 +225,7 @@ def main():
     if not args.no_cpu:
         threads = min(16, len(os.sched_getaffinity(0)))
         log(f"CPU baseline on {threads} threads ...")
-        cpu = cpu_baseline(scene, camera, threads, spp=4)
+        cpu = cpu_baseline(scene, camera, threads, spp=40)
         log("cpu:", cpu)
 
     line = {"metric": "Mrays/s at 1080p, 1M tris", "value": round(mrays, 2), "unit": "Mrays/s", "n_gpus": 1,

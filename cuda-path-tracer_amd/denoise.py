@@ -213,7 +213,8 @@ def train(net, feats, targets, steps=2000, batch=16, lr=3e-4, seed=3, chunk=100,
 
 
 # ---------------------------------------------------------------- evaluation
-def time_to_relmse(scene, camera, width, height, net, target=0.01, reference_spp=16384, plain_spp=(64, 256, 1024, 4096),
+def time_to_relmse(scene, camera, width, height, net, target=0.01, reference_spp=16384,
+                   plain_spp=(64, 128, 256, 512, 1024, 2048, 4096),
                    denoised_spp=(1, 4, 16), seed=7, log=print):
     """Wall time for plain progressive accumulation and for render + denoise to
     reach relMSE <= target against a high-spp render of the same view."""
@@ -224,8 +225,8 @@ def time_to_relmse(scene, camera, width, height, net, target=0.01, reference_spp
         step = min(1024, reference_spp - done)
         ref.add_samples(step)
         done += step
+        log(f"reference {done}/{reference_spp} spp: {time.perf_counter() - t0:.1f}s")
     reference = ref.image()
-    log(f"reference {reference_spp} spp: {time.perf_counter() - t0:.1f}s")
     del ref
     plain = []
     r = ptrace.Renderer(scene, camera, width, height, seed, False, ptrace.DEBUG_SHADED, True, 8)
@@ -270,14 +271,28 @@ def main():
     t.add_argument("--steps", type=int, default=3000)
     t.add_argument("--batch", type=int, default=16)
     t.add_argument("--lr", type=float, default=3e-4)
+    pr = sub.add_parser("profile", help="20 training steps of batch 16 on synthetic 256x256 crops (for rocprofv3)")
+    pr.add_argument("--steps", type=int, default=20)
     e = sub.add_parser("evaluate")
     e.add_argument("--weights", default=WEIGHTS_PATH)
-    e.add_argument("--width", type=int, default=960)
-    e.add_argument("--height", type=int, default=540)
+    e.add_argument("--width", type=int, default=1920)
+    e.add_argument("--height", type=int, default=1080)
     e.add_argument("--reference-spp", type=int, default=16384)
     e.add_argument("--out", default=None)
     args = ap.parse_args()
     log = lambda *a: print(*a, flush=True)  # noqa: E731
+    if args.cmd == "profile":
+        rng = np.random.default_rng(0)
+        shape = (32, 256, 256)
+        feats = np.concatenate([rng.gamma(1.0, 0.5, shape + (3,)), rng.uniform(0, 1, shape + (3,)),
+                                rng.normal(0, 1, shape + (3,))], -1).astype(np.float32)
+        targets = rng.gamma(1.0, 0.5, shape + (3,)).astype(np.float32)
+        net = Denoiser(seed=1)
+        net.set_training_data(feats, targets)
+        _, ms = net.train(rng.integers(0, 32, 16 * args.steps), 16, 1e-4)
+        log(f"{args.steps} steps of batch 16 at 256x256: {ms / args.steps:.2f} ms/step")
+        net.forward(feats[:1])
+        return 0
     if args.cmd == "train":
         feats, targets = make_training_set(args.views, args.frame, args.crop, tuple(args.spp_in), args.spp_target, log=log)
         log(f"training set: {len(feats)} crops of {args.crop}x{args.crop}")
