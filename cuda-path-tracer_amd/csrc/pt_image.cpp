@@ -9,9 +9,9 @@
 //                   DEFLATE decoder (RFC 1951); 16-bit samples keep their high
 //                   byte, sub-byte grey is scaled by 255/(2^depth-1), alpha is
 //                   dropped;
-//   binary PNM:     P5 (grey, replicated) and P6, maxval <= 255.
-// JPEG is not decoded here; such a texture fails the load with the
-// reference's own "could not load image" error.
+//   binary PNM:     P5 (grey, replicated) and P6, maxval <= 255;
+//   JPEG:           baseline and progressive Huffman, 8-bit (see below).
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -338,6 +338,618 @@ std::vector<uint8_t> decode_pnm(const std::vector<uint8_t>& file, int& width, in
 
 } // namespace
 
+// ---------------------------------------------------------------- JPEG
+// Baseline and progressive Huffman JPEG (ITU-T T.81), 8-bit, 1/3/4
+// components, decoded to the values stb_image (v2.30, the reference's
+// vendored loader) returns: coefficients dequantized into 16-bit integers,
+// the integer "islow" IDCT of IJG's jidctint.c with stb's rounding (12-bit
+// constants, 2 guard bits after the column pass, +128 and a clamp at the
+// end), stb's centred upsampling filters for 2x horizontal, vertical or
+// both (nearest for other ratios) and its fixed-point YCbCr conversion.
+namespace {
+
+const uint8_t zigzag_to_natural[64 + 15] = {
+    0, 1, 8, 16, 9, 2, 3, 10, 17, 24, 32, 25, 18, 11, 4, 5, 12, 19, 26, 33, 40, 48, 41, 34, 27, 20, 13, 6, 7, 14,
+    21, 28, 35, 42, 49, 56, 57, 50, 43, 36, 29, 22, 15, 23, 30, 37, 44, 51, 58, 59, 52, 45, 38, 31, 39, 46, 53, 60,
+    61, 54, 47, 55, 62, 63,
+    // runs past the end of a corrupt block land on the last coefficient
+    63, 63, 63, 63, 63, 63, 63, 63, 63, 63, 63, 63, 63, 63, 63};
+
+struct jpeg_huffman {
+    uint8_t values[256] = {};
+    int32_t maxcode[18] = {}, valptr[17] = {}, mincode[17] = {};
+    bool defined = false;
+    void build(const int counts[16]) {
+        int code = 0, k = 0;
+        for(int len = 1; len <= 16; len++) {
+            valptr[len] = k;
+            mincode[len] = code;
+            code += counts[len - 1];
+            k += counts[len - 1];
+            if(counts[len - 1] && code - 1 >= (1 << len)) corrupt("bad Huffman code lengths");
+            maxcode[len] = counts[len - 1] ? code - 1 : -1;
+            code <<= 1;
+        }
+        maxcode[17] = 0x7fffffff;
+        defined = true;
+    }
+};
+
+struct jpeg_component {
+    int id = 0, h = 1, v = 1, tq = 0, hd = 0, ha = 0;
+    int x = 0, y = 0, w2 = 0, h2 = 0, dc_pred = 0;
+    std::vector<uint8_t> pixels;   // w2 x h2
+    std::vector<int16_t> coeff;    // progressive: 64 per block, (w2/8) x (h2/8) blocks
+};
+
+class jpeg_decoder {
+public:
+    explicit jpeg_decoder(const std::vector<uint8_t>& f) : file(f) {}
+
+    std::vector<uint8_t> decode(int& width, int& height) {
+        if(file.size() < 2 || file[0] != 0xFF || file[1] != 0xD8) corrupt("not a JPEG file");
+        pos = 2;
+        // Header: table and application segments up to the frame; stray
+        // bytes between segments are skipped.
+        int m = next_marker();
+        while(m != 0xC0 && m != 0xC1 && m != 0xC2) {
+            process_marker(m);
+            m = next_marker();
+            while(m < 0) {
+                if(pos >= file.size()) corrupt("no JPEG frame");
+                m = next_marker();
+            }
+        }
+        progressive = m == 0xC2;
+        frame_header();
+        // Body: scans and tables up to EOI.  As in stb_image, a marker the
+        // decoder does not know (or the end of the file) ends decoding with
+        // what has been decoded so far.
+        m = next_marker();
+        while(m != 0xD9) {
+            if(m == 0xDA) {
+                scan_header();
+                entropy_coded_data();
+                if(marker < 0) marker = marker_after_junk();
+                m = marker >= 0 ? marker : next_marker();
+                marker = -1;
+                if(m >= 0xD0 && m <= 0xD7) m = next_marker();
+            } else if(m == 0xDC) {  // DNL
+                if(be16() != 4 || be16() != height_px) corrupt("bad DNL");
+                m = next_marker();
+            } else {
+                try {
+                    process_marker(m);
+                } catch(const std::runtime_error&) {
+                    break;
+                }
+                m = next_marker();
+            }
+        }
+        if(progressive) finish_progressive();
+        width = width_px; height = height_px;
+        return assemble();
+    }
+
+private:
+    const std::vector<uint8_t>& file;
+    size_t pos = 0;
+    int width_px = 0, height_px = 0, ncomp = 0, hmax = 1, vmax = 1, mcux = 0, mcuy = 0;
+    bool progressive = false, jfif = false;
+    int adobe_transform = -1, rgb_ids = 0;
+    int restart_interval = 0;
+    uint16_t dequant[4][64] = {};
+    jpeg_huffman huff_dc[4], huff_ac[4];
+    jpeg_component comp[4];
+    int scan_n = 0, order[4] = {}, spec_start = 0, spec_end = 63, succ_high = 0, succ_low = 0;
+    // entropy decoder state
+    uint32_t bits = 0;
+    int nbits = 0, marker = -1, eob_run = 0, todo = 0;
+    bool nomore = false;
+
+    uint8_t byte() { if(pos >= file.size()) corrupt("truncated JPEG"); return file[pos++]; }
+    int be16() { int a = byte(); return (a << 8) | byte(); }
+    // Skips entropy-coded junk after a scan: returns the first 0xFF-prefixed
+    // byte that is neither a stuffed zero nor fill, or -1 at the end.
+    int marker_after_junk() {
+        while(pos < file.size()) {
+            int x = file[pos++];
+            while(x == 0xFF) {
+                if(pos >= file.size()) return -1;
+                x = file[pos++];
+                if(x != 0x00 && x != 0xFF) return x;
+            }
+        }
+        return -1;
+    }
+    static bool fits_int(long long v) { return v >= INT32_MIN && v <= INT32_MAX; }
+    static bool fits_short(long long v) { return v >= -32768 && v <= 32767; }
+    int next_marker() {
+        if(pos >= file.size()) return -1;
+        int x = byte();
+        if(x != 0xFF) return -1;
+        while(x == 0xFF) { if(pos >= file.size()) return -1; x = byte(); }
+        return x;
+    }
+
+    void process_marker(int m) {
+        if(m == 0xDD) {
+            if(be16() != 4) corrupt("bad DRI");
+            restart_interval = be16();
+        } else if(m == 0xDB) {
+            int L = be16() - 2;
+            while(L > 0) {
+                int q = byte(), p = q >> 4, t = q & 15;
+                if(p > 1 || t > 3) corrupt("bad DQT");
+                for(int i = 0; i < 64; i++) dequant[t][zigzag_to_natural[i]] = uint16_t(p ? be16() : byte());
+                L -= p ? 129 : 65;
+            }
+            if(L != 0) corrupt("bad DQT length");
+        } else if(m == 0xC4) {
+            int L = be16() - 2;
+            while(L > 0) {
+                int q = byte(), tc = q >> 4, th = q & 15;
+                if(tc > 1 || th > 3) corrupt("bad DHT");
+                int counts[16], n = 0;
+                for(int i = 0; i < 16; i++) { counts[i] = byte(); n += counts[i]; }
+                if(n > 256) corrupt("bad DHT");
+                jpeg_huffman& h = tc ? huff_ac[th] : huff_dc[th];
+                h.build(counts);
+                for(int i = 0; i < n; i++) h.values[i] = byte();
+                L -= 17 + n;
+            }
+            if(L != 0) corrupt("bad DHT length");
+        } else if((m >= 0xE0 && m <= 0xEF) || m == 0xFE) {
+            int L = be16();
+            if(L < 2) corrupt("bad APP length");
+            L -= 2;
+            const size_t start = pos;
+            if(m == 0xE0 && L >= 5 && file.size() - pos >= 5 && std::memcmp(&file[pos], "JFIF\0", 5) == 0) jfif = true;
+            if(m == 0xEE && L >= 12 && file.size() - pos >= 12 && std::memcmp(&file[pos], "Adobe", 6) == 0)
+                adobe_transform = file[pos + 11];
+            pos = start + size_t(L);
+        } else {
+            corrupt("unsupported JPEG marker");
+        }
+    }
+
+    void frame_header() {
+        const int L = be16();
+        if(byte() != 8) corrupt("only 8-bit JPEG");
+        height_px = be16();
+        width_px = be16();
+        if(height_px == 0 || width_px == 0) corrupt("bad JPEG size");
+        ncomp = byte();
+        if(3LL * width_px * height_px + 1 > INT32_MAX || 1LL * width_px * height_px * ncomp > INT32_MAX)
+            corrupt("JPEG too large");
+        if(ncomp != 1 && ncomp != 3 && ncomp != 4) corrupt("bad JPEG component count");
+        if(L != 8 + 3 * ncomp) corrupt("bad SOF length");
+        static const uint8_t rgb[3] = {'R', 'G', 'B'};
+        for(int i = 0; i < ncomp; i++) {
+            comp[i].id = byte();
+            if(ncomp == 3 && comp[i].id == rgb[i]) rgb_ids++;
+            int q = byte();
+            comp[i].h = q >> 4; comp[i].v = q & 15;
+            if(comp[i].h < 1 || comp[i].h > 4 || comp[i].v < 1 || comp[i].v > 4) corrupt("bad sampling factors");
+            comp[i].tq = byte();
+            if(comp[i].tq > 3) corrupt("bad quantization table");
+        }
+        for(int i = 0; i < ncomp; i++) { hmax = std::max(hmax, comp[i].h); vmax = std::max(vmax, comp[i].v); }
+        for(int i = 0; i < ncomp; i++)
+            if(hmax % comp[i].h || vmax % comp[i].v) corrupt("bad sampling factors");
+        mcux = (width_px + hmax * 8 - 1) / (hmax * 8);
+        mcuy = (height_px + vmax * 8 - 1) / (vmax * 8);
+        for(int i = 0; i < ncomp; i++) {
+            jpeg_component& c = comp[i];
+            c.x = (width_px * c.h + hmax - 1) / hmax;
+            c.y = (height_px * c.v + vmax - 1) / vmax;
+            c.w2 = mcux * c.h * 8;
+            c.h2 = mcuy * c.v * 8;
+            c.pixels.assign(size_t(c.w2) * size_t(c.h2), 0);
+            if(progressive) c.coeff.assign(size_t(c.w2 / 8) * size_t(c.h2 / 8) * 64, 0);
+        }
+    }
+
+    void scan_header() {
+        const int L = be16();
+        scan_n = byte();
+        if(scan_n < 1 || scan_n > 4 || scan_n > ncomp) corrupt("bad SOS component count");
+        if(L != 6 + 2 * scan_n) corrupt("bad SOS length");
+        for(int i = 0; i < scan_n; i++) {
+            const int id = byte(), q = byte();
+            int which = -1;
+            for(int k = 0; k < ncomp; k++) if(comp[k].id == id) which = k;
+            if(which < 0) corrupt("bad SOS component");
+            comp[which].hd = q >> 4; comp[which].ha = q & 15;
+            if(comp[which].hd > 3 || comp[which].ha > 3) corrupt("bad SOS table");
+            order[i] = which;
+        }
+        spec_start = byte(); spec_end = byte();
+        const int a = byte();
+        succ_high = a >> 4; succ_low = a & 15;
+        if(progressive) {
+            if(spec_start > 63 || spec_end > 63 || spec_start > spec_end || succ_high > 13 || succ_low > 13)
+                corrupt("bad progressive scan");
+        } else {
+            if(spec_start != 0 || succ_high != 0 || succ_low != 0) corrupt("bad baseline scan");
+            spec_end = 63;
+        }
+    }
+
+    // ---- entropy-coded segment: 0xFF00 is a data byte; any other marker
+    // ends the segment and further bits read as zeros.
+    void fill() {
+        while(nbits <= 24) {
+            int b = 0;
+            if(!nomore) {
+                b = pos < file.size() ? file[pos++] : 0;
+                if(b == 0xFF) {
+                    int c = pos < file.size() ? file[pos++] : 0;
+                    while(c == 0xFF) c = pos < file.size() ? file[pos++] : 0;
+                    if(c != 0) { marker = c; nomore = true; b = 0; }
+                }
+            }
+            bits |= uint32_t(b) << (24 - nbits);
+            nbits += 8;
+        }
+    }
+    int get_bit() {
+        if(nbits < 1) fill();
+        const int b = int(bits >> 31);
+        bits <<= 1; nbits--;
+        return b;
+    }
+    int get_bits(int n) {
+        if(n == 0) return 0;
+        if(nbits < n) fill();
+        const int v = int(bits >> (32 - n));
+        bits <<= n; nbits -= n;
+        return v;
+    }
+    int receive_extend(int n) {
+        const int v = get_bits(n);
+        return v < (1 << (n - 1)) ? v - (1 << n) + 1 : v;
+    }
+    int decode(const jpeg_huffman& h) {
+        if(!h.defined) corrupt("undefined Huffman table");
+        int code = 0;
+        for(int len = 1; len <= 16; len++) {
+            code = (code << 1) | get_bit();
+            if(h.maxcode[len] >= 0 && code <= h.maxcode[len]) {
+                const int k = h.valptr[len] + code - h.mincode[len];
+                if(k < 0 || k > 255) corrupt("bad Huffman code");
+                return h.values[k];
+            }
+        }
+        corrupt("bad Huffman code");
+    }
+    void reset() {
+        bits = 0; nbits = 0; nomore = false; marker = -1; eob_run = 0;
+        for(int i = 0; i < ncomp; i++) comp[i].dc_pred = 0;
+        todo = restart_interval ? restart_interval : 0x7fffffff;
+    }
+    // After each MCU: at the end of a restart interval expect RSTn; false
+    // ends the scan (anything else is left for the marker loop).
+    bool restart_point() {
+        if(--todo > 0) return true;
+        if(nbits < 24) fill();
+        if(!(marker >= 0xD0 && marker <= 0xD7)) return false;
+        reset();
+        return true;
+    }
+
+    void decode_block(int16_t data[64], jpeg_component& c) {
+        std::memset(data, 0, 64 * sizeof(int16_t));
+        const int t = decode(huff_dc[c.hd]);
+        if(t > 15) corrupt("bad DC code");
+        const int diff = t ? receive_extend(t) : 0;
+        if(!fits_int(1LL * c.dc_pred + diff)) corrupt("bad DC delta");
+        c.dc_pred += diff;
+        const uint16_t *dq = dequant[c.tq];
+        if(!fits_short(1LL * c.dc_pred * dq[0])) corrupt("DC coefficient overflow");
+        data[0] = int16_t(c.dc_pred * int(dq[0]));
+        int k = 1;
+        do {
+            const int rs = decode(huff_ac[c.ha]);
+            const int s = rs & 15, r = rs >> 4;
+            if(s == 0) {
+                if(rs != 0xF0) break;
+                k += 16;
+            } else {
+                k += r;
+                const int zig = zigzag_to_natural[k++];
+                data[zig] = int16_t(receive_extend(s) * int(dq[zig]));
+            }
+        } while(k < 64);
+    }
+
+    void decode_dc_progressive(int16_t data[64], jpeg_component& c) {
+        if(spec_end != 0) corrupt("bad progressive DC scan");
+        if(succ_high == 0) {
+            std::memset(data, 0, 64 * sizeof(int16_t));
+            const int t = decode(huff_dc[c.hd]);
+            if(t > 15) corrupt("bad DC code");
+            const int diff = t ? receive_extend(t) : 0;
+            if(!fits_int(1LL * c.dc_pred + diff)) corrupt("bad DC delta");
+            c.dc_pred += diff;
+            if(!fits_short(1LL * c.dc_pred * (1 << succ_low))) corrupt("DC coefficient overflow");
+            data[0] = int16_t(c.dc_pred * (1 << succ_low));
+        } else if(get_bit()) {
+            data[0] = int16_t(data[0] + (1 << succ_low));
+        }
+    }
+
+    void decode_ac_progressive(int16_t data[64], jpeg_component& c) {
+        if(spec_start == 0) corrupt("bad progressive AC scan");
+        if(succ_high == 0) {
+            if(eob_run) { eob_run--; return; }
+            int k = spec_start;
+            do {
+                const int rs = decode(huff_ac[c.ha]);
+                const int s = rs & 15, r = rs >> 4;
+                if(s == 0) {
+                    if(r < 15) {
+                        eob_run = (1 << r) - 1;
+                        if(r) eob_run += get_bits(r);
+                        break;
+                    }
+                    k += 16;
+                } else {
+                    k += r;
+                    const int zig = zigzag_to_natural[k++];
+                    data[zig] = int16_t(receive_extend(s) * (1 << succ_low));
+                }
+            } while(k <= spec_end);
+            return;
+        }
+        const int16_t bit = int16_t(1 << succ_low);
+        auto refine = [&](int16_t *p) {
+            if(get_bit() && (*p & bit) == 0) *p = int16_t(*p > 0 ? *p + bit : *p - bit);
+        };
+        if(eob_run) {
+            eob_run--;
+            for(int k = spec_start; k <= spec_end; k++) {
+                int16_t *p = &data[zigzag_to_natural[k]];
+                if(*p != 0) refine(p);
+            }
+            return;
+        }
+        int k = spec_start;
+        do {
+            const int rs = decode(huff_ac[c.ha]);
+            int s = rs & 15, r = rs >> 4;
+            if(s == 0) {
+                if(r < 15) {
+                    eob_run = (1 << r) - 1;
+                    if(r) eob_run += get_bits(r);
+                    r = 64;  // refine the rest of the band, then end the block
+                }
+            } else {
+                if(s != 1) corrupt("bad refinement code");
+                s = get_bit() ? bit : -bit;
+            }
+            while(k <= spec_end) {
+                int16_t *p = &data[zigzag_to_natural[k++]];
+                if(*p != 0) refine(p);
+                else {
+                    if(r == 0) { *p = int16_t(s); break; }
+                    --r;
+                }
+            }
+        } while(k <= spec_end);
+    }
+
+    void entropy_coded_data() {
+        reset();
+        int16_t block[64];
+        if(scan_n == 1) {
+            jpeg_component& c = comp[order[0]];
+            const int w = (c.x + 7) >> 3, h = (c.y + 7) >> 3;
+            for(int j = 0; j < h; j++)
+                for(int i = 0; i < w; i++) {
+                    if(!progressive) {
+                        decode_block(block, c);
+                        idct(block, &c.pixels[size_t(c.w2) * size_t(j * 8) + size_t(i * 8)], c.w2);
+                    } else {
+                        int16_t *data = &c.coeff[64 * (size_t(i) + size_t(j) * size_t(c.w2 / 8))];
+                        if(spec_start == 0) decode_dc_progressive(data, c);
+                        else decode_ac_progressive(data, c);
+                    }
+                    if(!restart_point()) return;
+                }
+            return;
+        }
+        if(progressive && spec_start != 0) corrupt("interleaved progressive AC scan");
+        for(int j = 0; j < mcuy; j++)
+            for(int i = 0; i < mcux; i++) {
+                for(int k = 0; k < scan_n; k++) {
+                    jpeg_component& c = comp[order[k]];
+                    for(int y = 0; y < c.v; y++)
+                        for(int x = 0; x < c.h; x++) {
+                            const int bx = i * c.h + x, by = j * c.v + y;
+                            if(!progressive) {
+                                decode_block(block, c);
+                                idct(block, &c.pixels[size_t(c.w2) * size_t(by * 8) + size_t(bx * 8)], c.w2);
+                            } else {
+                                decode_dc_progressive(&c.coeff[64 * (size_t(bx) + size_t(by) * size_t(c.w2 / 8))], c);
+                            }
+                        }
+                }
+                if(!restart_point()) return;
+            }
+    }
+
+    void finish_progressive() {
+        for(int n = 0; n < ncomp; n++) {
+            jpeg_component& c = comp[n];
+            const int w = (c.x + 7) >> 3, h = (c.y + 7) >> 3;
+            for(int j = 0; j < h; j++)
+                for(int i = 0; i < w; i++) {
+                    int16_t *data = &c.coeff[64 * (size_t(i) + size_t(j) * size_t(c.w2 / 8))];
+                    for(int k = 0; k < 64; k++) data[k] = int16_t(data[k] * int(dequant[c.tq][k]));
+                    idct(data, &c.pixels[size_t(c.w2) * size_t(j * 8) + size_t(i * 8)], c.w2);
+                }
+        }
+    }
+
+    // Integer IDCT (jidctint "islow" with 12-bit constants; stb's rounding).
+    static int fix(float x) { return int(double(x * 4096) + 0.5); }
+    static uint8_t clamp255(int x) { return uint8_t(x < 0 ? 0 : x > 255 ? 255 : x); }
+    struct idct_terms { int t0, t1, t2, t3, x0, x1, x2, x3; };
+    static idct_terms idct_1d(int s0, int s1, int s2, int s3, int s4, int s5, int s6, int s7) {
+        static const int c0541 = fix(0.5411961f), cm1847 = fix(-1.847759065f), c0765 = fix(0.765366865f);
+        static const int c1175 = fix(1.175875602f), c0298 = fix(0.298631336f), c2053 = fix(2.053119869f);
+        static const int c3072 = fix(3.072711026f), c1501 = fix(1.501321110f), cm0899 = fix(-0.899976223f);
+        static const int cm2562 = fix(-2.562915447f), cm1961 = fix(-1.961570560f), cm0390 = fix(-0.390180644f);
+        idct_terms o;
+        int p2 = s2, p3 = s6;
+        int p1 = (p2 + p3) * c0541;
+        int t2 = p1 + p3 * cm1847, t3 = p1 + p2 * c0765;
+        p2 = s0; p3 = s4;
+        int t0 = (p2 + p3) * 4096, t1 = (p2 - p3) * 4096;
+        o.x0 = t0 + t3; o.x3 = t0 - t3; o.x1 = t1 + t2; o.x2 = t1 - t2;
+        t0 = s7; t1 = s5; t2 = s3; t3 = s1;
+        p3 = t0 + t2;
+        int p4 = t1 + t3;
+        p1 = t0 + t3;
+        p2 = t1 + t2;
+        const int p5 = (p3 + p4) * c1175;
+        t0 = t0 * c0298; t1 = t1 * c2053; t2 = t2 * c3072; t3 = t3 * c1501;
+        p1 = p5 + p1 * cm0899; p2 = p5 + p2 * cm2562;
+        p3 = p3 * cm1961; p4 = p4 * cm0390;
+        t3 += p1 + p4; t2 += p2 + p3; t1 += p2 + p4; t0 += p1 + p3;
+        o.t0 = t0; o.t1 = t1; o.t2 = t2; o.t3 = t3;
+        return o;
+    }
+    static void idct(const int16_t *d, uint8_t *out, int stride) {
+        int v[64];
+        for(int i = 0; i < 8; i++) {
+            const int16_t *c = d + i;
+            if(!c[8] && !c[16] && !c[24] && !c[32] && !c[40] && !c[48] && !c[56]) {
+                const int dc = c[0] * 4;
+                for(int r = 0; r < 8; r++) v[i + 8 * r] = dc;
+                continue;
+            }
+            idct_terms o = idct_1d(c[0], c[8], c[16], c[24], c[32], c[40], c[48], c[56]);
+            o.x0 += 512; o.x1 += 512; o.x2 += 512; o.x3 += 512;
+            v[i] = (o.x0 + o.t3) >> 10; v[i + 56] = (o.x0 - o.t3) >> 10;
+            v[i + 8] = (o.x1 + o.t2) >> 10; v[i + 48] = (o.x1 - o.t2) >> 10;
+            v[i + 16] = (o.x2 + o.t1) >> 10; v[i + 40] = (o.x2 - o.t1) >> 10;
+            v[i + 24] = (o.x3 + o.t0) >> 10; v[i + 32] = (o.x3 - o.t0) >> 10;
+        }
+        for(int r = 0; r < 8; r++) {
+            const int *w = v + 8 * r;
+            idct_terms o = idct_1d(w[0], w[1], w[2], w[3], w[4], w[5], w[6], w[7]);
+            const int bias = 65536 + (128 << 17);
+            o.x0 += bias; o.x1 += bias; o.x2 += bias; o.x3 += bias;
+            uint8_t *p = out + size_t(r) * size_t(stride);
+            p[0] = clamp255((o.x0 + o.t3) >> 17); p[7] = clamp255((o.x0 - o.t3) >> 17);
+            p[1] = clamp255((o.x1 + o.t2) >> 17); p[6] = clamp255((o.x1 - o.t2) >> 17);
+            p[2] = clamp255((o.x2 + o.t1) >> 17); p[5] = clamp255((o.x2 - o.t1) >> 17);
+            p[3] = clamp255((o.x3 + o.t0) >> 17); p[4] = clamp255((o.x3 - o.t0) >> 17);
+        }
+    }
+
+    // ---- upsampling (centred triangle filters for factors of 2) and colour
+    static void upsample_row(const uint8_t *near_row, const uint8_t *far_row, int w, int hs, int vs, uint8_t *out) {
+        if(hs == 1 && vs == 1) { std::memcpy(out, near_row, size_t(w)); return; }
+        if(hs == 1 && vs == 2) {
+            for(int i = 0; i < w; i++) out[i] = uint8_t((3 * near_row[i] + far_row[i] + 2) >> 2);
+            return;
+        }
+        if(hs == 2 && vs == 1) {
+            if(w == 1) { out[0] = out[1] = near_row[0]; return; }
+            out[0] = near_row[0];
+            out[1] = uint8_t((near_row[0] * 3 + near_row[1] + 2) >> 2);
+            int i = 1;
+            for(; i < w - 1; i++) {
+                const int n = 3 * near_row[i] + 2;
+                out[2 * i] = uint8_t((n + near_row[i - 1]) >> 2);
+                out[2 * i + 1] = uint8_t((n + near_row[i + 1]) >> 2);
+            }
+            out[2 * i] = uint8_t((near_row[w - 2] * 3 + near_row[w - 1] + 2) >> 2);
+            out[2 * i + 1] = near_row[w - 1];
+            return;
+        }
+        if(hs == 2 && vs == 2) {
+            if(w == 1) { out[0] = out[1] = uint8_t((3 * near_row[0] + far_row[0] + 2) >> 2); return; }
+            int t1 = 3 * near_row[0] + far_row[0];
+            out[0] = uint8_t((t1 + 2) >> 2);
+            for(int i = 1; i < w; i++) {
+                const int t0 = t1;
+                t1 = 3 * near_row[i] + far_row[i];
+                out[2 * i - 1] = uint8_t((3 * t0 + t1 + 8) >> 4);
+                out[2 * i] = uint8_t((3 * t1 + t0 + 8) >> 4);
+            }
+            out[2 * w - 1] = uint8_t((t1 + 2) >> 2);
+            return;
+        }
+        for(int i = 0; i < w; i++)
+            for(int j = 0; j < hs; j++) out[i * hs + j] = near_row[i];
+    }
+
+    static uint8_t blinn(int x, int y) { const unsigned t = unsigned(x * y + 128); return uint8_t((t + (t >> 8)) >> 8); }
+
+    std::vector<uint8_t> assemble() {
+        const bool is_rgb = ncomp == 3 && (rgb_ids == 3 || (adobe_transform == 0 && !jfif));
+        const int decode_n = ncomp;
+        struct row_state { int hs, vs, ystep, w_lores, ypos; size_t line0, line1; std::vector<uint8_t> buf; };
+        std::vector<row_state> rs(static_cast<size_t>(decode_n));
+        for(int k = 0; k < decode_n; k++) {
+            row_state& r = rs[size_t(k)];
+            r.hs = hmax / comp[k].h; r.vs = vmax / comp[k].v;
+            r.ystep = r.vs >> 1;
+            r.w_lores = (width_px + r.hs - 1) / r.hs;
+            r.ypos = 0; r.line0 = r.line1 = 0;
+            r.buf.assign(size_t(width_px) + 3, 0);
+        }
+        auto fixed = [](float x) { return int(x * 4096.0f + 0.5f) << 8; };
+        const int cr_r = fixed(1.40200f), cr_g = -fixed(0.71414f), cb_g = -fixed(0.34414f), cb_b = fixed(1.77200f);
+        std::vector<uint8_t> rgb(size_t(width_px) * size_t(height_px) * 3);
+        std::vector<const uint8_t *> row(static_cast<size_t>(decode_n));
+        for(int j = 0; j < height_px; j++) {
+            for(int k = 0; k < decode_n; k++) {
+                row_state& r = rs[size_t(k)];
+                const bool bottom = r.ystep >= (r.vs >> 1);
+                const uint8_t *a = &comp[k].pixels[bottom ? r.line1 : r.line0];
+                const uint8_t *b = &comp[k].pixels[bottom ? r.line0 : r.line1];
+                if(r.hs == 1 && r.vs == 1) row[size_t(k)] = a;
+                else { upsample_row(a, b, r.w_lores, r.hs, r.vs, r.buf.data()); row[size_t(k)] = r.buf.data(); }
+                if(++r.ystep >= r.vs) {
+                    r.ystep = 0;
+                    r.line0 = r.line1;
+                    if(++r.ypos < comp[k].y) r.line1 += size_t(comp[k].w2);
+                }
+            }
+            uint8_t *out = &rgb[size_t(j) * size_t(width_px) * 3];
+            const uint8_t *y = row[0];
+            for(int i = 0; i < width_px; i++, out += 3) {
+                if(ncomp == 1) { out[0] = out[1] = out[2] = y[i]; continue; }
+                if(ncomp == 3 && is_rgb) { out[0] = y[i]; out[1] = row[1][i]; out[2] = row[2][i]; continue; }
+                if(ncomp == 4 && adobe_transform == 0) {  // CMYK
+                    const uint8_t m = row[3][i];
+                    out[0] = blinn(y[i], m); out[1] = blinn(row[1][i], m); out[2] = blinn(row[2][i], m);
+                    continue;
+                }
+                const int y_fixed = (y[i] << 20) + (1 << 19);
+                const int cr = row[2][i] - 128, cb = row[1][i] - 128;
+                int r = y_fixed + cr * cr_r;
+                int g = y_fixed + (cr * cr_g) + int(unsigned(cb * cb_g) & 0xffff0000u);
+                int b = y_fixed + cb * cb_b;
+                r >>= 20; g >>= 20; b >>= 20;
+                out[0] = clamp255(r); out[1] = clamp255(g); out[2] = clamp255(b);
+                if(ncomp == 4 && adobe_transform == 2) {  // YCCK
+                    const uint8_t m = row[3][i];
+                    out[0] = blinn(255 - out[0], m); out[1] = blinn(255 - out[1], m); out[2] = blinn(255 - out[2], m);
+                }
+            }
+        }
+        return rgb;
+    }
+};
+
+} // namespace
+
 namespace pts {
 
 // Decode an image file to 8-bit RGB; throws "could not load image: <path>".
@@ -351,6 +963,8 @@ std::vector<uint8_t> load_image(const std::string& path, int& width, int& height
             return decode_png(file, width, height);
         if(file.size() >= 2 && file[0] == 'P' && (file[1] == '5' || file[1] == '6'))
             return decode_pnm(file, width, height);
+        if(file.size() >= 2 && file[0] == 0xFF && file[1] == 0xD8)
+            return jpeg_decoder(file).decode(width, height);
         throw std::runtime_error("unsupported image format");
     } catch(const std::runtime_error&) {
         throw std::runtime_error("could not load image: " + path);

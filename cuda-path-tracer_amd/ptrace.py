@@ -531,7 +531,7 @@ def load_obj(path, aspect, scale=(1, 1, 1), rotate=(0, 0, 0), translate=(0, 0, 0
 
 
 def load_image(path):
-    """Decode a PNG or binary PNM texture to an (height, width, 3) uint8 array."""
+    """Decode a PNG, binary PNM or JPEG texture to an (height, width, 3) uint8 array."""
     L = scene_lib()
     w, h, buf = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_void_p()
     if L.pts_load_image(os.fsencode(path), ctypes.byref(w), ctypes.byref(h), ctypes.byref(buf)) != 0:

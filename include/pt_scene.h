@@ -69,8 +69,9 @@ pts_scene *pts_load_obj(const char *filename, const pts_obj_options *options, pt
 /* Loader warnings of a scene from pts_load_obj ("" otherwise). */
 const char *pts_warnings(const pts_scene *scene);
 
-/* Decode a PNG or binary PNM file to 8-bit RGB rows, top row first, as the
- * reference's stbi_load(..., STBI_rgb) returns them; free with pts_free_image.
+/* Decode a PNG, binary PNM or JPEG (baseline or progressive) file to 8-bit
+ * RGB rows, top row first, as the reference's stbi_load(..., STBI_rgb)
+ * returns them (src/textures/texture.h:65-72); free with pts_free_image.
  * Returns 0, or -1 with pt_scene_last_error() set. */
 int32_t pts_load_image(const char *path, int32_t *width, int32_t *height, uint8_t **rgb);
 void pts_free_image(uint8_t *rgb);

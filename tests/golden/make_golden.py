@@ -12,6 +12,7 @@ project's builder; both are stored as flat arrays, so the fixtures do not
 depend on the reference tree at test time.
 """
 import ctypes
+import io
 import os
 import sys
 
@@ -167,10 +168,222 @@ def encode_png(samples, color, depth, interlace=False, level=6, strategy=None, p
     return out
 
 
+def encode_jpeg(width, height, planes, factors, quality=75, ids=None, restart=0, scans=None, jfif=True, adobe=None, junk=False,
+                dqt16=False):
+    """Minimal baseline JPEG writer (ITU-T T.81) for decoder fixtures.
+
+    planes[k] is component k at its own resolution, ceil(height*v/vmax) rows
+    of ceil(width*h/hmax) samples, factors[k] its (h, v) sampling factors.  Each scan in
+    `scans` lists the components it codes (one component: non-interleaved
+    blocks; several: interleaved MCUs).  Huffman tables are built per file
+    from the symbols used (lengths 3..10, most frequent shortest).  Options add
+    a restart interval, an Adobe APP14 segment with the given transform, junk
+    bytes and 0xFF fill before EOI, and 16-bit quantization tables."""
+    import struct
+    n = len(planes)
+    hmax = max(h for h, _ in factors)
+    vmax = max(v for _, v in factors)
+    W, H = width, height
+    for k in range(n):
+        assert planes[k].shape == (-(-H * factors[k][1] // vmax), -(-W * factors[k][0] // hmax))
+    mcux, mcuy = -(-W // (8 * hmax)), -(-H // (8 * vmax))
+    scans = scans or [list(range(n))]
+    ids = ids or [k + 1 for k in range(n)]
+    u = np.arange(8)
+    dct = np.sqrt(2 / 8) * np.cos((2 * u[None, :] + 1) * u[:, None] * np.pi / 16)
+    dct[0] /= np.sqrt(2)
+    scale = 5000 / quality if quality < 50 else 200 - 2 * quality
+    base = 16 + 2 * (u[:, None] + u[None, :])
+    quant = [np.clip((base * scale + 50) // 100, 1, 65535 if dqt16 else 255).astype(int),
+             np.clip(((base + 8) * scale + 50) // 100, 1, 65535 if dqt16 else 255).astype(int)]
+    zz = sorted(((y, x) for y in range(8) for x in range(8)), key=lambda p: (p[0] + p[1], p[0] if (p[0] + p[1]) % 2 else p[1]))
+    coeff = []
+    for k, img in enumerate(planes):
+        h, v = factors[k]
+        w2, h2 = mcux * h * 8, mcuy * v * 8
+        pad = np.pad(img.astype(float), ((0, h2 - img.shape[0]), (0, w2 - img.shape[1])), mode="edge") - 128
+        q = quant[min(k, 1)]
+        blocks = pad.reshape(h2 // 8, 8, w2 // 8, 8).transpose(0, 2, 1, 3)
+        f = np.einsum("ux,abxy,vy->abuv", dct, blocks, dct)
+        coeff.append(np.rint(f / q).astype(int))
+
+    def category(x):
+        return int(abs(x)).bit_length()
+
+    # symbol stream: (kind, table, symbol, bits, nbits) with kind 'h'; ('rst', n) markers
+    def scan_symbols(comps):
+        out, pred, count = [], {k: 0 for k in comps}, 0
+        if len(comps) == 1:
+            k = comps[0]
+            cy = -(-planes[k].shape[0] // 8)
+            cx = -(-planes[k].shape[1] // 8)
+            units = [[(k, bx, by)] for by in range(cy) for bx in range(cx)]
+        else:
+            units = [[(k, i * factors[k][0] + x, j * factors[k][1] + y) for k in comps for y in range(factors[k][1])
+                      for x in range(factors[k][0])] for j in range(mcuy) for i in range(mcux)]
+        for ui, unit in enumerate(units):
+            if restart and ui and ui % restart == 0:
+                out.append(("rst", (ui // restart - 1) % 8))
+                pred = {k: 0 for k in comps}
+            for k, bx, by in unit:
+                c = coeff[k][by, bx]
+                t = min(k, 1)
+                diff = int(c[0, 0]) - pred[k]
+                pred[k] = int(c[0, 0])
+                s = category(diff)
+                out.append(("dc", t, s, diff if diff >= 0 else diff + (1 << s) - 1, s))
+                run = 0
+                seq = [int(c[y, x]) for y, x in zz[1:]]
+                last = max([i for i, x in enumerate(seq) if x] + [-1])
+                for i, x in enumerate(seq[:last + 1]):
+                    if x == 0:
+                        run += 1
+                        continue
+                    while run > 15:
+                        out.append(("ac", t, 0xF0, 0, 0))
+                        run -= 16
+                    s = category(x)
+                    out.append(("ac", t, (run << 4) | s, x if x >= 0 else x + (1 << s) - 1, s))
+                    run = 0
+                if last < 62:
+                    out.append(("ac", t, 0x00, 0, 0))
+        return out
+
+    streams = [scan_symbols(c) for c in scans]
+    tables = {}
+    for kind in ("dc", "ac"):
+        for t in (0, 1):
+            freq = {}
+            for st in streams:
+                for e in st:
+                    if e[0] == kind and e[1] == t:
+                        freq[e[2]] = freq.get(e[2], 0) + 1
+            if not freq:
+                continue
+            syms = sorted(freq, key=lambda x: (-freq[x], x))
+            counts, lengths, i = [0] * 16, {}, 0
+            for length in range(3, 11):
+                room = (1 << (length - 3)) - (1 if length == 10 else 0)
+                take = syms[i:i + room]
+                counts[length - 1] = len(take)
+                for x in take:
+                    lengths[x] = length
+                i += len(take)
+            assert i == len(syms)
+            ordered = [x for length in range(1, 17) for x in syms if lengths[x] == length]
+            code, codes = 0, {}
+            for length in range(1, 17):
+                for x in ordered:
+                    if lengths[x] == length:
+                        codes[x] = (code, length)
+                        code += 1
+                code <<= 1
+            tables[(kind, t)] = (counts, ordered, codes)
+
+    def seg(marker, body):
+        return bytes([0xFF, marker]) + struct.pack(">H", len(body) + 2) + body
+
+    out = b"\xff\xd8"
+    if jfif:
+        out += seg(0xE0, b"JFIF\x00\x01\x01\x00\x00\x01\x00\x01\x00\x00")
+    if adobe is not None:
+        out += seg(0xEE, b"Adobe\x00" + bytes([100, 0, 0, 0, 0, adobe]))
+    out += seg(0xFE, b"decoder fixture")
+    for t in range(min(n, 2)):
+        q = quant[t][tuple(np.array(zz).T)]
+        if dqt16:
+            out += seg(0xDB, bytes([0x10 | t]) + b"".join(struct.pack(">H", int(x)) for x in q))
+        else:
+            out += seg(0xDB, bytes([t]) + bytes(int(x) for x in q))
+    out += seg(0xC0, struct.pack(">BHHB", 8, H, W, n) + b"".join(
+        bytes([ids[k], (factors[k][0] << 4) | factors[k][1], min(k, 1)]) for k in range(n)))
+    for (kind, t), (counts, ordered, _) in sorted(tables.items()):
+        out += seg(0xC4, bytes([(0 if kind == "dc" else 0x10) | t]) + bytes(counts) + bytes(ordered))
+    if restart:
+        out += seg(0xDD, struct.pack(">H", restart))
+    for comps, st in zip(scans, streams):
+        out += seg(0xDA, bytes([len(comps)]) + b"".join(bytes([ids[k], (min(k, 1) << 4) | min(k, 1)]) for k in comps)
+                   + bytes([0, 63, 0]))
+        bits, data = "", bytearray()
+
+        def flush(bits):
+            bits += "1" * (-len(bits) % 8)
+            for i in range(0, len(bits), 8):
+                b = int(bits[i:i + 8], 2)
+                data.append(b)
+                if b == 0xFF:
+                    data.append(0)
+        for e in st:
+            if e[0] == "rst":
+                flush(bits)
+                bits = ""
+                data.extend(bytes([0xFF, 0xD0 + e[1]]))
+                continue
+            code, length = tables[(e[0], e[1])][2][e[2]]
+            bits += format(code, f"0{length}b") + (format(e[3], f"0{e[4]}b") if e[4] else "")
+        flush(bits)
+        out += bytes(data)
+        if junk:
+            out += b"\x12\x34\xff\x00\x56"
+    if junk:
+        out += b"\xff\xff\xff"
+    return out + b"\xff\xd9"
+
+
+def jpeg_planes(rng, width, height, factors):
+    """Smooth gradients plus noise for each component at its own resolution."""
+    hmax = max(h for h, _ in factors)
+    vmax = max(v for _, v in factors)
+    out = []
+    for k, (h, v) in enumerate(factors):
+        rows, cols = -(-height * v // vmax), -(-width * h // hmax)
+        yy, xx = np.mgrid[0:rows, 0:cols]
+        out.append(((xx * (5 + 3 * k) + yy * (7 - k) + rng.integers(0, 40, (rows, cols))) % 256).astype(np.uint8))
+    return out
+
+
+# Written baseline files: every resampling filter of stb_image (1:1, 2:1
+# horizontal, vertical and both, nearest for other ratios), non-interleaved
+# and partly interleaved scans, restart intervals, Adobe RGB / CMYK / YCCK,
+# 16-bit quantization tables, junk between scans, 1-pixel images.
+JPEG_WRITTEN = {
+    "h1v2": (37, 29, [(1, 2), (1, 1), (1, 1)], {}),
+    "h4v1": (45, 21, [(4, 1), (1, 1), (1, 1)], {}),
+    "h3v2": (50, 35, [(3, 2), (1, 1), (1, 1)], dict(restart=2)),
+    "mixed_split": (41, 39, [(2, 4), (1, 2), (2, 1)], dict(scans=[[0], [1], [2]], restart=3)),
+    "partial_interleave": (33, 17, [(2, 2), (1, 1), (1, 1)], dict(scans=[[0], [1, 2]], junk=True, dqt16=True)),
+    "rgb_adobe": (19, 23, [(1, 1)] * 3, dict(ids=[ord("R"), ord("G"), ord("B")], jfif=False, adobe=0)),
+    "rgb_transform0": (19, 23, [(1, 1)] * 3, dict(jfif=False, adobe=0)),
+    "ycck": (26, 18, [(2, 2), (1, 1), (1, 1), (2, 2)], dict(jfif=False, adobe=2)),
+    "cmyk": (26, 18, [(1, 1)] * 4, dict(jfif=False, adobe=0, restart=1)),
+    "ycc_alpha": (12, 12, [(1, 1)] * 4, dict(jfif=False, adobe=1)),
+    "gray_restart": (31, 9, [(1, 1)], dict(restart=1, quality=95)),
+    "tiny_h2v2": (1, 1, [(2, 2), (1, 1), (1, 1)], {}),
+    "tiny_h2v1": (1, 3, [(2, 1), (1, 1), (1, 1)], {}),
+    "q10": (40, 40, [(2, 2), (1, 1), (1, 1)], dict(quality=10)),
+}
+# Files from libjpeg through Pillow: progressive (spectral selection and
+# successive approximation) and baseline, with and without restart markers.
+JPEG_PILLOW = {
+    "pil_prog_444": ("RGB", 45, 31, dict(progressive=True, subsampling=0, quality=90)),
+    "pil_prog_422_rst": ("RGB", 50, 27, dict(progressive=True, subsampling=1, quality=70, restart_marker_blocks=2)),
+    "pil_prog_420": ("RGB", 67, 66, dict(progressive=True, subsampling=2, quality=50)),
+    "pil_prog_gray": ("L", 29, 40, dict(progressive=True, quality=80)),
+    "pil_prog_cmyk": ("CMYK", 21, 19, dict(progressive=True, quality=85)),
+    "pil_base_420_rst": ("RGB", 35, 43, dict(subsampling=2, quality=60, restart_marker_blocks=1)),
+    "pil_base_422": ("RGB", 9, 57, dict(subsampling=1, quality=98)),
+    "pil_prog_q100": ("RGB", 16, 16, dict(progressive=True, subsampling=2, quality=100)),
+}
+
+
 def import_fixtures(L):
     """OBJ import, reference-order BVH and texture decoding, from the
     reference's loader, bvh_node, flat exporter and stb_image."""
     import zlib
+    # The ngon fixture's JPEG texture (4:2:0, restart markers).
+    tile = jpeg_planes(np.random.default_rng(11), 24, 20, [(2, 2), (1, 1), (1, 1)])
+    with open(os.path.join(ASSETS, "ngon", "maps", "tile.jpg"), "wb") as fh:
+        fh.write(encode_jpeg(24, 20, tile, [(2, 2), (1, 1), (1, 1)], quality=80, restart=2))
     # OBJ assets with transforms, seeds and aspect ratios on both sides of 1.
     cases = [("gallery", "gallery/gallery.obj", dict(seed=7)),
              ("gallery_t", "gallery/gallery.obj", dict(seed=41, aspect=0.75, scale=(1.5, 0.5, 2), rotate=(15, 30, -45),
@@ -278,8 +491,50 @@ def import_fixtures(L):
         if L.ref_load_image(path.encode(), ctypes.byref(wi), ctypes.byref(hi), buf.ctypes.data, buf.size) != 0:
             raise RuntimeError(f"stb_image rejected {name}")
         expected[name] = buf[:wi.value * hi.value * 3].reshape(hi.value, wi.value, 3)
+    # JPEG: written baseline files and libjpeg-made progressive ones.
+    from PIL import Image
+    jrng = np.random.default_rng(17)
+    jpeg_files = {}
+    for name, (w, h, factors, kw) in JPEG_WRITTEN.items():
+        jpeg_files[name] = encode_jpeg(w, h, jpeg_planes(jrng, w, h, factors), factors, **kw)
+    for name, (mode, w, h, kw) in JPEG_PILLOW.items():
+        ch = {"RGB": 3, "L": 1, "CMYK": 4}[mode]
+        yy, xx = np.mgrid[0:h, 0:w]
+        px = ((jrng.integers(0, 60, (h, w, ch)) + (xx * 4 + yy * 3)[..., None] * np.arange(1, ch + 1)) % 256)
+        img = Image.fromarray(px[..., 0].astype(np.uint8) if ch == 1 else px.astype(np.uint8), mode)
+        buf = io.BytesIO()
+        img.save(buf, "JPEG", **kw)
+        jpeg_files[name] = buf.getvalue()
+    for name, data in jpeg_files.items():
+        path = os.path.join(img_dir, f"jpeg_{name}.jpg")
+        with open(path, "wb") as fh:
+            fh.write(data)
+        wi, hi = ctypes.c_int(), ctypes.c_int()
+        buf = np.zeros(1 << 20, np.uint8)
+        if L.ref_load_image(path.encode(), ctypes.byref(wi), ctypes.byref(hi), buf.ctypes.data, buf.size) != 0:
+            raise RuntimeError(f"stb_image rejected jpeg_{name}")
+        expected[f"jpeg_{name}"] = buf[:wi.value * hi.value * 3].reshape(hi.value, wi.value, 3)
     np.savez_compressed(os.path.join(HERE, "images.npz"), **expected)
-    print("wrote images.npz and", len(cases) + 1, "images")
+    print("wrote images.npz,", len(cases) + 1, "PNG/PNM and", len(jpeg_files), "JPEG images")
+    # The reference's own JPEG textures (vodka model, 2048x2048): too large to
+    # keep as fixtures, so they are checked here, against stb_image, when the
+    # reference tree is present.
+    vodka = os.path.join(REF_ROOT, "assets", "models", "vodka", "textures")
+    if os.path.isdir(vodka):
+        for fname in sorted(os.listdir(vodka)):
+            if not fname.lower().endswith((".jpg", ".jpeg")):
+                continue
+            path = os.path.join(vodka, fname)
+            wi, hi = ctypes.c_int(), ctypes.c_int()
+            buf = np.zeros(1 << 24, np.uint8)
+            if L.ref_load_image(path.encode(), ctypes.byref(wi), ctypes.byref(hi), buf.ctypes.data, buf.size) != 0:
+                raise RuntimeError(f"stb_image rejected {fname}")
+            want = buf[:wi.value * hi.value * 3].reshape(hi.value, wi.value, 3)
+            got = ptrace.load_image(path)
+            same = got.shape == want.shape and np.array_equal(got, want)
+            print(f"{fname}: {wi.value}x{hi.value}", "identical to stb_image" if same else "DIFFERS from stb_image")
+            if not same:
+                raise RuntimeError(f"JPEG decode differs from stb_image for {fname}")
 
 
 def main():

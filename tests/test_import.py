@@ -132,13 +132,34 @@ def _image_path(name):
     special = {"rgb_ppm": os.path.join(GOLDEN, "images", "rgb.ppm"),
                "checker": os.path.join(ASSETS, "blender_sphere", "checker.png"),
                "four_colors": os.path.join(ASSETS, "textured_triangle", "materials", "four colors.png")}
-    return special.get(name, os.path.join(GOLDEN, "images", name + ".png"))
+    ext = ".jpg" if name.startswith("jpeg_") else ".png"
+    return special.get(name, os.path.join(GOLDEN, "images", name + ext))
 
 
 @pytest.mark.parametrize("name", sorted(IMAGES))
 def test_image_decoding_matches_stb(name):
     img = ptrace.load_image(_image_path(name))
     assert img.shape == IMAGES[name].shape and np.array_equal(img, IMAGES[name])
+
+
+def test_jpeg_without_eoi_and_unsupported_frames(tmp_path):
+    """stb_image semantics around the frame: a file that ends after its last
+    scan (no EOI) decodes in full; 12-bit samples, arithmetic coding and
+    lossless frames are rejected."""
+    data = open(_image_path("jpeg_pil_prog_420"), "rb").read()
+    assert data.endswith(b"\xff\xd9")
+    cut = tmp_path / "cut.jpg"
+    cut.write_bytes(data[:-2])
+    assert np.array_equal(ptrace.load_image(str(cut)), IMAGES["jpeg_pil_prog_420"])
+    base = open(_image_path("jpeg_h1v2"), "rb").read()
+    sof = base.index(b"\xff\xc0")
+    for patch in (bytes([0xFF, 0xC9]) + base[sof + 2:sof + 4] + bytes([8]),     # arithmetic
+                  bytes([0xFF, 0xC3]) + base[sof + 2:sof + 4] + bytes([8]),     # lossless
+                  bytes([0xFF, 0xC0]) + base[sof + 2:sof + 4] + bytes([12])):   # 12-bit
+        bad = tmp_path / "bad.jpg"
+        bad.write_bytes(base[:sof] + patch + base[sof + 5:])
+        with pytest.raises(RuntimeError, match="could not load image"):
+            ptrace.load_image(str(bad))
 
 
 def _write(tmp_path, files):
