@@ -80,51 +80,92 @@ __device__ inline bf16x8 gather8(const conv_src& s, int n, int y, int x, int ch)
     return load8(s.b + idx * size_t(s.cpb) + size_t(ch - s.cpa));
 }
 
+// Master channel of padded input channel pi (channels of a, then of b), or -1
+// for padding.
+__device__ inline int real_channel(int pi, int cin_a, int cpa, int cin_b) {
+    if(pi < cpa) return pi < cin_a ? pi : -1;
+    return pi - cpa < cin_b ? cin_a + (pi - cpa) : -1;
+}
+
 enum epilogue { EPI_BF16 = 0, EPI_BF16_RELU = 1, EPI_F32 = 2 };
 
-// One workgroup: 4 waves x 64 pixels; each wave MT tiles of 16 output
-// channels x 4 tiles of 16 pixels.  wk: [9][kchunks][cpo][32] bf16.
+// One workgroup computes a 16 x 16 pixel tile of one image for MT
+// 16-channel output tiles; wk is [9][kchunks][cpo][32] bf16.  The tile's 18 x 18 input halo is
+// staged through LDS one 32-channel chunk at a time (double-buffered: the
+// next chunk's global loads are in flight while the current one is
+// multiplied), laid out [channel group of 8][halo pixel] so that the 16
+// pixels of an MFMA B fragment read 256 contiguous bytes.  Wave w owns tile
+// rows 4w..4w+3 (one 16-pixel N tile per row).
+constexpr int TILE = 16, HALO = TILE + 2, HALO_PIX = HALO * HALO;
+constexpr int STAGE_ITEMS = HALO_PIX * 4;                       // 16-byte items per chunk
+constexpr int STAGE_PER_THREAD = (STAGE_ITEMS + 255) / 256;
+
 template<int MT, int EPI>
-__global__ __launch_bounds__(256) void conv_kernel(conv_src src, const uint16_t *__restrict__ wk, int kchunks, int cpo,
-                                                   const float *__restrict__ bias, int cout, void *__restrict__ out) {
-    const int lane = int(threadIdx.x) & 63, wave = int(threadIdx.x) >> 6;
-    const int64_t total = int64_t(src.n) * src.h * src.w;
-    const int64_t base = (int64_t(blockIdx.x) * 4 + wave) * 64;
-    const int co0 = int(blockIdx.y) * MT * 16;
+__global__ __launch_bounds__(256) void conv_tile_kernel(conv_src src, const uint16_t *__restrict__ wk, int kchunks,
+                                                        int cpo, const float *__restrict__ bias, int cout,
+                                                        void *__restrict__ out, int tiles_x, int tiles_y) {
+    __shared__ u16x8 halo[2][4][HALO_PIX];
+    const int tid = int(threadIdx.x), lane = tid & 63, wave = tid >> 6;
     const int kq = lane >> 4, col = lane & 15;
-    int pn[4], py[4], px[4];
-    bool pv[4];
-    for(int nt = 0; nt < 4; nt++) {
-        int64_t p = base + nt * 16 + col;
-        pv[nt] = p < total;
-        if(!pv[nt]) p = 0;
-        const int64_t hw = int64_t(src.h) * src.w;
-        pn[nt] = int(p / hw);
-        const int64_t r = p - int64_t(pn[nt]) * hw;
-        py[nt] = int(r / src.w);
-        px[nt] = int(r - int64_t(py[nt]) * src.w);
-    }
+    int tile = int(blockIdx.x);
+    const int tx = tile % tiles_x;
+    tile /= tiles_x;
+    const int ty = tile % tiles_y;
+    const int img = tile / tiles_y;
+    const int x0 = tx * TILE, y0 = ty * TILE;
+    const int co0 = int(blockIdx.y) * MT * 16;
+
+    u16x8 staged[STAGE_PER_THREAD];
+    auto fetch = [&](int kc) {
+        for(int i = 0; i < STAGE_PER_THREAD; i++) {
+            const int item = tid + 256 * i;
+            if(item < STAGE_ITEMS) {
+                const int hp = item >> 2, q = item & 3;
+                const int hy = hp / HALO, hx = hp - hy * HALO;
+                staged[i] = __builtin_bit_cast(u16x8, gather8(src, img, y0 - 1 + hy, x0 - 1 + hx, kc * 32 + 8 * q));
+            }
+        }
+    };
+    auto store = [&](int buf) {
+        for(int i = 0; i < STAGE_PER_THREAD; i++) {
+            const int item = tid + 256 * i;
+            if(item < STAGE_ITEMS) halo[buf][item & 3][item >> 2] = staged[i];
+        }
+    };
+
     f32x4 acc[MT][4];
     for(int mt = 0; mt < MT; mt++)
         for(int nt = 0; nt < 4; nt++) acc[mt][nt] = f32x4{0, 0, 0, 0};
-    for(int t = 0; t < 9; t++) {
-        const int dy = t / 3 - 1, dx = t % 3 - 1;
-        for(int kc = 0; kc < kchunks; kc++) {
+
+    fetch(0);
+    store(0);
+    __syncthreads();
+    for(int kc = 0; kc < kchunks; kc++) {
+        const int buf = kc & 1;
+        if(kc + 1 < kchunks) fetch(kc + 1);
+        const uint16_t *wchunk = wk + (size_t(kc) * size_t(cpo) + size_t(co0 + col)) * 32 + size_t(8 * kq);
+#pragma unroll
+        for(int t = 0; t < 9; t++) {
+            const int dy = t / 3, dx = t % 3;
+            bf16x8 afrag[MT];
+            const uint16_t *wrow = wchunk + size_t(t) * size_t(kchunks) * size_t(cpo) * 32;
+            for(int mt = 0; mt < MT; mt++) afrag[mt] = load8(wrow + size_t(mt) * 16 * 32);
             bf16x8 bfrag[4];
             for(int nt = 0; nt < 4; nt++)
-                bfrag[nt] = pv[nt] ? gather8(src, pn[nt], py[nt] + dy, px[nt] + dx, kc * 32 + 8 * kq) : zero8();
-            const uint16_t *wrow = wk + ((size_t(t) * size_t(kchunks) + size_t(kc)) * size_t(cpo) + size_t(co0 + col)) * 32 +
-                                   size_t(8 * kq);
-            for(int mt = 0; mt < MT; mt++) {
-                const bf16x8 afrag = load8(wrow + size_t(mt) * 16 * 32);
+                bfrag[nt] = __builtin_bit_cast(bf16x8, halo[buf][kq][(4 * wave + nt + dy) * HALO + col + dx]);
+            for(int mt = 0; mt < MT; mt++)
                 for(int nt = 0; nt < 4; nt++)
-                    acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afrag, bfrag[nt], acc[mt][nt], 0, 0, 0);
-            }
+                    acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afrag[mt], bfrag[nt], acc[mt][nt], 0, 0, 0);
         }
+        if(kc + 1 < kchunks) store(buf ^ 1);
+        __syncthreads();
     }
+
+    const int x = x0 + col;
     for(int nt = 0; nt < 4; nt++) {
-        if(!pv[nt]) continue;
-        const size_t p = size_t(base + nt * 16 + col);
+        const int y = y0 + 4 * wave + nt;
+        if(x >= src.w || y >= src.h) continue;
+        const size_t p = (size_t(img) * size_t(src.h) + size_t(y)) * size_t(src.w) + size_t(x);
         for(int mt = 0; mt < MT; mt++) {
             const int co = co0 + mt * 16 + 4 * kq;
             float v[4];
@@ -145,62 +186,68 @@ __global__ __launch_bounds__(256) void conv_kernel(conv_src src, const uint16_t 
     }
 }
 
-// dW partials: for tap t (blockIdx.y % 9), input chunk kc (blockIdx.y / 9)
-// and pixel split blockIdx.x: partial[s][t][kc*32 + ci][co] =
-// sum over the split's pixels p of X(p + offset(t))[ci] * dY[p][co].
-// NCO = cpo / 16 output-channel tiles; 2 x NCO tiles over 4 waves.
+// dW partials: for input chunk kc (blockIdx.y) and pixel split blockIdx.x,
+// partial[s][t][kc*32 + ci][co] = sum over the split's pixels p of
+// X(p + offset(t))[ci] * dY[p][co], all nine taps t at once so that each
+// 64-pixel sub-block of dY is staged once.  Both operands are transposed
+// through LDS so that K (pixels) is contiguous per lane.  NCO = cpo / 16
+// output-channel tiles; 9 x 2 x NCO MFMA tiles spread over the 4 waves.
 constexpr int WG_PIX = 64;
 constexpr int LDS_ROW = WG_PIX + 8;  // bf16 per LDS row (16-byte aligned rows, fewer bank conflicts)
 
 template<int NCO>
 __global__ __launch_bounds__(256) void wgrad_kernel(conv_src src, const uint16_t *__restrict__ dy, int cpo,
                                                     int64_t blocks_per_split, int cpi, float *__restrict__ partial) {
-    __shared__ __attribute__((aligned(16))) uint16_t xt[32 * LDS_ROW];
+    __shared__ __attribute__((aligned(16))) uint16_t xt[9 * 32 * LDS_ROW];
     __shared__ __attribute__((aligned(16))) uint16_t dt[NCO * 16 * LDS_ROW];
     const int tid = int(threadIdx.x), lane = tid & 63, wave = tid >> 6;
     const int kq = lane >> 4, col = lane & 15;
-    const int t = int(blockIdx.y) % 9, kc = int(blockIdx.y) / 9;
-    const int dyo = t / 3 - 1, dxo = t % 3 - 1;
+    const int kc = int(blockIdx.y);
     const int64_t total = int64_t(src.n) * src.h * src.w;
     const int64_t hw = int64_t(src.h) * src.w;
     const int64_t nsub = (total + WG_PIX - 1) / WG_PIX;
     const int64_t sb0 = int64_t(blockIdx.x) * blocks_per_split;
     const int64_t sb1 = std::min(nsub, sb0 + blocks_per_split);
-    constexpr int TILES = 2 * NCO;
+    constexpr int TILES = 9 * 2 * NCO;
     constexpr int TPW = (TILES + 3) / 4;
     f32x4 acc[TPW];
     for(int i = 0; i < TPW; i++) acc[i] = f32x4{0, 0, 0, 0};
     const int groups = cpo / 8;
+    const int pxl = tid >> 2, g = tid & 3;  // this thread's pixel and 8-channel group of the X tile
     for(int64_t sb = sb0; sb < sb1; sb++) {
         const int64_t p0 = sb * WG_PIX;
-        {   // X tile: one 8-channel group of one pixel per thread, transposed into xt[ci][px].
-            const int pxl = tid >> 2, g = tid & 3;
+        {   // X at the nine tap offsets, transposed into xt[t][ci][px].
             const int64_t p = p0 + pxl;
-            bf16x8 v = zero8();
+            int n = 0, y = 0, x = 0;
             if(p < total) {
-                const int n = int(p / hw);
+                n = int(p / hw);
                 const int64_t r = p - int64_t(n) * hw;
-                const int y = int(r / src.w), x = int(r - int64_t(y) * src.w);
-                v = gather8(src, n, y + dyo, x + dxo, kc * 32 + 8 * g);
+                y = int(r / src.w);
+                x = int(r - int64_t(y) * src.w);
             }
-            const u16x8 u = __builtin_bit_cast(u16x8, v);
-            for(int j = 0; j < 8; j++) xt[(8 * g + j) * LDS_ROW + pxl] = u[j];
+            for(int t = 0; t < 9; t++) {
+                bf16x8 v = zero8();
+                if(p < total) v = gather8(src, n, y + t / 3 - 1, x + t % 3 - 1, kc * 32 + 8 * g);
+                const u16x8 u = __builtin_bit_cast(u16x8, v);
+                for(int j = 0; j < 8; j++) xt[(t * 32 + 8 * g + j) * LDS_ROW + pxl] = u[j];
+            }
         }
         for(int i = tid; i < WG_PIX * groups; i += 256) {  // dY tile, transposed into dt[co][px]
-            const int pxl = i / groups, g = i - pxl * groups;
-            const int64_t p = p0 + pxl;
+            const int q = i / groups, gg = i - q * groups;
+            const int64_t p = p0 + q;
             u16x8 u = u16x8{0, 0, 0, 0, 0, 0, 0, 0};
-            if(p < total) u = *reinterpret_cast<const u16x8 *>(dy + size_t(p) * size_t(cpo) + size_t(8 * g));
-            for(int j = 0; j < 8; j++) dt[(8 * g + j) * LDS_ROW + pxl] = u[j];
+            if(p < total) u = *reinterpret_cast<const u16x8 *>(dy + size_t(p) * size_t(cpo) + size_t(8 * gg));
+            for(int j = 0; j < 8; j++) dt[(8 * gg + j) * LDS_ROW + q] = u[j];
         }
         __syncthreads();
         for(int i = 0; i < TPW; i++) {
             const int tile = wave + 4 * i;
             if(tile >= TILES) break;
-            const int ti = tile & 1, tj = tile >> 1;
+            const int t = tile / (2 * NCO), r = tile - t * (2 * NCO);
+            const int ti = r & 1, tj = r >> 1;
             for(int ks = 0; ks < WG_PIX / 32; ks++) {
                 const bf16x8 a = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(
-                    &xt[(ti * 16 + col) * LDS_ROW + ks * 32 + 8 * kq]));
+                    &xt[(t * 32 + ti * 16 + col) * LDS_ROW + ks * 32 + 8 * kq]));
                 const bf16x8 b = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(
                     &dt[(tj * 16 + col) * LDS_ROW + ks * 32 + 8 * kq]));
                 acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[i], 0, 0, 0);
@@ -211,7 +258,8 @@ __global__ __launch_bounds__(256) void wgrad_kernel(conv_src src, const uint16_t
     for(int i = 0; i < TPW; i++) {
         const int tile = wave + 4 * i;
         if(tile >= TILES) break;
-        const int ti = tile & 1, tj = tile >> 1;
+        const int t = tile / (2 * NCO), r = tile - t * (2 * NCO);
+        const int ti = r & 1, tj = r >> 1;
         for(int v = 0; v < 4; v++) {
             const int ci = kc * 32 + ti * 16 + 4 * kq + v, co = tj * 16 + col;
             partial[((size_t(blockIdx.x) * 9 + size_t(t)) * size_t(cpi) + size_t(ci)) * size_t(cpo) + size_t(co)] = acc[i][v];
@@ -220,43 +268,67 @@ __global__ __launch_bounds__(256) void wgrad_kernel(conv_src src, const uint16_t
 }
 
 // Sum the dW partials into the FP32 gradient in the master layout [co][t][ci].
-__global__ void wgrad_reduce_kernel(const float *partial, int splits, int cpi, int cpo, int cin_a, int cpa, int cin_b,
-                                    int cout, float *grad) {
-    const int cin = cin_a + cin_b;
-    const int64_t count = int64_t(cout) * 9 * cin;
-    for(int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += int64_t(gridDim.x) * blockDim.x) {
-        const int ci = int(i % cin), t = int((i / cin) % 9), co = int(i / (9 * cin));
-        const int pi = ci < cin_a ? ci : cpa + (ci - cin_a);
-        float s = 0;
-        for(int k = 0; k < splits; k++) s += partial[((size_t(k) * 9 + size_t(t)) * size_t(cpi) + size_t(pi)) * size_t(cpo) + size_t(co)];
-        grad[i] = s;
-    }
+// Sum the dW partials over the pixel splits into the FP32 gradient in the
+// master layout [co][t][ci].  Threads run along the partial layout
+// [t][pi][co] (coalesced reads); threadIdx.y takes every fourth split and the
+// four sums are combined in a fixed order.
+__global__ __launch_bounds__(256) void wgrad_reduce_kernel(const float *partial, int splits, int cpi, int cpo, int cin_a,
+                                                           int cpa, int cin_b, int cout, float *grad) {
+    __shared__ float part[4][64];
+    const int64_t per_split = int64_t(9) * cpi * cpo;
+    const int64_t j = int64_t(blockIdx.x) * 64 + threadIdx.x;
+    const int sg = int(threadIdx.y);
+    float s = 0;
+    if(j < per_split)
+        for(int k = sg; k < splits; k += 4) s += partial[size_t(k) * size_t(per_split) + size_t(j)];
+    part[sg][threadIdx.x] = s;
+    __syncthreads();
+    if(sg != 0 || j >= per_split) return;
+    const int co = int(j % cpo), pi = int((j / cpo) % cpi), t = int(j / (int64_t(cpo) * cpi));
+    const int ci = real_channel(pi, cin_a, cpa, cin_b);
+    if(co >= cout || ci < 0) return;
+    const float total = (part[0][threadIdx.x] + part[1][threadIdx.x]) + (part[2][threadIdx.x] + part[3][threadIdx.x]);
+    grad[(size_t(co) * 9 + size_t(t)) * size_t(cin_a + cin_b) + size_t(ci)] = total;
 }
 
-// Bias gradient partials: partial[block][co] = sum over the block's pixels of dY[p][co].
+// Bias gradient partials: partial[block][co] = sum over the block's pixels of
+// dY[p][co].  Each thread reads 16-byte groups of 8 channels; consecutive
+// threads cover consecutive groups of consecutive pixels.
 __global__ __launch_bounds__(256) void bgrad_kernel(const uint16_t *dy, int64_t total, int cpo, int64_t per_block, float *partial) {
-    __shared__ float red[256];
+    __shared__ float red[256][9];
     const int tid = int(threadIdx.x);
-    const int rows = 256 / cpo, co = tid % cpo, r = tid / cpo;
-    float s = 0;
+    const int groups = cpo / 8, rows = 256 / groups;
+    const int g = tid % groups, r = tid / groups;
+    float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     if(r < rows) {
         const int64_t p0 = int64_t(blockIdx.x) * per_block, p1 = std::min(total, p0 + per_block);
-        for(int64_t p = p0 + r; p < p1; p += rows) s += from_bf16(dy[size_t(p) * size_t(cpo) + size_t(co)]);
+        for(int64_t p = p0 + r; p < p1; p += rows) {
+            const u16x8 v = *reinterpret_cast<const u16x8 *>(dy + size_t(p) * size_t(cpo) + size_t(8 * g));
+            for(int j = 0; j < 8; j++) s[j] += from_bf16(v[j]);
+        }
     }
-    red[tid] = s;
+    for(int j = 0; j < 8; j++) red[tid][j] = s[j];
     __syncthreads();
     if(tid < cpo) {
+        const int gg = tid / 8, j = tid % 8;
         float t = 0;
-        for(int k = 0; k < rows; k++) t += red[k * cpo + tid];
+        for(int k = 0; k < rows; k++) t += red[k * groups + gg][j];
         partial[size_t(blockIdx.x) * size_t(cpo) + size_t(tid)] = t;
     }
 }
-__global__ void bgrad_reduce_kernel(const float *partial, int blocks, int cpo, int cout, float *grad) {
-    const int co = int(threadIdx.x);
-    if(co >= cout) return;
+// One block per output channel: a fixed-order tree sum of the block partials.
+__global__ __launch_bounds__(256) void bgrad_reduce_kernel(const float *partial, int blocks, int cpo, int cout, float *grad) {
+    __shared__ float red[256];
+    const int co = int(blockIdx.x), tid = int(threadIdx.x);
     float s = 0;
-    for(int k = 0; k < blocks; k++) s += partial[size_t(k) * size_t(cpo) + size_t(co)];
-    grad[co] = s;
+    for(int k = tid; k < blocks; k += 256) s += partial[size_t(k) * size_t(cpo) + size_t(co)];
+    red[tid] = s;
+    __syncthreads();
+    for(int w = 128; w > 0; w >>= 1) {
+        if(tid < w) red[tid] += red[tid + w];
+        __syncthreads();
+    }
+    if(tid == 0 && co < cout) grad[co] = red[0];
 }
 
 // ---------------------------------------------------------------- elementwise stages
@@ -437,11 +509,7 @@ __global__ void adam_kernel(float *w, const float *g, float *m, float *v, int64_
     }
 }
 
-// Packed bf16 weights from the master layout W[co][t][ci] (ci over a's then b's real channels).
-__device__ inline int real_channel(int pi, int cin_a, int cpa, int cin_b) {
-    if(pi < cpa) return pi < cin_a ? pi : -1;
-    return pi - cpa < cin_b ? cin_a + (pi - cpa) : -1;
-}
+// Packed bf16 weights from the master layout W[co][t][ci].
 // Forward: wk[t][kc][co][k] = W[co][t][ci(kc*32 + k)].
 __global__ void pack_forward_kernel(const float *w, int cin_a, int cpa, int cin_b, int cpb, int cout, int cpo, uint16_t *wk) {
     const int cpi = cpa + cpb, cin = cin_a + cin_b;
@@ -662,11 +730,11 @@ struct ptd_network {
     void wgrad_plan(int l, int64_t& splits, int64_t& per_split) const {
         const int cpi = cpa[l] + cpb[l];
         const int64_t nsub = (level_pixels(SPECS[l].level) + WG_PIX - 1) / WG_PIX;
-        const int64_t want = std::max<int64_t>(1, 2048 / (9 * (cpi / 32)));
+        const int64_t want = std::max<int64_t>(1, 768 / (cpi / 32));
         per_split = (nsub + std::min(nsub, want) - 1) / std::min(nsub, want);
         splits = (nsub + per_split - 1) / per_split;
     }
-    static constexpr int64_t BIAS_PIXELS = 4096;  // pixels per block of the bias-gradient reduction
+    static constexpr int64_t BIAS_PIXELS = 2048;  // pixels per block of the bias-gradient reduction
 
     conv_src source(int l) const {
         const layer_spec& s = SPECS[l];
@@ -682,17 +750,19 @@ struct ptd_network {
     void launch_conv(const conv_src& src, const uint16_t *weights, int kchunks, int co_total, const float *bias, int cout,
                      void *dst) {
         const int tiles = co_total / 16;
-        const int64_t pixels = int64_t(src.n) * src.h * src.w;
-        const unsigned gx = unsigned((pixels + 255) / 256);
+        const int tiles_x = (src.w + TILE - 1) / TILE, tiles_y = (src.h + TILE - 1) / TILE;
+        const int64_t blocks = int64_t(src.n) * tiles_x * tiles_y;
+        if(blocks > INT32_MAX) throw std::invalid_argument("denoiser input too large");
+        const unsigned gx = unsigned(blocks);
         if(tiles % 6 == 0)
-            hipLaunchKernelGGL((conv_kernel<6, EPI>), dim3(gx, unsigned(tiles / 6)), dim3(256), 0, stream, src, weights,
-                               kchunks, co_total, bias, cout, dst);
+            hipLaunchKernelGGL((conv_tile_kernel<6, EPI>), dim3(gx, unsigned(tiles / 6)), dim3(256), 0, stream, src,
+                               weights, kchunks, co_total, bias, cout, dst, tiles_x, tiles_y);
         else if(tiles % 4 == 0)
-            hipLaunchKernelGGL((conv_kernel<4, EPI>), dim3(gx, unsigned(tiles / 4)), dim3(256), 0, stream, src, weights,
-                               kchunks, co_total, bias, cout, dst);
+            hipLaunchKernelGGL((conv_tile_kernel<4, EPI>), dim3(gx, unsigned(tiles / 4)), dim3(256), 0, stream, src,
+                               weights, kchunks, co_total, bias, cout, dst, tiles_x, tiles_y);
         else
-            hipLaunchKernelGGL((conv_kernel<2, EPI>), dim3(gx, unsigned(tiles / 2)), dim3(256), 0, stream, src, weights,
-                               kchunks, co_total, bias, cout, dst);
+            hipLaunchKernelGGL((conv_tile_kernel<2, EPI>), dim3(gx, unsigned(tiles / 2)), dim3(256), 0, stream, src,
+                               weights, kchunks, co_total, bias, cout, dst, tiles_x, tiles_y);
         HC(hipGetLastError());
     }
 
@@ -731,7 +801,7 @@ struct ptd_network {
         const int64_t pixels = level_pixels(s.level);
         int64_t used, bps;
         wgrad_plan(l, used, bps);
-        const dim3 grid(unsigned(used), unsigned(9 * (cpi / 32)));
+        const dim3 grid(unsigned(used), unsigned(cpi / 32));
         switch(cpo[l] / 16) {
             case 2: hipLaunchKernelGGL(wgrad_kernel<2>, grid, dim3(256), 0, stream, src, dy.p, cpo[l], bps, cpi, wpartial.p); break;
             case 4: hipLaunchKernelGGL(wgrad_kernel<4>, grid, dim3(256), 0, stream, src, dy.p, cpo[l], bps, cpi, wpartial.p); break;
@@ -739,13 +809,14 @@ struct ptd_network {
             default: throw std::logic_error("unsupported output width");
         }
         const int64_t nw = int64_t(s.cout) * 9 * (s.cin_a + s.cin_b);
-        hipLaunchKernelGGL(wgrad_reduce_kernel, dim3(grid_for(nw)), dim3(256), 0, stream, wpartial.p, int(used), cpi,
-                           cpo[l], s.cin_a, cpa[l], s.cin_b, s.cout, grads.p + offsets[l]);
+        const int64_t per_split = int64_t(9) * cpi * cpo[l];
+        hipLaunchKernelGGL(wgrad_reduce_kernel, dim3(unsigned((per_split + 63) / 64)), dim3(64, 4), 0, stream, wpartial.p,
+                           int(used), cpi, cpo[l], s.cin_a, cpa[l], s.cin_b, s.cout, grads.p + offsets[l]);
         const int64_t bblocks = (pixels + BIAS_PIXELS - 1) / BIAS_PIXELS;
         hipLaunchKernelGGL(bgrad_kernel, dim3(unsigned(bblocks)), dim3(256), 0, stream, dy.p, pixels, cpo[l], BIAS_PIXELS,
                            bpartial.p);
-        hipLaunchKernelGGL(bgrad_reduce_kernel, dim3(1), dim3(128), 0, stream, bpartial.p, int(bblocks), cpo[l], s.cout,
-                           grads.p + offsets[l] + nw);
+        hipLaunchKernelGGL(bgrad_reduce_kernel, dim3(unsigned(s.cout)), dim3(256), 0, stream, bpartial.p, int(bblocks),
+                           cpo[l], s.cout, grads.p + offsets[l] + nw);
         HC(hipGetLastError());
     }
 

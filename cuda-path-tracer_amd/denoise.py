@@ -214,7 +214,7 @@ def train(net, feats, targets, steps=2000, batch=16, lr=3e-4, seed=3, chunk=100,
 
 # ---------------------------------------------------------------- evaluation
 def time_to_relmse(scene, camera, width, height, net, target=0.01, reference_spp=16384,
-                   plain_spp=(64, 128, 256, 512, 1024, 2048, 4096),
+                   plain_spp=(4, 8, 16, 24, 32, 40, 48, 56, 64, 96, 128, 256, 512, 1024, 2048, 4096),
                    denoised_spp=(1, 4, 16), seed=7, log=print):
     """Wall time for plain progressive accumulation and for render + denoise to
     reach relMSE <= target against a high-spp render of the same view."""
@@ -236,8 +236,19 @@ def time_to_relmse(scene, camera, width, height, net, target=0.01, reference_spp
         r.add_samples(spp - have)
         spent += time.perf_counter() - t0
         have = spp
-        plain.append({"spp": spp, "seconds": spent, "relmse": relmse(r.image(), reference)})
+        # The image read back to the host is part of the result, as it is for
+        # the denoised form (denoise_renderer returns host memory).
+        t0 = time.perf_counter()
+        image = r.image()
+        readback = time.perf_counter() - t0
+        plain.append({"spp": spp, "seconds": spent + readback, "relmse": relmse(image, reference)})
         log("plain", plain[-1])
+    # Size the network's buffers for this frame once, untimed (set-up, like
+    # the renderers' own allocations).
+    with ptrace.Renderer(scene, camera, width, height, seed + 1, False, ptrace.DEBUG_SHADED, True, 8) as warm:
+        warm.enable_aovs()
+        warm.add_samples(1)
+        net.denoise_renderer(warm)
     den = []
     for spp in sorted(denoised_spp):
         r = ptrace.Renderer(scene, camera, width, height, seed, False, ptrace.DEBUG_SHADED, True, 8)
@@ -273,6 +284,12 @@ def main():
     t.add_argument("--lr", type=float, default=3e-4)
     pr = sub.add_parser("profile", help="20 training steps of batch 16 on synthetic 256x256 crops (for rocprofv3)")
     pr.add_argument("--steps", type=int, default=20)
+    inf = sub.add_parser("infer", help="render the benchmark frame at low spp and denoise it (for rocprofv3)")
+    inf.add_argument("--weights", default=WEIGHTS_PATH)
+    inf.add_argument("--width", type=int, default=1920)
+    inf.add_argument("--height", type=int, default=1080)
+    inf.add_argument("--spp", type=int, default=1)
+    inf.add_argument("--repeat", type=int, default=10)
     e = sub.add_parser("evaluate")
     e.add_argument("--weights", default=WEIGHTS_PATH)
     e.add_argument("--width", type=int, default=1920)
@@ -292,6 +309,23 @@ def main():
         _, ms = net.train(rng.integers(0, 32, 16 * args.steps), 16, 1e-4)
         log(f"{args.steps} steps of batch 16 at 256x256: {ms / args.steps:.2f} ms/step")
         net.forward(feats[:1])
+        return 0
+    if args.cmd == "infer":
+        net = Denoiser(weights=args.weights)
+        scene, cam = ptrace.icosphere(224, 7, args.width, args.height)
+        with ptrace.Renderer(scene, cam, args.width, args.height, 7, False, ptrace.DEBUG_SHADED, True, 8) as r:
+            r.enable_aovs()
+            t0 = time.perf_counter()
+            r.add_samples(args.spp)
+            log(f"render {args.spp} spp at {args.width}x{args.height}: {1e3 * (time.perf_counter() - t0):.2f} ms")
+            device, wall = [], []
+            for _ in range(args.repeat):
+                t0 = time.perf_counter()
+                _, ms = net.denoise_renderer(r)
+                wall.append(1e3 * (time.perf_counter() - t0))
+                device.append(ms)
+            log(f"denoise: device {np.median(device):.3f} ms, wall with readback {np.median(wall):.3f} ms "
+                f"(median of {args.repeat})")
         return 0
     if args.cmd == "train":
         feats, targets = make_training_set(args.views, args.frame, args.crop, tuple(args.spp_in), args.spp_target, log=log)

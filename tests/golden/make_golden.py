@@ -537,12 +537,47 @@ def import_fixtures(L):
                 raise RuntimeError(f"JPEG decode differs from stb_image for {fname}")
 
 
+def converged_fixture(L, spp=16384, threads=None):
+    """The small-scene benchmark room (320x200, seed 7, MIS, the reference's
+    path limit of 50) rendered by the reference host path at 16384 samples per
+    pixel: the target that multi-bounce renders must converge to.  Stored as
+    float16 (relative rounding 2^-11, far below the 1e-3 relMSE bar)."""
+    import time
+    w, h, seed = 320, 200, 99991
+    scene, cam = ptrace.cornell(7, w, h)
+    s, keep = oracle.scene_struct(scene)
+    c = np.ascontiguousarray(np.asarray(cam).reshape(()))
+    threads = threads or len(os.sched_getaffinity(0))
+    total = np.zeros((h, w, 3), np.float64)
+    rays_all = 0
+    band = 25
+    for y0 in range(0, h, band):
+        out = np.zeros((band * w, 3), np.float32)
+        rays = ctypes.c_uint64()
+        t = time.perf_counter()
+        rc = L.ref_render_rows(ctypes.byref(s), c.ctypes.data, w, h, seed, 1, spp, y0, y0 + band, threads,
+                               out.ctypes.data, ctypes.byref(rays))
+        if rc != 0:
+            raise RuntimeError("reference render failed")
+        total[y0:y0 + band] = out.reshape(band, w, 3) / spp
+        rays_all += rays.value
+        print(f"rows {y0}..{y0 + band}: {time.perf_counter() - t:.1f} s", flush=True)
+    # Pixel (x, y) at image[y, x], y = 0 the bottom row, as the renderer's image().
+    path = os.path.join(HERE, "converged_cornell.npz")
+    np.savez_compressed(path, image=total.astype(np.float16), spp=np.int64(spp), seed=np.int64(seed),
+                        width=np.int64(w), height=np.int64(h), rays=np.uint64(rays_all))
+    print("wrote", path, os.path.getsize(path), "bytes")
+
+
 def main():
     if not oracle.ref_available():
         raise SystemExit("oracle/_ref/libptref.so is missing: run oracle/build_ref.sh first")
     L = oracle.ref()
     if sys.argv[1:] == ["import"]:
         import_fixtures(L)
+        return
+    if sys.argv[1:] == ["converged"]:
+        converged_fixture(L)
         return
 
     # PCG streams: the reference's pixel_rng for several (seed, pixel) pairs,
