@@ -593,6 +593,8 @@ struct partition {
     dev_buf<int32_t> queue0, queue1, shadow_queue, finish_queue;
     dev_buf<int2> spill;
     dev_buf<uint32_t> log;
+    dev_buf<int4> save_head[2], save_best[2];   // queries handed on between trace launches
+    dev_buf<int2> save_stack[2];
     pt::paths_dev paths{};
     std::vector<hipEvent_t> events;      // 5 per iteration of a check interval
     hipEvent_t checked = nullptr, drain0 = nullptr, drain1 = nullptr, finished = nullptr;
@@ -605,6 +607,18 @@ struct partition {
         if(ctl_host) (void)hipHostFree(ctl_host);
         if(stream) (void)hipStreamDestroy(stream);
     }
+    // Buffers for the queries a trace launch hands on to the next: up to
+    // `capacity` per launch, `stride` stack entries each.
+    void ensure_saves(int64_t capacity, int stride) {
+        if(paths.save_capacity >= capacity && paths.save_stride == stride) return;
+        for(int k = 0; k < 2; k++) {
+            save_head[k].alloc(size_t(capacity)); save_best[k].alloc(size_t(capacity));
+            save_stack[k].alloc(size_t(capacity) * size_t(std::max(1, stride)));
+            paths.saved[k] = pt::saved_queries{save_head[k].get(), save_best[k].get(), save_stack[k].get()};
+        }
+        paths.save_capacity = int32_t(capacity);
+        paths.save_stride = stride;
+    }
     void read_ctl_async() { read_ctl_async(stream); }
     void read_ctl_async(hipStream_t s) {
         HIP_CHECK(hipMemcpyAsync(ctl_host, ctl.get(), sizeof(pt::loop_ctl), hipMemcpyDeviceToHost, s));
@@ -613,6 +627,7 @@ struct partition {
 };
 
 constexpr int partition_chunk = 4096;  // pixels per chunk dealt round-robin to partitions
+constexpr int64_t max_saved_queries = 262144;  // queries one trace launch can hand on to the next
 constexpr uint32_t log_capacity = 1u << 16;
 
 struct pt_renderer {
@@ -637,7 +652,7 @@ struct pt_renderer {
     dev_buf<int4> hit;
     std::vector<std::unique_ptr<partition>> parts;
     dev_buf<float> aov;                  // first-hit albedo and normal sums (6 per pixel), when enabled
-    pt_tuning tuning{128, 256, 16, 8, 0, 0, 0, 262144, 48, 1};
+    pt_tuning tuning{128, 256, 16, 8, 0, 0, 0, 262144, 48, 1, 16};
     int cus = 1;
     pt_launch_stats last{};
 
@@ -760,6 +775,8 @@ struct pt_renderer {
         HIP_CHECK(hipEventRecord(start, stream));
         for(auto& part : parts) {
             if(part->spill.size() < spill_need) { part->spill.alloc(spill_need); part->paths.spill = part->spill.get(); }
+            if(use_lanes) part->ensure_saves(std::min<int64_t>(max_saved_queries, 2 * std::max<int64_t>(1, part->count)),
+                                             scene->view.wide_stack_capacity);
             if(tuning.time_kernels && part->events.size() < size_t(5 * interval)) {
                 for(auto e : part->events) (void)hipEventDestroy(e);
                 part->events.assign(size_t(5 * interval), nullptr);
@@ -782,7 +799,7 @@ struct pt_renderer {
                 if(tuning.time_kernels) HIP_CHECK(hipEventRecord(part.events[size_t(5 * k)], part.stream));
                 if(use_lanes)
                     pt::launch_trace_lanes(scene->view, frame, part.paths, c, par, trace_grid, trace_threads,
-                                           tuning.lane_refill, tuning.count_tests != 0, part.stream);
+                                           tuning.lane_refill, tuning.handoff_steps, tuning.count_tests != 0, part.stream);
                 else
                     pt::launch_trace(scene->view, frame, part.paths, c, par, trace_grid, trace_threads,
                                      tuning.count_tests != 0, part.stream);
@@ -820,7 +837,8 @@ struct pt_renderer {
                 if(live == 0 || h.failure) { part->done = true; continue; }
                 if(drain_threshold && int64_t(live) <= drain_threshold) {
                     if(tuning.time_kernels) HIP_CHECK(hipEventRecord(part->drain0, part->stream));
-                    pt::launch_drain(scene->view, frame, part->paths, part->ctl.get(), part->parity, h.n_shadow, live,
+                    pt::launch_drain(scene->view, frame, part->paths, part->ctl.get(), part->parity, h.n_shadow,
+                                     h.n_saved[part->parity], live,
                                      drain_block, part->stream);
                     HIP_CHECK(hipGetLastError());
                     if(tuning.time_kernels) HIP_CHECK(hipEventRecord(part->drain1, part->stream));
@@ -1021,13 +1039,13 @@ struct pt_batch {
     partition part;
     hipStream_t stream = nullptr;
     hipEvent_t start = nullptr, stop = nullptr;
-    pt_tuning tuning{128, 256, 16, 8, 0, 0, 0, 0, 48, 1};
+    pt_tuning tuning{128, 256, 16, 8, 0, 0, 0, 0, 48, 1, 16};
     int cus = 1;
     int32_t samples = 0;
     uint64_t rays = 0;
     float milliseconds = 0;
     hipGraphExec_t graph = nullptr;
-    int graph_iterations = 0, graph_trace_block = 0, graph_blocks = 0, graph_shade_block = 0, graph_refill = 0;
+    int graph_iterations = 0, graph_trace_block = 0, graph_blocks = 0, graph_shade_block = 0, graph_refill = 0, graph_tail = 0;
     pt_launch_stats last{};
 
     ~pt_batch() {
@@ -1058,7 +1076,8 @@ struct pt_batch {
     // `iterations` times (even, so each replay ends on parity 0).
     void capture(int iterations, int block, int blocks, hipStream_t s) {
         if(graph && graph_iterations == iterations && graph_trace_block == block && graph_blocks == blocks &&
-           graph_shade_block == tuning.shade_block && graph_refill == tuning.lane_refill)
+           graph_shade_block == tuning.shade_block && graph_refill == tuning.lane_refill &&
+           graph_tail == tuning.handoff_steps)
             return;
         if(graph) { HIP_CHECK(hipGraphExecDestroy(graph)); graph = nullptr; }
         const int shade_grid = int(std::max<size_t>(1, std::min<size_t>(
@@ -1068,7 +1087,7 @@ struct pt_batch {
         for(int k = 0; k < iterations; k++) {
             const int par = k & 1;
             pt::launch_trace_lanes(scene->view, frame, part.paths, part.ctl.get(), par, blocks, block, tuning.lane_refill,
-                                   false, s, &bd);
+                                   tuning.handoff_steps, false, s, &bd);
             pt::launch_bookkeep(part.ctl.get(), par, s);
             pt::launch_shade(scene->view, frame, part.paths, part.ctl.get(), par, shade_grid, tuning.shade_block, s, &bd);
             pt::launch_regen(frame, part.paths, part.ctl.get(), par, shade_grid, tuning.shade_block, s, &bd);
@@ -1078,7 +1097,7 @@ struct pt_batch {
         (void)hipGraphDestroy(g);
         HIP_CHECK(e);
         graph_iterations = iterations; graph_trace_block = block; graph_blocks = blocks;
-        graph_shade_block = tuning.shade_block; graph_refill = tuning.lane_refill;
+        graph_shade_block = tuning.shade_block; graph_refill = tuning.lane_refill; graph_tail = tuning.handoff_steps;
     }
 
     void add_samples(int32_t count) {
@@ -1093,6 +1112,8 @@ struct pt_batch {
         const size_t spill_need = std::max(size_t(blocks) * size_t(block), drain_threads) *
                                   size_t(pt::wide_spill_entries(scene->view.wide_stack_capacity));
         if(part.spill.size() < spill_need) { part.spill.alloc(spill_need); part.paths.spill = part.spill.get(); }
+        part.ensure_saves(std::min<int64_t>(max_saved_queries, 2 * int64_t(std::max<size_t>(1, pixels))),
+                          scene->view.wide_stack_capacity);
         const int iterations = 2 * std::max(1, tuning.check_interval / 2);
         capture(iterations, block, blocks, stream);
         HIP_CHECK(hipEventRecord(start, stream));
@@ -1110,7 +1131,8 @@ struct pt_batch {
             const uint32_t live = h.n_ext[0];
             if(live == 0) break;
             if(threshold && int64_t(live) <= threshold) {
-                pt::launch_drain(scene->view, frame, part.paths, part.ctl.get(), 0, h.n_shadow, live, drain_block, stream, &bd);
+                pt::launch_drain(scene->view, frame, part.paths, part.ctl.get(), 0, h.n_shadow, h.n_saved[0], live,
+                                 drain_block, stream, &bd);
                 HIP_CHECK(hipGetLastError());
                 st.drained_paths = live;
                 drained = true;
@@ -1546,6 +1568,10 @@ pt_status pt_renderer_set_tuning(pt_renderer *renderer, const pt_tuning *tuning)
             if(tuning->partitions < 1 || tuning->partitions > 8) fail(PT_EINVAL, "partitions must be 1..8");
             t.partitions = tuning->partitions;
         }
+        if(tuning->handoff_steps) {
+            if(tuning->handoff_steps > 65536) fail(PT_EINVAL, "handoff steps must be 1..65536 or -1");
+            t.handoff_steps = tuning->handoff_steps < 0 ? -1 : tuning->handoff_steps;
+        }
         renderer->tuning = t;
     });
 }
@@ -1627,6 +1653,10 @@ pt_status pt_batch_set_tuning(pt_batch *batch, const pt_tuning *tuning) {
         if(tuning->check_interval) { if(tuning->check_interval < 1 || tuning->check_interval > 64) fail(PT_EINVAL, "check interval must be 1..64"); t.check_interval = tuning->check_interval; }
         if(tuning->drain_threshold) t.drain_threshold = tuning->drain_threshold < 0 ? -1 : tuning->drain_threshold;
         if(tuning->lane_refill) { if(tuning->lane_refill < 1 || tuning->lane_refill > 64) fail(PT_EINVAL, "lane refill must be 1..64"); t.lane_refill = tuning->lane_refill; }
+        if(tuning->handoff_steps) {
+            if(tuning->handoff_steps > 65536) fail(PT_EINVAL, "handoff steps must be 1..65536 or -1");
+            t.handoff_steps = tuning->handoff_steps < 0 ? -1 : tuning->handoff_steps;
+        }
         batch->tuning = t;
     });
 }
@@ -1641,6 +1671,7 @@ pt_status pt_batch_autotune(pt_batch *batch, int32_t probe_samples, int32_t pass
             {&pt_tuning::trace_block, {64, 128, 192, 256}},
             {&pt_tuning::blocks_per_cu, {8, 16, 24, 32}},
             {&pt_tuning::lane_refill, {16, 32, 48, 64}},
+            {&pt_tuning::handoff_steps, {-1, 8, 16, 32}},
             {&pt_tuning::shade_block, {256, 512, 1024}},
             {&pt_tuning::drain_threshold, {std::max(1024, px / 32), std::max(2048, px / 16), std::max(4096, px / 8), std::max(8192, px / 4)}},
             {&pt_tuning::check_interval, {2, 4, 8, 16}},

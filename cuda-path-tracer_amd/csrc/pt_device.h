@@ -70,23 +70,36 @@ constexpr int wide_lds_entries = 12;  // per-lane stack entries kept in LDS
 constexpr int32_t empty_root = INT32_MIN;
 
 // Per-pixel path state, structure of arrays in HBM.
+// Traversal state of queries that a trace launch hands on to the next one
+// (persistent lanes, see trace_lanes_kernel): slot k holds the query's
+// pixel (~pixel for a shadow query), current node, stack depth and leaf rank
+// of the closest hit so far (-1: none), that hit, and its stack entries.
+struct saved_queries {
+    int4 *head;         // pixel or ~pixel, cur, count, best_rank
+    int4 *best;         // t, u, v (bit patterns), code
+    int2 *stack;        // entry j of slot k at [k * stride + j]
+};
+
 struct paths_dev {
     float4 *ray_o;      // origin.xyz, time
-    float4 *ray_d;      // direction.xyz, unused
+    float4 *ray_d;      // direction.xyz; w = 1 while the extension query is handed on between launches
     float4 *thr;        // throughput.xyz, previous_pdf
     float4 *rad;        // radiance.xyz, unused
     int2 *ctr;          // depth, samples left in this add_samples call
     int4 *hit;          // t, u, v (bit patterns), code = primitive*2+sphere or -1
     float4 *sh_o;       // shadow origin.xyz, t_max
-    float4 *sh_d;       // shadow direction.xyz, unused
+    float4 *sh_d;       // shadow direction.xyz; w = 1 while the shadow query is handed on
     float4 *sh_c;       // unblocked contribution.xyz, unused
     uint64_t *rng;      // PCG state; the increment is (pixel << 1) | 1
     float *sum;         // 3 floats per pixel (vec3 sum[], cuda_renderer.cu:145)
-    int32_t *queue[2];  // extension queues (pixel ids), ping-pong
+    int32_t *queue[2];  // extension queues (pixel ids; ~pixel: a path waiting for a handed-on query), ping-pong
     int32_t *shadow_queue;
     int32_t *finish_queue; // paths whose sample ended this bounce
     int2 *spill;        // wide traversal stack entries beyond wide_lds_entries
     float *aov;         // optional: 6 floats per pixel, first-hit albedo.xyz and normal.xyz summed over samples
+    saved_queries saved[2]; // written by the trace launch of parity p ^ 1, resumed by the one of parity p
+    int32_t save_capacity;  // slots per buffer
+    int32_t save_stride;    // stack entries per slot (the wide stack capacity)
 };
 
 // Device control block of the bounce loop.
@@ -95,6 +108,8 @@ struct loop_ctl {
     uint32_t n_shadow;
     uint32_t n_finish;
     uint32_t work;           // persistent-fetch cursor of the trace kernel
+    uint32_t n_saved[2];     // queries handed on into saved[p] (may exceed the capacity: then only that many)
+    uint32_t n_waiting;      // ~pixel entries the last shade stage queued (no ray to trace)
     unsigned long long rays; // trace_world calls, shadow queries included
     unsigned long long ext_rays, shadow_rays;
     uint32_t iterations;
@@ -149,9 +164,11 @@ void launch_clear_ctl(loop_ctl *ctl, hipStream_t stream);
 void launch_trace(const scene_dev& s, const frame_params& f, const paths_dev& p, loop_ctl *ctl, int parity,
                   int grid, int block, bool count_tests, hipStream_t stream);
 // Wide-layout trace stage with persistent lanes (refill: idle lanes that
-// trigger a new claim, 1..64).
+// trigger a new claim, 1..64; hand_steps: steps a wave traces once the
+// queues are empty before handing the queries still running on to the next
+// launch, -1 never).
 void launch_trace_lanes(const scene_dev& s, const frame_params& f, const paths_dev& p, loop_ctl *ctl, int parity,
-                        int grid, int block, int refill, bool count_tests, hipStream_t stream,
+                        int grid, int block, int refill, int hand_steps, bool count_tests, hipStream_t stream,
                         const batch_dev *batch = nullptr);
 void launch_bookkeep(loop_ctl *ctl, int parity, hipStream_t stream);
 void launch_shade(const scene_dev& s, const frame_params& f, const paths_dev& p, loop_ctl *ctl, int parity,
@@ -160,8 +177,11 @@ void launch_regen(const frame_params& f, const paths_dev& p, loop_ctl *ctl, int 
                   hipStream_t stream, const batch_dev *batch = nullptr);
 // Finishes the pending shadow queries and then every queued path in one
 // thread each; `shadows`/`paths` are the host's copies of the queue sizes.
+// Tail: shadow queries pending (and queries handed on by the last trace
+// launch), then every remaining path to its end in one thread.
 void launch_drain(const scene_dev& s, const frame_params& f, const paths_dev& p, loop_ctl *ctl, int parity,
-                  uint32_t shadows, uint32_t paths, int block, hipStream_t stream, const batch_dev *batch = nullptr);
+                  uint32_t shadows, uint32_t saved, uint32_t paths, int block, hipStream_t stream,
+                  const batch_dev *batch = nullptr);
 void launch_pack(const float *sum, uint8_t *rgba, int64_t count, int32_t samples, hipStream_t stream);
 void launch_add_rays(loop_ctl *ctl, unsigned long long rays, hipStream_t stream);
 

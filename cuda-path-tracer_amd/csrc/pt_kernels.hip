@@ -896,6 +896,7 @@ __global__ void start_kernel(scene_dev s, frame_params f, paths_dev p, loop_ctl 
     const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if(j == 0) {
         ctl->n_ext[0] = uint32_t(count); ctl->n_ext[1] = 0; ctl->n_shadow = 0; ctl->n_finish = 0; ctl->work = 0;
+        ctl->n_saved[0] = 0; ctl->n_saved[1] = 0; ctl->n_waiting = 0;
         ctl->iterations = 0; ctl->ext_rays = 0; ctl->shadow_rays = 0;
         ctl->box_tests = 0; ctl->triangle_tests = 0; ctl->sphere_tests = 0;
     }
@@ -911,6 +912,7 @@ __global__ void start_kernel(scene_dev s, frame_params f, paths_dev p, loop_ctl 
     p.rad[i] = make_float4(0, 0, 0, 0);
     p.ctr[i] = make_int2(0, samples);
     p.rng[i] = r.state;
+    p.sh_d[i].w = 0;
     p.queue[0][j] = int32_t(i);
 }
 
@@ -990,13 +992,28 @@ __global__ __launch_bounds__(256) void trace_kernel(scene_dev s, frame_params f,
 // `refill` lanes of a wave are idle, the wave claims that many new queries
 // (one atomic) and hands them to its idle lanes, so a wave does not wait for
 // its slowest query before taking new work.
+//
+// Handing on.  Half of a launch used to pass after its queues ran dry,
+// waiting for a few queries of hundreds of steps (rays grazing the surface)
+// in waves with most lanes idle.  Once the queues are empty, a wave traces
+// at most `hand_steps` more steps, then saves the queries still running
+// (node, stack, closest hit so far) into saved[parity ^ 1] and ends; w of
+// the query's direction is set to 1.  The next launch resumes them first,
+// packed into full waves.  Meanwhile the shade stage
+// queues the paths of handed-on queries again as ~pixel, unshaded, and the
+// trace stage skips such entries; a path is shaded once both of its queries
+// are done, so every result, and the order in which a path's radiance
+// terms are added, is unchanged.
 template<bool COUNT, bool BATCH>
 __global__ __launch_bounds__(256) void trace_lanes_kernel(scene_dev s, frame_params f, paths_dev p, loop_ctl *ctl,
-                                                          int parity, int refill, batch_dev bd) {
+                                                          int parity, int refill, int hand_steps, batch_dev bd) {
     extern __shared__ int32_t lds[];
+    const uint32_t n_resume = min(ctl->n_saved[parity], uint32_t(p.save_capacity));
     const uint32_t n_ext = ctl->n_ext[parity];
-    const uint32_t total = n_ext + ctl->n_shadow;
+    const uint32_t ext_end = n_resume + n_ext;
+    const uint32_t total = ext_end + ctl->n_shadow;
     const int32_t *queue = p.queue[parity];
+    const saved_queries in = p.saved[parity], out = p.saved[parity ^ 1];
     const int lane = __lane_id();
     wide_stack wstk{reinterpret_cast<int2 *>(lds) + threadIdx.x, int(blockDim.x), p.spill,
                     size_t(gridDim.x) * blockDim.x, size_t(blockIdx.x) * blockDim.x + threadIdx.x};
@@ -1004,7 +1021,8 @@ __global__ __launch_bounds__(256) void trace_lanes_kernel(scene_dev s, frame_par
     unsigned long long n_box = 0, n_leaf = 0, n_sph = 0;
     int boxes = 0, prims = 0, spheres = 0;
     wide_query q;
-    bool active = false, exhausted = false;
+    bool active = false, exhausted = false, handed = false;
+    int tail_left = hand_steps;  // wave-uniform
     int32_t pix = 0;
     for(;;) {
         const uint64_t idle = __ballot(!active);
@@ -1016,18 +1034,37 @@ __global__ __launch_bounds__(256) void trace_lanes_kernel(scene_dev s, frame_par
             if(b >= total) exhausted = true;
             else if(!active) {
                 const uint32_t item = b + uint32_t(__popcll(idle & ((1ull << lane) - 1ull)));
-                if(item < total) {
-                    if(item < n_ext) {
-                        pix = queue[item];
+                if(item < n_resume) {
+                    const int4 h = in.head[item];
+                    const bool any = h.x < 0;
+                    pix = any ? ~h.x : h.x;
+                    const float4 o = any ? p.sh_o[pix] : p.ray_o[pix], d = any ? p.sh_d[pix] : p.ray_d[pix];
+                    q.r = setup_ray(o.x, o.y, o.z, d.x, d.y, d.z);
+                    wide_begin(query_root<BATCH>(s, bd, pix), q, any, any ? o.w : FLT_MAX);
+                    q.cur = h.y; q.count = h.z; q.best_rank = h.w; q.found = h.w >= 0;
+                    if(q.found) {
+                        const int4 bb = in.best[item];
+                        q.closest = __int_as_float(bb.x);
+                        q.best.t = __int_as_float(bb.x); q.best.u = __int_as_float(bb.y);
+                        q.best.v = __int_as_float(bb.z); q.best.code = bb.w;
+                    }
+                    const int2 *src = in.stack + size_t(item) * size_t(p.save_stride);
+                    for(int k = 0; k < q.count; k++) { const int2 e = src[k]; wstk.store(k, e.x, __int_as_float(e.y)); }
+                    active = true;
+                } else if(item < ext_end) {
+                    const int32_t v = queue[item - n_resume];
+                    if(v >= 0) {  // ~pixel: a path waiting for a handed-on query, nothing to trace
+                        pix = v;
                         const float4 o = p.ray_o[pix], d = p.ray_d[pix];
                         q.r = setup_ray(o.x, o.y, o.z, d.x, d.y, d.z);
                         wide_begin(query_root<BATCH>(s, bd, pix), q, false, FLT_MAX);
-                    } else {
-                        pix = p.shadow_queue[item - n_ext];
-                        const float4 o = p.sh_o[pix], d = p.sh_d[pix];
-                        q.r = setup_ray(o.x, o.y, o.z, d.x, d.y, d.z);
-                        wide_begin(query_root<BATCH>(s, bd, pix), q, true, o.w);
+                        active = true;
                     }
+                } else if(item < total) {
+                    pix = p.shadow_queue[item - ext_end];
+                    const float4 o = p.sh_o[pix], d = p.sh_d[pix];
+                    q.r = setup_ray(o.x, o.y, o.z, d.x, d.y, d.z);
+                    wide_begin(query_root<BATCH>(s, bd, pix), q, true, o.w);
                     active = true;
                 }
             }
@@ -1036,16 +1073,42 @@ __global__ __launch_bounds__(256) void trace_lanes_kernel(scene_dev s, frame_par
             if(exhausted) break;
             continue;
         }
+        if(exhausted && hand_steps > 0 && !handed) {
+            if(tail_left > 0) {
+                tail_left--;
+            } else {
+                handed = true;
+                const uint64_t hand = __ballot(active);
+                uint32_t base = 0;
+                if(lane == 0) base = atomicAdd(&ctl->n_saved[parity ^ 1], uint32_t(__popcll(hand)));
+                base = __shfl(base, 0);
+                const uint32_t slot = base + uint32_t(__popcll(hand & ((1ull << lane) - 1ull)));
+                if(active && slot < uint32_t(p.save_capacity)) {  // beyond the capacity a query runs on
+                    out.head[slot] = make_int4(q.any ? ~pix : pix, q.cur, q.count, q.found ? q.best_rank : -1);
+                    out.best[slot] = make_int4(__float_as_int(q.best.t), __float_as_int(q.best.u),
+                                               __float_as_int(q.best.v), q.best.code);
+                    int2 *dst = out.stack + size_t(slot) * size_t(p.save_stride);
+                    for(int k = 0; k < q.count; k++) dst[k] = wstk.load(k);
+                    if(q.any) p.sh_d[pix].w = 1.0f;
+                    else p.ray_d[pix].w = 1.0f;
+                    active = false;
+                }
+            }
+        }
         if(active && wide_step<COUNT>(s, q, wstk, boxes, prims, spheres, error)) {
             active = false;
             if(!q.any) {
                 p.hit[pix] = q.found ? make_int4(__float_as_int(q.best.t), __float_as_int(q.best.u),
                                                  __float_as_int(q.best.v), q.best.code)
                                      : make_int4(0, 0, 0, -1);
-            } else if(!q.found) {
-                float4 c = p.sh_c[pix], ra = p.rad[pix];
-                ra.x = ra.x + c.x; ra.y = ra.y + c.y; ra.z = ra.z + c.z;
-                p.rad[pix] = ra;
+                p.ray_d[pix].w = 0;
+            } else {
+                if(!q.found) {
+                    float4 c = p.sh_c[pix], ra = p.rad[pix];
+                    ra.x = ra.x + c.x; ra.y = ra.y + c.y; ra.z = ra.z + c.z;
+                    p.rad[pix] = ra;
+                }
+                p.sh_d[pix].w = 0;
             }
             if(COUNT) { n_box += unsigned(boxes); n_leaf += unsigned(prims); n_sph += unsigned(spheres); boxes = prims = spheres = 0; }
         }
@@ -1063,12 +1126,16 @@ __global__ __launch_bounds__(256) void trace_lanes_kernel(scene_dev s, frame_par
 
 __global__ void bookkeep_kernel(loop_ctl *ctl, int parity) {
     uint32_t e = ctl->n_ext[parity], sh = ctl->n_shadow;
+    // ~pixel entries were not traced; resumed queries were counted when they started.
+    const uint32_t w = ctl->n_waiting;
     if(e && ctl->log && ctl->iterations < ctl->log_capacity) {
-        ctl->log[2 * ctl->iterations] = e;
+        ctl->log[2 * ctl->iterations] = e - w;
         ctl->log[2 * ctl->iterations + 1] = sh;
     }
-    ctl->rays += (unsigned long long)e + sh;
-    ctl->ext_rays += e; ctl->shadow_rays += sh;
+    ctl->rays += (unsigned long long)(e - w) + sh;
+    ctl->ext_rays += e - w; ctl->shadow_rays += sh;
+    ctl->n_waiting = 0;
+    ctl->n_saved[parity] = 0;
     ctl->n_shadow = 0;
     ctl->n_finish = 0;
     ctl->n_ext[parity ^ 1] = 0;
@@ -1190,8 +1257,16 @@ __global__ __launch_bounds__(1024) void shade_kernel(scene_dev s, frame_params f
         uint32_t item = base + threadIdx.x;
         bool cont = false, shadow = false, done = false;
         int32_t pix = 0;
+        // A path whose extension or shadow query was handed on waits, queued
+        // again as ~pixel and not shaded, until both of its queries are done.
+        bool wait = false;
         if(item < n) {
-            pix = q[item];
+            const int32_t v = q[item];
+            pix = v >= 0 ? v : ~v;
+            wait = p.ray_d[pix].w != 0 || p.sh_d[pix].w != 0;
+            if(wait) { cont = true; atomicAdd(&ctl->n_waiting, 1u); }
+        }
+        if(item < n && !wait) {
             const int4 hb = p.hit[pix];
             path_state st = load_path(p, pix);
             rng_t r = pixel_rng(p.rng[pix], local_index<BATCH>(bd, pix));
@@ -1209,7 +1284,7 @@ __global__ __launch_bounds__(1024) void shade_kernel(scene_dev s, frame_params f
             else p.rad[pix] = make_float4(st.radiance.x, st.radiance.y, st.radiance.z, 0);
             done = !cont;
         }
-        block_append3(cont, shadow, done, pix, qn, &ctl->n_ext[parity ^ 1], p.shadow_queue, &ctl->n_shadow,
+        block_append3(cont, shadow, done, wait ? ~pix : pix, qn, &ctl->n_ext[parity ^ 1], p.shadow_queue, &ctl->n_shadow,
                       p.finish_queue, &ctl->n_finish);
     }
 }
@@ -1227,7 +1302,7 @@ __device__ __forceinline__ bool trace_query(const scene_dev& s, int32_t root, bo
 // from the last wavefront bounce.
 template<bool BATCH>
 __global__ __launch_bounds__(256) void drain_shadow_kernel(scene_dev s, frame_params f, paths_dev p, loop_ctl *ctl,
-                                                           batch_dev bd) {
+                                                           int parity, batch_dev bd) {
     extern __shared__ int32_t lds[];
     const uint32_t n = ctl->n_shadow;
     const bool brute = f.brute != 0;
@@ -1249,9 +1324,30 @@ __global__ __launch_bounds__(256) void drain_shadow_kernel(scene_dev s, frame_pa
             p.rad[pix] = ra;
         }
     }
+    // Queries the last trace launch handed on: shadow queries are traced here
+    // from the start (the extension ones by drain_kernel, whose queue holds
+    // their paths as ~pixel).  Not counted again.
+    const uint32_t ns = min(ctl->n_saved[parity], uint32_t(p.save_capacity));
+    for(uint32_t item = blockIdx.x * blockDim.x + threadIdx.x; item < ns; item += step) {
+        const int32_t v = p.saved[parity].head[item].x;
+        if(v >= 0) { p.ray_d[v].w = 0; continue; }
+        const int32_t pix = ~v;
+        const float4 o = p.sh_o[pix], d = p.sh_d[pix];
+        best_hit b;
+        if(!trace_query<true>(s, query_root<BATCH>(s, bd, pix), brute, wide, setup_ray(o.x, o.y, o.z, d.x, d.y, d.z), o.w,
+                              stk, stride, wstk, b, error)) {
+            float4 c = p.sh_c[pix], ra = p.rad[pix];
+            ra.x = ra.x + c.x; ra.y = ra.y + c.y; ra.z = ra.z + c.z;
+            p.rad[pix] = ra;
+        }
+        p.sh_d[pix].w = 0;
+    }
     if(error) atomicExch(&ctl->failure, 1);
     if(blockIdx.x == 0 && threadIdx.x == 0) {
-        ctl->rays += n; ctl->shadow_rays += n;
+        // drain_kernel traces the ~pixel entries' extension queries again.
+        const uint32_t w = ctl->n_waiting;
+        ctl->rays = ctl->rays + n - w; ctl->shadow_rays += n; ctl->ext_rays -= w;
+        ctl->n_waiting = 0;
     }
 }
 
@@ -1277,8 +1373,9 @@ __global__ __launch_bounds__(256) void drain_kernel(scene_dev s, frame_params f,
     unsigned long long ext = 0, shadow = 0;
     const uint32_t step = gridDim.x * blockDim.x;
     for(uint32_t item = blockIdx.x * blockDim.x + threadIdx.x; item < n; item += step) {
-        const int32_t pix = q[item];
+        const int32_t pix = q[item] >= 0 ? q[item] : ~q[item];
         path_state st = load_path(p, pix);
+        p.ray_d[pix].w = 0;
         rng_t r = pixel_rng(p.rng[pix], local_index<BATCH>(bd, pix));
         const int32_t root = query_root<BATCH>(s, bd, pix);
         const light_set ls = pixel_lights<BATCH>(s, bd, pix);
@@ -1419,17 +1516,21 @@ void launch_trace(const scene_dev& s, const frame_params& f, const paths_dev& p,
         hipLaunchKernelGGL(trace_kernel<false>, dim3(unsigned(grid)), dim3(unsigned(block)), lds, stream, s, f, p, ctl, parity);
 }
 void launch_trace_lanes(const scene_dev& s, const frame_params& f, const paths_dev& p, loop_ctl *ctl, int parity,
-                        int grid, int block, int refill, bool count_tests, hipStream_t stream, const batch_dev *batch) {
+                        int grid, int block, int refill, int hand_steps, bool count_tests, hipStream_t stream,
+                        const batch_dev *batch) {
     size_t lds = wide_stack_bytes(s.wide_stack_capacity, block);
     const dim3 g{unsigned(grid)}, b{unsigned(block)};
+    // The counting pass runs every query to its end within its launch; nothing
+    // is handed on without save buffers either.
+    const int tail = count_tests || p.save_capacity <= 0 ? -1 : hand_steps;
     if(batch) {
-        if(count_tests) hipLaunchKernelGGL((trace_lanes_kernel<true, true>), g, b, lds, stream, s, f, p, ctl, parity, refill, *batch);
-        else hipLaunchKernelGGL((trace_lanes_kernel<false, true>), g, b, lds, stream, s, f, p, ctl, parity, refill, *batch);
+        if(count_tests) hipLaunchKernelGGL((trace_lanes_kernel<true, true>), g, b, lds, stream, s, f, p, ctl, parity, refill, tail, *batch);
+        else hipLaunchKernelGGL((trace_lanes_kernel<false, true>), g, b, lds, stream, s, f, p, ctl, parity, refill, tail, *batch);
     } else {
         if(count_tests)
-            hipLaunchKernelGGL((trace_lanes_kernel<true, false>), g, b, lds, stream, s, f, p, ctl, parity, refill, batch_dev{});
+            hipLaunchKernelGGL((trace_lanes_kernel<true, false>), g, b, lds, stream, s, f, p, ctl, parity, refill, tail, batch_dev{});
         else
-            hipLaunchKernelGGL((trace_lanes_kernel<false, false>), g, b, lds, stream, s, f, p, ctl, parity, refill, batch_dev{});
+            hipLaunchKernelGGL((trace_lanes_kernel<false, false>), g, b, lds, stream, s, f, p, ctl, parity, refill, tail, batch_dev{});
     }
 }
 void launch_bookkeep(loop_ctl *ctl, int parity, hipStream_t stream) {
@@ -1454,13 +1555,14 @@ void launch_regen(const frame_params& f, const paths_dev& p, loop_ctl *ctl, int 
                            batch_dev{});
 }
 void launch_drain(const scene_dev& s, const frame_params& f, const paths_dev& p, loop_ctl *ctl, int parity,
-                  uint32_t shadows, uint32_t paths, int block, hipStream_t stream, const batch_dev *batch) {
+                  uint32_t shadows, uint32_t saved, uint32_t paths, int block, hipStream_t stream, const batch_dev *batch) {
     size_t lds = trace_lds_bytes(s, f, block);
     const batch_dev bd = batch ? *batch : batch_dev{};
-    if(shadows) {
-        unsigned grid = unsigned((shadows + uint32_t(block) - 1) / uint32_t(block));
-        if(batch) hipLaunchKernelGGL(drain_shadow_kernel<true>, dim3(grid), dim3(unsigned(block)), lds, stream, s, f, p, ctl, bd);
-        else hipLaunchKernelGGL(drain_shadow_kernel<false>, dim3(grid), dim3(unsigned(block)), lds, stream, s, f, p, ctl, bd);
+    saved = std::min(saved, uint32_t(std::max(0, p.save_capacity)));
+    if(shadows || saved) {
+        unsigned grid = unsigned((std::max(shadows, saved) + uint32_t(block) - 1) / uint32_t(block));
+        if(batch) hipLaunchKernelGGL(drain_shadow_kernel<true>, dim3(grid), dim3(unsigned(block)), lds, stream, s, f, p, ctl, parity, bd);
+        else hipLaunchKernelGGL(drain_shadow_kernel<false>, dim3(grid), dim3(unsigned(block)), lds, stream, s, f, p, ctl, parity, bd);
     }
     if(paths) {
         unsigned grid = unsigned((paths + uint32_t(block) - 1) / uint32_t(block));

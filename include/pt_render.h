@@ -188,7 +188,7 @@ pt_status pt_renderer_enable_aovs(pt_renderer *renderer, int32_t enable);
 pt_status pt_renderer_read_aovs(const pt_renderer *renderer, float *albedo, float *normal, size_t count);
 
 /* Per-renderer tuning of the wavefront loop (0 keeps the current value).
- * Defaults: 128, 256, 16, 8, 0, 0, 0, 262144, 48, 1. */
+ * Defaults: 128, 256, 16, 8, 0, 0, 0, 262144, 48, 1, 16. */
 typedef struct pt_tuning {
     int32_t trace_block;     /* threads per block of the trace kernel, multiple of 64 */
     int32_t shade_block;     /* threads per block of the shade kernel, multiple of 64 */
@@ -204,6 +204,10 @@ typedef struct pt_tuning {
                                 trigger a claim of new queries (1..64; -1: one batch per wave) */
     int32_t partitions;      /* pixel partitions whose bounce loops run concurrently on their own
                                 streams (1..8) */
+    int32_t handoff_steps;   /* persistent lanes: steps a wave keeps tracing once the queues are
+                                empty before it saves the queries still running and ends; the next
+                                bounce iteration resumes them first and shades their paths then
+                                (1..65536; -1: never) */
 } pt_tuning;
 pt_status pt_renderer_set_tuning(pt_renderer *renderer, const pt_tuning *tuning);
 
@@ -236,11 +240,11 @@ pt_status pt_batch_stats(const pt_batch *batch, int32_t *samples, float *millise
 pt_status pt_batch_last_launch_stats(const pt_batch *batch, pt_launch_stats *stats);
 /* Fields used: trace_block, shade_block, blocks_per_cu, check_interval (bounce
  * iterations per graph replay, rounded down to even), drain_threshold (live
- * paths at which the tail kernel takes over; -1 never), lane_refill. */
+ * paths at which the tail kernel takes over; -1 never), lane_refill, handoff_steps. */
 pt_status pt_batch_get_tuning(const pt_batch *batch, pt_tuning *tuning);
 pt_status pt_batch_set_tuning(pt_batch *batch, const pt_tuning *tuning);
 /* Autotuner: coordinate descent over workgroup sizes, persistent blocks per
- * CU, lane refill, the drain (compaction) threshold and the graph length,
+ * CU, lane refill, the hand-off step budget, the drain (compaction) threshold and the graph length,
  * each candidate timed on probe_samples samples of the whole batch; `passes`
  * sweeps.  Adopts the best tuning, resets the accumulation, and reports the
  * best rate in Mrays/s and up to trial_capacity trials. */

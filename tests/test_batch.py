@@ -54,6 +54,14 @@ def test_batch_tuning_and_autotuner_keep_images():
     batch.add_samples(4)
     for k in range(3):
         assert np.array_equal(batch.image(k), ref[k]), k
+    rays = batch.stats()["rays"]
+    # Queries handed on between launches after one step on empty queues.
+    batch.reset()
+    batch.set_tuning(handoff_steps=1)
+    batch.add_samples(4)
+    for k in range(3):
+        assert np.array_equal(batch.image(k), ref[k]), k
+    assert batch.stats()["rays"] == rays
 
 
 def test_batch_argument_checks():

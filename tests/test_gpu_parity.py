@@ -345,6 +345,29 @@ def test_pixel_partitions_match_single_loop(parts):
     assert np.array_equal(ref.image(), r.image()) and ref.rays_traced() == r.rays_traced()
 
 
+@pytest.mark.parametrize("steps,drain", [(1, 500), (4, 20000), (2, -1)])
+def test_handoff_leaves_results_unchanged(steps, drain):
+    # Queries still running a few steps after a trace launch's queues are
+    # empty are saved and resumed by the next launch, their paths shaded one
+    # iteration later.  Pixels, PCG states and ray counts must not change,
+    # with the tail kernel taking over while queries are saved (drain 20000)
+    # or never (-1).
+    scene, cam = ptrace.icosphere(40, 3, 256, 192)
+    ref = ptrace.Renderer(scene, cam, 256, 192, 17, False, 0, True, 8)
+    ref.set_tuning(handoff_steps=-1, drain_threshold=drain)
+    ref.add_samples(4)
+    r = ptrace.Renderer(scene, cam, 256, 192, 17, False, 0, True, 8)
+    r.set_tuning(handoff_steps=steps, drain_threshold=drain)
+    r.add_samples(4)
+    (a, ga), (b, gb) = ref.read_state(), r.read_state()
+    assert np.array_equal(a, b) and np.array_equal(ga, gb)
+    assert ref.rays_traced() == r.rays_traced()
+    st, sr = r.last_launch_stats(), ref.last_launch_stats()
+    assert (st["extend_rays"], st["shadow_rays"]) == (sr["extend_rays"], sr["shadow_rays"])
+    ext, sh = r.iteration_log()
+    assert int(ext.sum()) + int(sh.sum()) <= st["extend_rays"] + st["shadow_rays"]
+
+
 def test_multibounce_converges_to_reference_16384spp():
     """Full multi-bounce transport (MIS, the reference's path limit of 50) of
     the small benchmark room converges to the reference host path's own
