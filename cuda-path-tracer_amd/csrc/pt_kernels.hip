@@ -1413,6 +1413,117 @@ __global__ __launch_bounds__(256) void drain_kernel(scene_dev s, frame_params f,
     }
 }
 
+// The tail of part 2 on persistent lanes.  In drain_kernel a lane waits at
+// every bounce for the slowest traversal of its wave, so each bounce of the
+// last pixels costs about the longest of 64 queries.  Here every lane runs a
+// small state machine (trace a query one step at a time, shade, next
+// sample) and a wave shades the lanes that are ready together, once at least
+// `shade_batch` of them wait or no lane is traversing; lanes whose pixel is
+// done take the next pixel.  Per pixel the work and its order are those of
+// drain_kernel: extension query, shade_bounce, shadow query (its
+// contribution added if unblocked), then the next bounce or sample.
+constexpr int drain_shade_batch = 16;  // lanes waiting to shade before a wave shades them
+
+template<bool BATCH>
+__global__ __launch_bounds__(256) void drain_lanes_kernel(scene_dev s, frame_params f, paths_dev p, loop_ctl *ctl,
+                                                          int parity, int shade_batch, batch_dev bd) {
+    extern __shared__ int32_t lds[];
+    const uint32_t n = ctl->n_ext[parity];
+    const int32_t *queue = p.queue[parity];
+    const int lane = __lane_id();
+    wide_stack wstk{reinterpret_cast<int2 *>(lds) + threadIdx.x, int(blockDim.x), p.spill,
+                    size_t(gridDim.x) * blockDim.x, size_t(blockIdx.x) * blockDim.x + threadIdx.x};
+    int error = 0, boxes = 0, prims = 0, spheres = 0;
+    unsigned long long ext = 0, shadow = 0;
+    enum : int { IDLE = 0, TRACE = 1, SHADE = 2 };
+    int phase = IDLE;
+    bool exhausted = false, cont_after = false;
+    int32_t pix = 0, root = 0;
+    path_state st;
+    rng_t r{0, 1};
+    v3 contribution = mk(0, 0, 0);
+    wide_query q;
+    auto begin_ext = [&]() {
+        q.r = setup_ray(st.o.x, st.o.y, st.o.z, st.d.x, st.d.y, st.d.z);
+        wide_begin(root, q, false, FLT_MAX);
+        ext++;
+        phase = TRACE;
+    };
+    // After a bounce's queries: the next bounce, the next sample, or done.
+    auto next_bounce = [&](bool cont) {
+        if(cont) { begin_ext(); return; }
+        float *sum = p.sum + 3 * size_t(pix);
+        sum[0] = sum[0] + st.radiance.x; sum[1] = sum[1] + st.radiance.y; sum[2] = sum[2] + st.radiance.z;
+        if(--st.left <= 0) {
+            p.rng[pix] = r.state;
+            p.ctr[pix] = make_int2(0, 0);
+            phase = IDLE;
+            return;
+        }
+        pixel_camera_ray<BATCH>(f, bd, pix, r, st.o, st.d, st.time);
+        st.throughput = mk(1, 1, 1); st.radiance = mk(0, 0, 0); st.previous_pdf = 0; st.depth = 0;
+        begin_ext();
+    };
+    for(;;) {
+        const uint64_t idle = __ballot(phase == IDLE);
+        if(!exhausted && idle) {
+            uint32_t b = 0;
+            if(lane == 0) b = atomicAdd(&ctl->work, uint32_t(__popcll(idle)));
+            b = __shfl(b, 0);
+            if(b >= n) exhausted = true;
+            else if(phase == IDLE) {
+                const uint32_t item = b + uint32_t(__popcll(idle & ((1ull << lane) - 1ull)));
+                if(item < n) {
+                    const int32_t v = queue[item];
+                    pix = v >= 0 ? v : ~v;
+                    st = load_path(p, pix);
+                    p.ray_d[pix].w = 0;
+                    r = pixel_rng(p.rng[pix], local_index<BATCH>(bd, pix));
+                    root = query_root<BATCH>(s, bd, pix);
+                    begin_ext();
+                }
+            }
+        }
+        const uint64_t tracing = __ballot(phase == TRACE), ready = __ballot(phase == SHADE);
+        if((tracing | ready) == 0) {
+            if(exhausted) break;
+            continue;
+        }
+        if(ready && (__popcll(ready) >= shade_batch || tracing == 0)) {
+            if(phase == SHADE) {
+                shadow_query sq;
+                bool has_shadow;
+                const bool cont = shade_bounce(s, f, pixel_lights<BATCH>(s, bd, pix), st, q.found, q.best, r, sq, has_shadow,
+                                               p.aov ? p.aov + 6 * size_t(pix) : nullptr);
+                if(has_shadow) {
+                    shadow++;
+                    contribution = sq.contribution;
+                    cont_after = cont;
+                    q.r = setup_ray(sq.o.x, sq.o.y, sq.o.z, sq.d.x, sq.d.y, sq.d.z);
+                    wide_begin(root, q, true, sq.t_max);
+                    phase = TRACE;
+                } else {
+                    next_bounce(cont);
+                }
+            }
+        } else if(phase == TRACE && wide_step<false>(s, q, wstk, boxes, prims, spheres, error)) {
+            if(!q.any) {
+                phase = SHADE;
+            } else {
+                if(!q.found) st.radiance = add(st.radiance, contribution);
+                next_bounce(cont_after);
+            }
+        }
+    }
+    if(error) atomicExch(&ctl->failure, 1);
+    ext = wave_sum(ext); shadow = wave_sum(shadow);
+    if(lane == 0 && (ext || shadow)) {
+        atomicAdd(&ctl->rays, ext + shadow);
+        atomicAdd(&ctl->ext_rays, ext);
+        atomicAdd(&ctl->shadow_rays, shadow);
+    }
+}
+
 // End of a sample (cuda_renderer.cu:81): accumulated += radiance in sample
 // order, then the pixel's next camera ray while samples remain.
 template<bool BATCH>
@@ -1566,10 +1677,19 @@ void launch_drain(const scene_dev& s, const frame_params& f, const paths_dev& p,
     }
     if(paths) {
         unsigned grid = unsigned((paths + uint32_t(block) - 1) / uint32_t(block));
-        if(batch)
+        const bool wide = !f.brute && s.wroot_ref != empty_root && !f.exact_traversal;
+        if(wide) {  // one lane per path, each on its own state machine
+            if(batch)
+                hipLaunchKernelGGL(drain_lanes_kernel<true>, dim3(grid), dim3(unsigned(block)), lds, stream, s, f, p, ctl,
+                                   parity, drain_shade_batch, bd);
+            else
+                hipLaunchKernelGGL(drain_lanes_kernel<false>, dim3(grid), dim3(unsigned(block)), lds, stream, s, f, p, ctl,
+                                   parity, drain_shade_batch, bd);
+        } else if(batch) {
             hipLaunchKernelGGL(drain_kernel<true>, dim3(grid), dim3(unsigned(block)), lds, stream, s, f, p, ctl, parity, bd);
-        else
+        } else {
             hipLaunchKernelGGL(drain_kernel<false>, dim3(grid), dim3(unsigned(block)), lds, stream, s, f, p, ctl, parity, bd);
+        }
     }
 }
 void launch_pack(const float *sum, uint8_t *rgba, int64_t count, int32_t samples, hipStream_t stream) {
