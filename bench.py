@@ -34,6 +34,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path[:0] = [os.path.join(HERE, "cuda-path-tracer_amd"), os.path.join(HERE, "oracle")]
 
 WIDTH, HEIGHT, SPP, DEPTH, FREQ, SEED = 1920, 1080, 64, 8, 224, 7
+DRAIN = 1 << 30  # the renderer's default: every path on the path-lane kernel
 HBM_PEAK_GBS = 8000.0
 
 
@@ -41,16 +42,17 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def traffic_from_profiles():
-    """HBM bytes per trace launch measured by rocprofv3 --pmc (committed summary), or None."""
-    path = os.path.join(HERE, "profiles", "trace_kernel_icosphere1M_1080p_traffic.json")
+def traffic_from_profiles(kernel):
+    """HBM bytes per launch of `kernel` measured by rocprofv3 --pmc (committed summary), or None."""
+    path = os.path.join(HERE, "profiles", "path_kernel_icosphere1M_1080p_traffic.json")
     if not os.path.exists(path):
         return None
     try:
         with open(path) as f:
-            return json.load(f).get("hbm_bytes_per_launch")
+            data = json.load(f)
     except (OSError, ValueError):
         return None
+    return data.get("hbm_bytes_per_launch") if data.get("kernel") == kernel else None
 
 
 def cpu_baseline(scene, camera, threads, spp=1):
@@ -174,15 +176,16 @@ def main():
     r = ptrace.Renderer(scene, camera, WIDTH, HEIGHT, SEED, False, ptrace.DEBUG_SHADED, True, DEPTH)
     log(f"upload {time.perf_counter() - t0:.2f}s")
 
-    # Untimed counting pass (same rays as every timed frame: same seed, reset).
-    # The test counts cover the trace kernel's launches only; the queries it
-    # handled are the per-iteration queue sizes (the tail kernel traces the rest).
-    r.set_tuning(count_tests=True)
+    # Untimed counting pass: the same frame (same seed, reset) on the wavefront
+    # bounce loop with no tail kernel, so every query goes through the trace
+    # kernel and its box/triangle/sphere tests are counted.  The queries and
+    # their tests are the same whichever kernel traces them.
+    r.set_tuning(count_tests=True, drain_threshold=-1)
     r.add_samples(args.spp)
     counts = r.last_launch_stats()
     ext_log, sh_log = r.iteration_log()
     traced_ext, traced_sh = int(ext_log.sum()), int(sh_log.sum())
-    r.set_tuning(count_tests=False, time_kernels=True)
+    r.set_tuning(count_tests=False, time_kernels=True, drain_threshold=DRAIN)
     log("counting pass:", counts, "trace-kernel queries:", traced_ext, traced_sh)
 
     for _ in range(args.warmup):
@@ -207,16 +210,20 @@ def main():
     mrays = rays / elapsed / 1e6
     log(f"{args.steps} steps: {elapsed:.3f}s, {mrays:.1f} Mrays/s, trace {trace_ms:.1f} ms, shade {shade_ms:.1f} ms")
 
+    # Dominant kernel: the path-lane kernel (one launch per frame, every
+    # query of the frame) by default, else the wavefront trace kernel.
     alg_bytes_frame = (32 * counts["box_tests"] + 36 * counts["triangle_tests"] + 16 * counts["sphere_tests"]
                        + 48 * traced_ext + 36 * traced_sh)
-    per_step_launches = launches / args.steps
-    bytes_per_launch = alg_bytes_frame / per_step_launches
-    launch_s = trace_ms / 1e3 / launches
+    if launches == 0:
+        kernel, bytes_per_launch, launch_s = "path_lanes_kernel<false>", alg_bytes_frame, drain_ms / 1e3 / args.steps
+    else:
+        kernel, bytes_per_launch, launch_s = ("trace_lanes_kernel<false, false>", alg_bytes_frame / (launches / args.steps),
+                                              trace_ms / 1e3 / launches)
     achieved = bytes_per_launch / launch_s / 1e9
-    traffic = traffic_from_profiles()
+    traffic = traffic_from_profiles(kernel)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "trace_lanes_kernel<false, false>", "avg_launch_ms": round(launch_s * 1e3, 4),
+                "kernel": kernel, "avg_launch_ms": round(launch_s * 1e3, 4),
                 "alg_bytes_per_launch": int(bytes_per_launch),
                 "box_tests_per_query": round(counts["box_tests"] / (traced_ext + traced_sh), 2),
                 "triangle_tests_per_query": round(counts["triangle_tests"] / (traced_ext + traced_sh), 2)}
@@ -232,7 +239,7 @@ def main():
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 2),
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
             "data": "synthetic (seeded displaced icosphere, seed 7)",
-            "config": {"workload": "plain wavefront path tracing, icosphere 1,003,530 tris, 1920x1080, "
+            "config": {"workload": "plain path tracing (path lanes), icosphere 1,003,530 tris, 1920x1080, "
                                    f"{args.spp} spp, depth {DEPTH}, MIS", "width": WIDTH, "height": HEIGHT,
                        "spp": args.spp, "max_depth": DEPTH, "triangles": int(len(scene.triangles)),
                        "rays_per_step": rays // args.steps, "trace_ms_per_step": round(trace_ms / args.steps, 2),

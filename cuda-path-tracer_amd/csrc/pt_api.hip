@@ -628,6 +628,7 @@ struct partition {
 
 constexpr int partition_chunk = 4096;  // pixels per chunk dealt round-robin to partitions
 constexpr int64_t max_saved_queries = 262144;  // queries one trace launch can hand on to the next
+constexpr int32_t all_paths = 1 << 30;         // drain threshold: every path on path lanes
 constexpr uint32_t log_capacity = 1u << 16;
 
 struct pt_renderer {
@@ -652,7 +653,7 @@ struct pt_renderer {
     dev_buf<int4> hit;
     std::vector<std::unique_ptr<partition>> parts;
     dev_buf<float> aov;                  // first-hit albedo and normal sums (6 per pixel), when enabled
-    pt_tuning tuning{128, 256, 16, 8, 0, 0, 0, 262144, 48, 1, 16};
+    pt_tuning tuning{128, 256, 16, 8, 0, 0, 0, all_paths, 48, 1, 16, 24, -1};
     int cus = 1;
     pt_launch_stats last{};
 
@@ -678,6 +679,9 @@ struct pt_renderer {
     // Largest block of at most tuning.trace_block threads whose traversal
     // stacks fit the 64 KiB of LDS a workgroup may hold; the wide traversal is
     // switched off if even 64 threads do not fit.
+    pt::path_lanes path_lanes() const {
+        return pt::path_lanes{tuning.path_shade_batch, tuning.path_blocks_per_cu > 0 ? cus * tuning.path_blocks_per_cu : 0};
+    }
     int trace_block() {
         frame.exact_traversal = tuning.exact_traversal;
         int block = tuning.trace_block;
@@ -768,7 +772,11 @@ struct pt_renderer {
         const int trace_grid = cus * std::max(1, tuning.blocks_per_cu);
         const int drain_block = 256;
         const int64_t drain_threshold = tuning.drain_threshold > 0 ? std::max<int64_t>(1, tuning.drain_threshold / nparts) : 0;
-        const size_t drain_threads = drain_threshold ? size_t((drain_threshold + drain_block - 1) / drain_block * drain_block) : 0;
+        int64_t largest = 0;
+        for(auto& part : parts) largest = std::max(largest, part->count);
+        const int64_t drain_paths = std::min(drain_threshold, largest);  // most paths one drain launch can take
+        size_t drain_threads = drain_paths > 0 ? size_t((drain_paths + drain_block - 1) / drain_block * drain_block) : 0;
+        if(path_lanes().blocks > 0) drain_threads = std::min(drain_threads, size_t(path_lanes().blocks) * size_t(drain_block));
         const size_t spill_need = std::max(size_t(trace_grid) * size_t(trace_threads), drain_threads) *
                                   size_t(pt::wide_spill_entries(scene->view.wide_stack_capacity));
         const int interval = std::max(1, tuning.check_interval);
@@ -788,6 +796,16 @@ struct pt_renderer {
             part->drained = false;
             pt::launch_start(scene->view, frame, part->paths, part->ctl.get(), count, part->index, nparts,
                              partition_chunk, part->count, part->stream);
+            if(!part->done && drain_threshold && part->count <= drain_threshold) {
+                // Few enough paths for the tail kernel from the start: no wavefront bounces.
+                if(tuning.time_kernels) HIP_CHECK(hipEventRecord(part->drain0, part->stream));
+                pt::launch_drain(scene->view, frame, part->paths, part->ctl.get(), 0, 0, 0, uint32_t(part->count),
+                                 drain_block, path_lanes(), part->stream);
+                if(tuning.time_kernels) HIP_CHECK(hipEventRecord(part->drain1, part->stream));
+                st.drained_paths += uint64_t(part->count);
+                part->read_ctl_async();
+                part->done = part->drained = true;
+            }
         }
         HIP_CHECK(hipGetLastError());
         auto launch_iterations = [&](partition& part) {
@@ -838,8 +856,7 @@ struct pt_renderer {
                 if(drain_threshold && int64_t(live) <= drain_threshold) {
                     if(tuning.time_kernels) HIP_CHECK(hipEventRecord(part->drain0, part->stream));
                     pt::launch_drain(scene->view, frame, part->paths, part->ctl.get(), part->parity, h.n_shadow,
-                                     h.n_saved[part->parity], live,
-                                     drain_block, part->stream);
+                                     h.n_saved[part->parity], live, drain_block, path_lanes(), part->stream);
                     HIP_CHECK(hipGetLastError());
                     if(tuning.time_kernels) HIP_CHECK(hipEventRecord(part->drain1, part->stream));
                     st.drained_paths += live;
@@ -1039,7 +1056,7 @@ struct pt_batch {
     partition part;
     hipStream_t stream = nullptr;
     hipEvent_t start = nullptr, stop = nullptr;
-    pt_tuning tuning{128, 256, 16, 8, 0, 0, 0, 0, 48, 1, 16};
+    pt_tuning tuning{128, 256, 16, 8, 0, 0, 0, 0, 48, 1, 16, 24, -1};
     int cus = 1;
     int32_t samples = 0;
     uint64_t rays = 0;
@@ -1056,6 +1073,9 @@ struct pt_batch {
         if(stream) (void)hipStreamDestroy(stream);
     }
 
+    pt::path_lanes path_lanes() const {
+        return pt::path_lanes{tuning.path_shade_batch, tuning.path_blocks_per_cu > 0 ? cus * tuning.path_blocks_per_cu : 0};
+    }
     int trace_block() {
         int block = tuning.trace_block;
         while(block > 64 && pt::trace_lds_bytes(scene->view, frame, block) > 65536) block -= 64;
@@ -1108,7 +1128,9 @@ struct pt_batch {
         const int drain_block = 256;
         const int64_t threshold = tuning.drain_threshold > 0 ? tuning.drain_threshold
                                 : tuning.drain_threshold < 0 ? 0 : std::max<int64_t>(4096, int64_t(pixels) / 8);
-        const size_t drain_threads = threshold ? size_t((threshold + drain_block - 1) / drain_block * drain_block) : 0;
+        const int64_t drain_paths = std::min<int64_t>(threshold, int64_t(pixels));  // most paths one drain launch can take
+        size_t drain_threads = drain_paths > 0 ? size_t((drain_paths + drain_block - 1) / drain_block * drain_block) : 0;
+        if(path_lanes().blocks > 0) drain_threads = std::min(drain_threads, size_t(path_lanes().blocks) * size_t(drain_block));
         const size_t spill_need = std::max(size_t(blocks) * size_t(block), drain_threads) *
                                   size_t(pt::wide_spill_entries(scene->view.wide_stack_capacity));
         if(part.spill.size() < spill_need) { part.spill.alloc(spill_need); part.paths.spill = part.spill.get(); }
@@ -1121,7 +1143,15 @@ struct pt_batch {
                          stream, &bd);
         HIP_CHECK(hipGetLastError());
         bool drained = false;
-        for(;;) {
+        if(threshold && int64_t(pixels) <= threshold) {
+            // Few enough paths for the tail kernel from the start: no wavefront bounces.
+            pt::launch_drain(scene->view, frame, part.paths, part.ctl.get(), 0, 0, 0, uint32_t(pixels), drain_block,
+                             path_lanes(), stream, &bd);
+            HIP_CHECK(hipGetLastError());
+            st.drained_paths = pixels;
+            drained = true;
+        }
+        for(; !drained;) {
             HIP_CHECK(hipGraphLaunch(graph, stream));
             part.read_ctl_async(stream);
             st.trace_launches += iterations;
@@ -1132,7 +1162,7 @@ struct pt_batch {
             if(live == 0) break;
             if(threshold && int64_t(live) <= threshold) {
                 pt::launch_drain(scene->view, frame, part.paths, part.ctl.get(), 0, h.n_shadow, h.n_saved[0], live,
-                                 drain_block, stream, &bd);
+                                 drain_block, path_lanes(), stream, &bd);
                 HIP_CHECK(hipGetLastError());
                 st.drained_paths = live;
                 drained = true;
@@ -1572,6 +1602,14 @@ pt_status pt_renderer_set_tuning(pt_renderer *renderer, const pt_tuning *tuning)
             if(tuning->handoff_steps > 65536) fail(PT_EINVAL, "handoff steps must be 1..65536 or -1");
             t.handoff_steps = tuning->handoff_steps < 0 ? -1 : tuning->handoff_steps;
         }
+        if(tuning->path_shade_batch) {
+            if(tuning->path_shade_batch < 1 || tuning->path_shade_batch > 64) fail(PT_EINVAL, "path shade batch must be 1..64");
+            t.path_shade_batch = tuning->path_shade_batch;
+        }
+        if(tuning->path_blocks_per_cu) {
+            if(tuning->path_blocks_per_cu > 64) fail(PT_EINVAL, "path blocks per CU must be 1..64 or -1");
+            t.path_blocks_per_cu = tuning->path_blocks_per_cu < 0 ? -1 : tuning->path_blocks_per_cu;
+        }
         renderer->tuning = t;
     });
 }
@@ -1657,6 +1695,14 @@ pt_status pt_batch_set_tuning(pt_batch *batch, const pt_tuning *tuning) {
             if(tuning->handoff_steps > 65536) fail(PT_EINVAL, "handoff steps must be 1..65536 or -1");
             t.handoff_steps = tuning->handoff_steps < 0 ? -1 : tuning->handoff_steps;
         }
+        if(tuning->path_shade_batch) {
+            if(tuning->path_shade_batch < 1 || tuning->path_shade_batch > 64) fail(PT_EINVAL, "path shade batch must be 1..64");
+            t.path_shade_batch = tuning->path_shade_batch;
+        }
+        if(tuning->path_blocks_per_cu) {
+            if(tuning->path_blocks_per_cu > 64) fail(PT_EINVAL, "path blocks per CU must be 1..64 or -1");
+            t.path_blocks_per_cu = tuning->path_blocks_per_cu < 0 ? -1 : tuning->path_blocks_per_cu;
+        }
         batch->tuning = t;
     });
 }
@@ -1673,7 +1719,8 @@ pt_status pt_batch_autotune(pt_batch *batch, int32_t probe_samples, int32_t pass
             {&pt_tuning::lane_refill, {16, 32, 48, 64}},
             {&pt_tuning::handoff_steps, {-1, 8, 16, 32}},
             {&pt_tuning::shade_block, {256, 512, 1024}},
-            {&pt_tuning::drain_threshold, {std::max(1024, px / 32), std::max(2048, px / 16), std::max(4096, px / 8), std::max(8192, px / 4)}},
+            {&pt_tuning::drain_threshold, {std::max(1024, px / 32), std::max(2048, px / 16), std::max(4096, px / 8), std::max(8192, px / 4), all_paths}},
+            {&pt_tuning::path_shade_batch, {16, 24, 32}},
             {&pt_tuning::check_interval, {2, 4, 8, 16}},
         };
         int32_t count = 0;

@@ -179,8 +179,11 @@ void launch_regen(const frame_params& f, const paths_dev& p, loop_ctl *ctl, int 
 // thread each; `shadows`/`paths` are the host's copies of the queue sizes.
 // Tail: shadow queries pending (and queries handed on by the last trace
 // launch), then every remaining path to its end in one thread.
+// Path lanes of the tail kernel: lanes of a wave waiting to shade before the
+// wave shades them, and the resident grid (blocks 0: one lane per path).
+struct path_lanes { int shade_batch; int blocks; };
 void launch_drain(const scene_dev& s, const frame_params& f, const paths_dev& p, loop_ctl *ctl, int parity,
-                  uint32_t shadows, uint32_t saved, uint32_t paths, int block, hipStream_t stream,
+                  uint32_t shadows, uint32_t saved, uint32_t paths, int block, path_lanes lanes, hipStream_t stream,
                   const batch_dev *batch = nullptr);
 void launch_pack(const float *sum, uint8_t *rgba, int64_t count, int32_t samples, hipStream_t stream);
 void launch_add_rays(loop_ctl *ctl, unsigned long long rays, hipStream_t stream);

@@ -1413,19 +1413,21 @@ __global__ __launch_bounds__(256) void drain_kernel(scene_dev s, frame_params f,
     }
 }
 
-// The tail of part 2 on persistent lanes.  In drain_kernel a lane waits at
-// every bounce for the slowest traversal of its wave, so each bounce of the
-// last pixels costs about the longest of 64 queries.  Here every lane runs a
-// small state machine (trace a query one step at a time, shade, next
-// sample) and a wave shades the lanes that are ready together, once at least
-// `shade_batch` of them wait or no lane is traversing; lanes whose pixel is
-// done take the next pixel.  Per pixel the work and its order are those of
-// drain_kernel: extension query, shade_bounce, shadow query (its
-// contribution added if unblocked), then the next bounce or sample.
-constexpr int drain_shade_batch = 16;  // lanes waiting to shade before a wave shades them
-
+// Path lanes: every remaining pixel of an add_samples call on a lane of its
+// own (the renderer's default for all pixels; the tail of the wavefront loop
+// otherwise).  Each lane runs a small state machine -- trace its current
+// query one step (wide node or leaf cluster) at a time, shade, next bounce or
+// sample -- and takes the next pixel when its pixel is done.  A wave shades
+// the lanes that are ready together, once at least `shade_batch` of them wait
+// or no lane is traversing, so the shading code runs with many lanes active
+// and no lane waits for the slowest traversal of its wave (as in
+// drain_kernel, where each bounce costs the longest of 64 queries).  Path
+// state stays in registers: no queues, no per-bounce launches.  Per pixel the
+// work and its order are those of drain_kernel and of the wavefront loop:
+// extension query, shade_bounce, shadow query (its contribution added if
+// unblocked), then the next bounce or sample.
 template<bool BATCH>
-__global__ __launch_bounds__(256) void drain_lanes_kernel(scene_dev s, frame_params f, paths_dev p, loop_ctl *ctl,
+__global__ __launch_bounds__(256) void path_lanes_kernel(scene_dev s, frame_params f, paths_dev p, loop_ctl *ctl,
                                                           int parity, int shade_batch, batch_dev bd) {
     extern __shared__ int32_t lds[];
     const uint32_t n = ctl->n_ext[parity];
@@ -1666,7 +1668,8 @@ void launch_regen(const frame_params& f, const paths_dev& p, loop_ctl *ctl, int 
                            batch_dev{});
 }
 void launch_drain(const scene_dev& s, const frame_params& f, const paths_dev& p, loop_ctl *ctl, int parity,
-                  uint32_t shadows, uint32_t saved, uint32_t paths, int block, hipStream_t stream, const batch_dev *batch) {
+                  uint32_t shadows, uint32_t saved, uint32_t paths, int block, path_lanes lanes, hipStream_t stream,
+                  const batch_dev *batch) {
     size_t lds = trace_lds_bytes(s, f, block);
     const batch_dev bd = batch ? *batch : batch_dev{};
     saved = std::min(saved, uint32_t(std::max(0, p.save_capacity)));
@@ -1678,13 +1681,15 @@ void launch_drain(const scene_dev& s, const frame_params& f, const paths_dev& p,
     if(paths) {
         unsigned grid = unsigned((paths + uint32_t(block) - 1) / uint32_t(block));
         const bool wide = !f.brute && s.wroot_ref != empty_root && !f.exact_traversal;
-        if(wide) {  // one lane per path, each on its own state machine
+        if(wide) {  // lanes on their own state machines, taking path after path
+            const unsigned g = lanes.blocks > 0 ? std::min(grid, unsigned(lanes.blocks)) : grid;
+            const int batch_lanes = std::max(1, std::min(64, lanes.shade_batch));
             if(batch)
-                hipLaunchKernelGGL(drain_lanes_kernel<true>, dim3(grid), dim3(unsigned(block)), lds, stream, s, f, p, ctl,
-                                   parity, drain_shade_batch, bd);
+                hipLaunchKernelGGL(path_lanes_kernel<true>, dim3(g), dim3(unsigned(block)), lds, stream, s, f, p, ctl,
+                                   parity, batch_lanes, bd);
             else
-                hipLaunchKernelGGL(drain_lanes_kernel<false>, dim3(grid), dim3(unsigned(block)), lds, stream, s, f, p, ctl,
-                                   parity, drain_shade_batch, bd);
+                hipLaunchKernelGGL(path_lanes_kernel<false>, dim3(g), dim3(unsigned(block)), lds, stream, s, f, p, ctl,
+                                   parity, batch_lanes, bd);
         } else if(batch) {
             hipLaunchKernelGGL(drain_kernel<true>, dim3(grid), dim3(unsigned(block)), lds, stream, s, f, p, ctl, parity, bd);
         } else {

@@ -188,7 +188,7 @@ pt_status pt_renderer_enable_aovs(pt_renderer *renderer, int32_t enable);
 pt_status pt_renderer_read_aovs(const pt_renderer *renderer, float *albedo, float *normal, size_t count);
 
 /* Per-renderer tuning of the wavefront loop (0 keeps the current value).
- * Defaults: 128, 256, 16, 8, 0, 0, 0, 262144, 48, 1, 16. */
+ * Defaults: 128, 256, 16, 8, 0, 0, 0, 1073741824 (every path on path lanes), 48, 1, 16, 24, -1. */
 typedef struct pt_tuning {
     int32_t trace_block;     /* threads per block of the trace kernel, multiple of 64 */
     int32_t shade_block;     /* threads per block of the shade kernel, multiple of 64 */
@@ -208,6 +208,10 @@ typedef struct pt_tuning {
                                 empty before it saves the queries still running and ends; the next
                                 bounce iteration resumes them first and shades their paths then
                                 (1..65536; -1: never) */
+    int32_t path_shade_batch;  /* path lanes (the tail kernel): lanes of a wave that wait to shade
+                                  before the wave shades them together (1..64) */
+    int32_t path_blocks_per_cu; /* path lanes: resident 256-thread blocks per compute unit, each lane
+                                  taking pixel after pixel (-1: one lane per remaining pixel) */
 } pt_tuning;
 pt_status pt_renderer_set_tuning(pt_renderer *renderer, const pt_tuning *tuning);
 

@@ -368,6 +368,35 @@ def test_handoff_leaves_results_unchanged(steps, drain):
     assert int(ext.sum()) + int(sh.sum()) <= st["extend_rays"] + st["shadow_rays"]
 
 
+@pytest.mark.parametrize("scene_name,blocks,batch", [("icosphere", -1, 16), ("icosphere", 1, 1), ("cornell", 2, 64),
+                                                      ("cornell", -1, 16)])
+def test_path_lanes_match_wavefront(scene_name, blocks, batch):
+    # The path-lane kernel (every remaining pixel on a lane's state machine,
+    # shading batched per wave, lanes taking pixel after pixel) must equal the
+    # wavefront bounce loop exactly: pixels, PCG states and ray counts.
+    if scene_name == "icosphere":
+        scene, cam = ptrace.icosphere(40, 3, 256, 192)
+        w, h = 256, 192
+    else:
+        scene, cam = ptrace.cornell(7, 160, 100)
+        w, h = 160, 100
+    ref = ptrace.Renderer(scene, cam, w, h, 23, False, 0, True, 8)
+    ref.set_tuning(drain_threshold=-1)
+    ref.add_samples(3)
+    ref.add_samples(2)
+    r = ptrace.Renderer(scene, cam, w, h, 23, False, 0, True, 8)
+    r.set_tuning(drain_threshold=1 << 30, path_blocks_per_cu=blocks, path_shade_batch=batch)
+    r.add_samples(3)
+    r.add_samples(2)
+    st = r.last_launch_stats()
+    assert st["trace_launches"] == 0 and st["drained_paths"] == w * h
+    (a, ga), (b, gb) = ref.read_state(), r.read_state()
+    assert np.array_equal(a, b) and np.array_equal(ga, gb)
+    assert ref.rays_traced() == r.rays_traced()
+    sr = ref.last_launch_stats()
+    assert (st["extend_rays"], st["shadow_rays"]) == (sr["extend_rays"], sr["shadow_rays"])
+
+
 def test_multibounce_converges_to_reference_16384spp():
     """Full multi-bounce transport (MIS, the reference's path limit of 50) of
     the small benchmark room converges to the reference host path's own
