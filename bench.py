@@ -102,7 +102,7 @@ def bench_batch(args, ptrace):
     tuned = None
     if args.autotune:
         t0 = time.perf_counter()
-        best, rate, trials = b.autotune(probe_samples=1, passes=1)
+        best, rate, trials = b.autotune(probe_samples=4, passes=1)
         tuned = {"best": best, "probe_mrays": round(rate, 1), "trials": len(trials),
                  "seconds": round(time.perf_counter() - t0, 1)}
         log("autotune:", tuned)

@@ -653,7 +653,7 @@ struct pt_renderer {
     dev_buf<int4> hit;
     std::vector<std::unique_ptr<partition>> parts;
     dev_buf<float> aov;                  // first-hit albedo and normal sums (6 per pixel), when enabled
-    pt_tuning tuning{128, 256, 16, 8, 0, 0, 0, all_paths, 48, 1, 16, 24, -1};
+    pt_tuning tuning{128, 256, 16, 8, 0, 0, 0, all_paths, 48, 1, 16, 24, 12};
     int cus = 1;
     pt_launch_stats last{};
 
@@ -1056,13 +1056,15 @@ struct pt_batch {
     partition part;
     hipStream_t stream = nullptr;
     hipEvent_t start = nullptr, stop = nullptr;
-    pt_tuning tuning{128, 256, 16, 8, 0, 0, 0, 0, 48, 1, 16, 24, -1};
+    pt_tuning tuning{128, 256, 16, 8, 0, 0, 0, all_paths, 48, 1, 16, 24, 12};
     int cus = 1;
     int32_t samples = 0;
     uint64_t rays = 0;
     float milliseconds = 0;
     hipGraphExec_t graph = nullptr;
     int graph_iterations = 0, graph_trace_block = 0, graph_blocks = 0, graph_shade_block = 0, graph_refill = 0, graph_tail = 0;
+    const int2 *graph_spill = nullptr;     // buffers the captured kernels point to
+    const int4 *graph_saved = nullptr;
     pt_launch_stats last{};
 
     ~pt_batch() {
@@ -1095,9 +1097,13 @@ struct pt_batch {
     // The bounce loop as one graph: trace, bookkeep, shade, regenerate,
     // `iterations` times (even, so each replay ends on parity 0).
     void capture(int iterations, int block, int blocks, hipStream_t s) {
+        // The graph holds its kernels' arguments by value: any buffer they
+        // point to that was reallocated since (the traversal spill region
+        // grows with the drain threshold and launch shapes) needs a new graph.
         if(graph && graph_iterations == iterations && graph_trace_block == block && graph_blocks == blocks &&
            graph_shade_block == tuning.shade_block && graph_refill == tuning.lane_refill &&
-           graph_tail == tuning.handoff_steps)
+           graph_tail == tuning.handoff_steps && graph_spill == part.paths.spill &&
+           graph_saved == part.paths.saved[0].head)
             return;
         if(graph) { HIP_CHECK(hipGraphExecDestroy(graph)); graph = nullptr; }
         const int shade_grid = int(std::max<size_t>(1, std::min<size_t>(
@@ -1118,6 +1124,7 @@ struct pt_batch {
         HIP_CHECK(e);
         graph_iterations = iterations; graph_trace_block = block; graph_blocks = blocks;
         graph_shade_block = tuning.shade_block; graph_refill = tuning.lane_refill; graph_tail = tuning.handoff_steps;
+        graph_spill = part.paths.spill; graph_saved = part.paths.saved[0].head;
     }
 
     void add_samples(int32_t count) {

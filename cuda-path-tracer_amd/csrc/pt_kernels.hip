@@ -1426,8 +1426,11 @@ __global__ __launch_bounds__(256) void drain_kernel(scene_dev s, frame_params f,
 // work and its order are those of drain_kernel and of the wavefront loop:
 // extension query, shade_bounce, shadow query (its contribution added if
 // unblocked), then the next bounce or sample.
+// 4 waves per SIMD (at most 128 VGPRs, a few spilled): 2150 against 1868
+// Mrays/s at the compiler's own 140 VGPRs and 3 waves; 5 waves spill 106
+// VGPRs and fall to 1613.
 template<bool BATCH>
-__global__ __launch_bounds__(256) void path_lanes_kernel(scene_dev s, frame_params f, paths_dev p, loop_ctl *ctl,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void path_lanes_kernel(scene_dev s, frame_params f, paths_dev p, loop_ctl *ctl,
                                                           int parity, int shade_batch, batch_dev bd) {
     extern __shared__ int32_t lds[];
     const uint32_t n = ctl->n_ext[parity];

@@ -188,7 +188,7 @@ pt_status pt_renderer_enable_aovs(pt_renderer *renderer, int32_t enable);
 pt_status pt_renderer_read_aovs(const pt_renderer *renderer, float *albedo, float *normal, size_t count);
 
 /* Per-renderer tuning of the wavefront loop (0 keeps the current value).
- * Defaults: 128, 256, 16, 8, 0, 0, 0, 1073741824 (every path on path lanes), 48, 1, 16, 24, -1. */
+ * Defaults: 128, 256, 16, 8, 0, 0, 0, 1073741824 (every path on path lanes), 48, 1, 16, 24, 12. */
 typedef struct pt_tuning {
     int32_t trace_block;     /* threads per block of the trace kernel, multiple of 64 */
     int32_t shade_block;     /* threads per block of the shade kernel, multiple of 64 */
@@ -216,12 +216,13 @@ typedef struct pt_tuning {
 pt_status pt_renderer_set_tuning(pt_renderer *renderer, const pt_tuning *tuning);
 
 /* ---------------- batched rendering (no counterpart in the reference) ----------------
- * Many views of many scenes rendered by one persistent wavefront bounce
- * loop, captured once into a HIP graph and replayed: the scenes' arrays are
+ * Many views of many scenes rendered together: the scenes' arrays are
  * merged on the device, each scene keeping its own hierarchy and lights, and
- * the pixels of all views share the path queues.  Shaded mode, 4-wide
- * traversal.  Each view's image equals a single renderer's image of it
- * (same scene, camera, size and seed). */
+ * the pixels of all views share one schedule -- by default one persistent
+ * path-lane launch, or (drain_threshold below the pixel count) the wavefront
+ * bounce loop captured once into a HIP graph and replayed, its tail on path
+ * lanes.  Shaded mode, 4-wide traversal.  Each view's image equals a single
+ * renderer's image of it (same scene, camera, size and seed). */
 typedef struct pt_batch pt_batch;
 typedef struct pt_batch_view {
     uint64_t seed;           /* the view's render seed (pt_settings.seed of a single render) */

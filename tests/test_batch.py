@@ -64,6 +64,27 @@ def test_batch_tuning_and_autotuner_keep_images():
     assert batch.stats()["rays"] == rays
 
 
+@pytest.mark.gpu
+def test_batch_graph_follows_reallocated_buffers():
+    # Raising the drain threshold between calls grows the traversal spill
+    # region past what the captured bounce-loop graph was made with; the
+    # graph must be captured again (it holds buffer addresses by value).
+    a, ca = ptrace.soup(20000, 4, 128, 128)
+    b, cb = ptrace.icosphere(24, 2, 128, 128)
+    views = [(0, ca, 128, 128, 3), (1, cb, 128, 128, 4)]
+    ref = ptrace.Batch([a, b], views, direct_light=True, max_depth=6)
+    ref.set_tuning(drain_threshold=1000)
+    ref.add_samples(2)
+    ref.add_samples(2)
+    batch = ptrace.Batch([a, b], views, direct_light=True, max_depth=6)
+    batch.set_tuning(blocks_per_cu=1, trace_block=64, drain_threshold=1000)
+    batch.add_samples(2)
+    batch.set_tuning(drain_threshold=20000)
+    batch.add_samples(2)
+    for k in range(2):
+        assert np.array_equal(batch.image(k), ref.image(k)), k
+
+
 def test_batch_argument_checks():
     scenes, cams = _scenes()
     with pytest.raises(ptrace.InvalidArgument):
