@@ -1,4 +1,4 @@
-"""Benchmark of the plain wavefront render path on one MI355X.
+"""Benchmark of the plain render path on one MI355X.
 
 Workload: 1920x1080, 64 samples per pixel, path depth 8, MIS direct lighting,
 over a geodesic icosphere of 20*224^2 = 1,003,520 triangles displaced by
@@ -11,15 +11,16 @@ inputs resident in HBM.  ``value`` is rays per second over the timed steps,
 counting every closest-hit and shadow query (the reference's rays_traced).
 
 Also reported:
-  roofline     -- trace kernel (trace_lanes_kernel: closest-hit + any-hit
-                  traversal of the bounce loop on persistent lanes), HBM-bound: algorithmic bytes of its launches /
-                  HIP-event time of those launches on the renderer's stream,
-                  against 8 TB/s.  Bytes per query = 32*box tests +
-                  36*triangle tests + 16*sphere tests + 48 (closest hit) or
-                  + 36 (shadow), from an untimed counting pass over the same
-                  rays.  traffic = rocprofv3 FETCH_SIZE + WRITE_SIZE per launch
-                  (committed summary under profiles/), scaled by a plain-copy
-                  calibration.
+  roofline     -- path-lane kernel (every path of the frame, traversal and
+                  shading on per-lane state machines; a frame of more samples
+                  than the first pass runs it in two passes), HBM-bound:
+                  algorithmic bytes of the frame / HIP-event time of the
+                  path-lane work on the renderer's stream, against 8 TB/s.
+                  Bytes per query = 32*box tests + 36*triangle tests +
+                  16*sphere tests + 48 (closest hit) or + 36 (shadow), from an
+                  untimed counting pass over the same rays.  traffic =
+                  rocprofv3 FETCH_SIZE + WRITE_SIZE of that work (committed
+                  summary under profiles/), scaled by a plain-copy calibration.
   cpu_baseline -- the CPU oracle (a C restatement of the reference's host
                   transport path) on the host's cores over a 40-spp frame of
                   the same scene and camera (about 10 s on 16 threads).
@@ -43,7 +44,8 @@ def log(*a):
 
 
 def traffic_from_profiles(kernel):
-    """HBM bytes per launch of `kernel` measured by rocprofv3 --pmc (committed summary), or None."""
+    """HBM bytes per unit of `kernel` (one launch, or both passes of a split
+    frame) measured by rocprofv3 --pmc (committed summary), or None."""
     path = os.path.join(HERE, "profiles", "path_kernel_icosphere1M_1080p_traffic.json")
     if not os.path.exists(path):
         return None
@@ -193,7 +195,7 @@ def main():
         r.add_samples(args.spp)
     rays = 0
     trace_ms = shade_ms = regen_ms = drain_ms = 0.0
-    launches = 0
+    launches = splits = 0
     t0 = time.perf_counter()
     for _ in range(args.steps):
         r.reset(camera)
@@ -205,17 +207,27 @@ def main():
         regen_ms += st["regen_ms"]
         drain_ms += st["drain_ms"]
         launches += st["trace_launches"]
+        splits += st["split_launches"]
     elapsed = time.perf_counter() - t0
     ms_per_step = elapsed / args.steps * 1e3
     mrays = rays / elapsed / 1e6
     log(f"{args.steps} steps: {elapsed:.3f}s, {mrays:.1f} Mrays/s, trace {trace_ms:.1f} ms, shade {shade_ms:.1f} ms")
 
-    # Dominant kernel: the path-lane kernel (one launch per frame, every
-    # query of the frame) by default, else the wavefront trace kernel.
+    # Dominant kernel: the path-lane kernel (every query of the frame) by
+    # default, else the wavefront trace kernel.  A split frame runs it twice
+    # (first samples of every pixel, then the rest in cost order): the unit
+    # is then the frame's path-lane work, timed from the first pass's start to
+    # the second's end (HIP events on the renderer's stream; the 8-bit key
+    # sort between them included).
     alg_bytes_frame = (32 * counts["box_tests"] + 36 * counts["triangle_tests"] + 16 * counts["sphere_tests"]
                        + 48 * traced_ext + 36 * traced_sh)
+    passes = 1
     if launches == 0:
-        kernel, bytes_per_launch, launch_s = "path_lanes_kernel<false>", alg_bytes_frame, drain_ms / 1e3 / args.steps
+        if splits:
+            kernel, passes = "path_lanes_kernel<false, 1> + path_lanes_kernel<false, 2>", 2
+        else:
+            kernel = "path_lanes_kernel<false, 0>"
+        bytes_per_launch, launch_s = alg_bytes_frame, drain_ms / 1e3 / args.steps
     else:
         kernel, bytes_per_launch, launch_s = ("trace_lanes_kernel<false, false>", alg_bytes_frame / (launches / args.steps),
                                               trace_ms / 1e3 / launches)
@@ -223,7 +235,7 @@ def main():
     traffic = traffic_from_profiles(kernel)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": kernel, "avg_launch_ms": round(launch_s * 1e3, 4),
+                "kernel": kernel, "launches_per_unit": passes, "avg_launch_ms": round(launch_s * 1e3, 4),
                 "alg_bytes_per_launch": int(bytes_per_launch),
                 "box_tests_per_query": round(counts["box_tests"] / (traced_ext + traced_sh), 2),
                 "triangle_tests_per_query": round(counts["triangle_tests"] / (traced_ext + traced_sh), 2)}
@@ -246,7 +258,8 @@ def main():
                        "shade_ms_per_step": round(shade_ms / args.steps, 2),
                        "regen_ms_per_step": round(regen_ms / args.steps, 2),
                        "drain_ms_per_step": round(drain_ms / args.steps, 2),
-                       "bounce_iterations": counts["iterations"], "drained_paths": counts["drained_paths"]},
+                       "bounce_iterations": counts["iterations"], "drained_paths": counts["drained_paths"],
+                       "split_launches_per_step": splits / args.steps},
             "roofline": roofline, "cpu_baseline": cpu}
     if cpu:
         line["gpu_over_cpu"] = round(mrays / cpu["value"], 1)

@@ -16,6 +16,7 @@
 #include <thread>
 #include <vector>
 
+#include <hipcub/hipcub.hpp>
 #include "pt_device.h"
 
 namespace {
@@ -595,6 +596,9 @@ struct partition {
     dev_buf<uint32_t> log;
     dev_buf<int4> save_head[2], save_best[2];   // queries handed on between trace launches
     dev_buf<int2> save_stack[2];
+    // split path-lane launches: pixel costs, group keys and indices, sort space
+    dev_buf<uint32_t> costs, group_ids, sorted_ids;
+    dev_buf<uint8_t> group_keys, sorted_keys, sort_temp;
     pt::paths_dev paths{};
     std::vector<hipEvent_t> events;      // 5 per iteration of a check interval
     hipEvent_t checked = nullptr, drain0 = nullptr, drain1 = nullptr, finished = nullptr;
@@ -618,6 +622,49 @@ struct partition {
         }
         paths.save_capacity = int32_t(capacity);
         paths.save_stride = stride;
+    }
+    // Every path of the partition on path lanes, right after launch_start
+    // (each pixel at the start of its first sample, queue0 in pixel order).
+    // With more than `first` samples per pixel, more pixels than lanes in
+    // flight (so that lanes take pixels from a queue) and the 4-wide
+    // traversal, in two passes: `first` samples of every pixel, then the rest
+    // with the queue reordered by the first samples' cost, most expensive
+    // first, in groups of `group` consecutive pixels, so that no expensive
+    // pixel starts near the end of the launch.  Returns whether it split.
+    bool paths_from_start(const pt::scene_dev& view, const pt::frame_params& frame, int32_t samples, int32_t first,
+                          int32_t group, pt::path_lanes lanes, int block, int cus, hipStream_t s,
+                          const pt::batch_dev *batch = nullptr) {
+        const uint32_t n = uint32_t(count);
+        const uint64_t grid = (uint64_t(n) + uint64_t(block) - 1) / uint64_t(block);
+        const uint64_t launched = (lanes.blocks > 0 ? std::min<uint64_t>(grid, uint64_t(lanes.blocks)) : grid) * uint64_t(block);
+        const uint64_t in_flight = std::min<uint64_t>(launched, uint64_t(cus) * uint64_t(pt::path_lanes_per_cu));
+        if(first <= 0 || samples <= first || uint64_t(n) <= in_flight || !pt::wide_traversal(view, frame)) {
+            pt::launch_drain(view, frame, paths, ctl.get(), 0, 0, 0, n, block, lanes, s, batch);
+            return false;
+        }
+        group = std::max(1, group);
+        const uint32_t groups = (n + uint32_t(group) - 1) / uint32_t(group);
+        if(costs.size() < n) costs.alloc(n);
+        if(group_ids.size() < groups) {
+            group_ids.alloc(groups); sorted_ids.alloc(groups); group_keys.alloc(groups); sorted_keys.alloc(groups);
+        }
+        size_t temp = 0;
+        HIP_CHECK(hipcub::DeviceRadixSort::SortPairsDescending(nullptr, temp, group_keys.get(), sorted_keys.get(),
+                                                                group_ids.get(), sorted_ids.get(), int(groups), 0, 8, s));
+        if(sort_temp.size() < temp) sort_temp.alloc(temp);
+        pt::path_pass pass;
+        pass.mode = pt::PATHS_FIRST;
+        pass.first_samples = first;
+        pass.costs = costs.get();
+        pt::launch_path_lanes(view, frame, paths, ctl.get(), 0, queue0.get(), n, block, lanes, pass, s, batch);
+        pt::launch_zero_work(ctl.get(), s);
+        pt::launch_group_keys(costs.get(), n, group, group_keys.get(), group_ids.get(), s);
+        HIP_CHECK(hipcub::DeviceRadixSort::SortPairsDescending(sort_temp.get(), temp, group_keys.get(), sorted_keys.get(),
+                                                                group_ids.get(), sorted_ids.get(), int(groups), 0, 8, s));
+        pt::launch_group_gather(queue0.get(), sorted_ids.get(), n, group, queue1.get(), s);
+        pt::launch_path_lanes(view, frame, paths, ctl.get(), 0, queue1.get(), n, block, lanes, pt::path_pass{pt::PATHS_REST},
+                              s, batch);
+        return true;
     }
     void read_ctl_async() { read_ctl_async(stream); }
     void read_ctl_async(hipStream_t s) {
@@ -653,7 +700,7 @@ struct pt_renderer {
     dev_buf<int4> hit;
     std::vector<std::unique_ptr<partition>> parts;
     dev_buf<float> aov;                  // first-hit albedo and normal sums (6 per pixel), when enabled
-    pt_tuning tuning{128, 256, 16, 8, 0, 0, 0, all_paths, 48, 1, 16, 24, 12};
+    pt_tuning tuning{128, 256, 16, 8, 0, 0, 0, all_paths, 48, 1, 16, 24, 12, 4, 16};
     int cus = 1;
     pt_launch_stats last{};
 
@@ -799,8 +846,9 @@ struct pt_renderer {
             if(!part->done && drain_threshold && part->count <= drain_threshold) {
                 // Few enough paths for the tail kernel from the start: no wavefront bounces.
                 if(tuning.time_kernels) HIP_CHECK(hipEventRecord(part->drain0, part->stream));
-                pt::launch_drain(scene->view, frame, part->paths, part->ctl.get(), 0, 0, 0, uint32_t(part->count),
-                                 drain_block, path_lanes(), part->stream);
+                if(part->paths_from_start(scene->view, frame, count, tuning.path_first_samples, tuning.path_sort_group,
+                                          path_lanes(), drain_block, cus, part->stream))
+                    st.split_launches++;
                 if(tuning.time_kernels) HIP_CHECK(hipEventRecord(part->drain1, part->stream));
                 st.drained_paths += uint64_t(part->count);
                 part->read_ctl_async();
@@ -1056,7 +1104,7 @@ struct pt_batch {
     partition part;
     hipStream_t stream = nullptr;
     hipEvent_t start = nullptr, stop = nullptr;
-    pt_tuning tuning{128, 256, 16, 8, 0, 0, 0, all_paths, 48, 1, 16, 24, 12};
+    pt_tuning tuning{128, 256, 16, 8, 0, 0, 0, all_paths, 48, 1, 16, 24, 12, 4, 16};
     int cus = 1;
     int32_t samples = 0;
     uint64_t rays = 0;
@@ -1152,8 +1200,9 @@ struct pt_batch {
         bool drained = false;
         if(threshold && int64_t(pixels) <= threshold) {
             // Few enough paths for the tail kernel from the start: no wavefront bounces.
-            pt::launch_drain(scene->view, frame, part.paths, part.ctl.get(), 0, 0, 0, uint32_t(pixels), drain_block,
-                             path_lanes(), stream, &bd);
+            if(part.paths_from_start(scene->view, frame, count, tuning.path_first_samples, tuning.path_sort_group,
+                                     path_lanes(), drain_block, cus, stream, &bd))
+                st.split_launches++;
             HIP_CHECK(hipGetLastError());
             st.drained_paths = pixels;
             drained = true;
@@ -1617,6 +1666,14 @@ pt_status pt_renderer_set_tuning(pt_renderer *renderer, const pt_tuning *tuning)
             if(tuning->path_blocks_per_cu > 64) fail(PT_EINVAL, "path blocks per CU must be 1..64 or -1");
             t.path_blocks_per_cu = tuning->path_blocks_per_cu < 0 ? -1 : tuning->path_blocks_per_cu;
         }
+        if(tuning->path_first_samples) {
+            if(tuning->path_first_samples > 65536) fail(PT_EINVAL, "path first samples must be 1..65536 or -1");
+            t.path_first_samples = tuning->path_first_samples < 0 ? -1 : tuning->path_first_samples;
+        }
+        if(tuning->path_sort_group) {
+            if(tuning->path_sort_group < 1 || tuning->path_sort_group > 4096) fail(PT_EINVAL, "path sort group must be 1..4096");
+            t.path_sort_group = tuning->path_sort_group;
+        }
         renderer->tuning = t;
     });
 }
@@ -1709,6 +1766,14 @@ pt_status pt_batch_set_tuning(pt_batch *batch, const pt_tuning *tuning) {
         if(tuning->path_blocks_per_cu) {
             if(tuning->path_blocks_per_cu > 64) fail(PT_EINVAL, "path blocks per CU must be 1..64 or -1");
             t.path_blocks_per_cu = tuning->path_blocks_per_cu < 0 ? -1 : tuning->path_blocks_per_cu;
+        }
+        if(tuning->path_first_samples) {
+            if(tuning->path_first_samples > 65536) fail(PT_EINVAL, "path first samples must be 1..65536 or -1");
+            t.path_first_samples = tuning->path_first_samples < 0 ? -1 : tuning->path_first_samples;
+        }
+        if(tuning->path_sort_group) {
+            if(tuning->path_sort_group < 1 || tuning->path_sort_group > 4096) fail(PT_EINVAL, "path sort group must be 1..4096");
+            t.path_sort_group = tuning->path_sort_group;
         }
         batch->tuning = t;
     });

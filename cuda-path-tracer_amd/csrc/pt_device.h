@@ -185,6 +185,36 @@ struct path_lanes { int shade_batch; int blocks; };
 void launch_drain(const scene_dev& s, const frame_params& f, const paths_dev& p, loop_ctl *ctl, int parity,
                   uint32_t shadows, uint32_t saved, uint32_t paths, int block, path_lanes lanes, hipStream_t stream,
                   const batch_dev *batch = nullptr);
+// A path-lane launch over every pixel from a sample boundary may be split in
+// two passes: PATHS_FIRST runs at most `first_samples` samples of each pixel
+// and writes the pixel's cost (loop passes its lane spent on it; 0 when the
+// pixel is done) at its queue position; PATHS_REST starts each queued pixel
+// at its next sample.  PATHS_WHOLE takes queued paths where they stand and
+// runs them to the end.
+enum path_mode : int { PATHS_WHOLE = 0, PATHS_FIRST = 1, PATHS_REST = 2 };
+struct path_pass {
+    path_mode mode = PATHS_WHOLE;
+    int first_samples = 0;
+    uint32_t *costs = nullptr;
+};
+// Between the passes: groups of `group` consecutive queue positions are
+// ordered by their summed cost, most expensive first, each group keeping its
+// pixels' order.  launch_group_keys writes an 8-bit key and the index of
+// every group (a last, partial group gets key 0, so that it stays last);
+// after the (key, index) pairs are sorted in descending key order,
+// launch_group_gather writes the reordered queue.
+void launch_group_keys(const uint32_t *costs, uint32_t n, int group, uint8_t *keys, uint32_t *ids, hipStream_t stream);
+void launch_group_gather(const int32_t *queue, const uint32_t *ids, uint32_t n, int group, int32_t *out,
+                         hipStream_t stream);
+// Lanes of the path-lane kernel resident per compute unit: 4 waves of 64 on
+// each of the 4 SIMDs (the kernel is compiled for 4 waves per SIMD).
+constexpr int path_lanes_per_cu = 1024;
+// Path lanes over queue[0 .. ctl->n_ext[parity]) (`paths`: the host's copy of
+// that count, for the grid).  Requires the 4-wide traversal (wide_traversal).
+void launch_path_lanes(const scene_dev& s, const frame_params& f, const paths_dev& p, loop_ctl *ctl, int parity,
+                       const int32_t *queue, uint32_t paths, int block, path_lanes lanes, path_pass pass,
+                       hipStream_t stream, const batch_dev *batch = nullptr);
+void launch_zero_work(loop_ctl *ctl, hipStream_t stream);
 void launch_pack(const float *sum, uint8_t *rgba, int64_t count, int32_t samples, hipStream_t stream);
 void launch_add_rays(loop_ctl *ctl, unsigned long long rays, hipStream_t stream);
 
@@ -196,10 +226,14 @@ inline size_t wide_stack_bytes(int capacity, int threads) {
 }
 // Global overflow entries per lane of the wide traversal.
 inline int wide_spill_entries(int capacity) { return std::max(0, capacity - wide_lds_entries); }
+// Whether the shaded path runs on the 4-wide traversal; LDS is sized for it
+// only then, so the wide kernels (path lanes) must not run otherwise.
+inline bool wide_traversal(const scene_dev& s, const frame_params& f) {
+    return !f.brute && s.wroot_ref != empty_root && !f.exact_traversal;
+}
 inline size_t trace_lds_bytes(const scene_dev& s, const frame_params& f, int threads) {
     size_t exact = stack_bytes(s.stack_capacity, threads);
-    bool wide = !f.brute && s.wroot_ref != empty_root && !f.exact_traversal;
-    return wide ? std::max(exact, wide_stack_bytes(s.wide_stack_capacity, threads)) : exact;
+    return wide_traversal(s, f) ? std::max(exact, wide_stack_bytes(s.wide_stack_capacity, threads)) : exact;
 }
 
 void launch_batch_init(const batch_dev& b, float *sum, uint64_t *rng, int64_t count, loop_ctl *ctl, hipStream_t stream);

@@ -166,6 +166,7 @@ typedef struct pt_launch_stats {
     uint64_t sphere_tests;
     uint64_t drained_paths;  /* paths finished by the tail kernel */
     float drain_ms;
+    int32_t split_launches;  /* path-lane launches run in two passes (pt_tuning.path_first_samples) */
 } pt_launch_stats;
 pt_status pt_renderer_last_launch_stats(const pt_renderer *renderer, pt_launch_stats *stats);
 
@@ -188,7 +189,7 @@ pt_status pt_renderer_enable_aovs(pt_renderer *renderer, int32_t enable);
 pt_status pt_renderer_read_aovs(const pt_renderer *renderer, float *albedo, float *normal, size_t count);
 
 /* Per-renderer tuning of the wavefront loop (0 keeps the current value).
- * Defaults: 128, 256, 16, 8, 0, 0, 0, 1073741824 (every path on path lanes), 48, 1, 16, 24, 12. */
+ * Defaults: 128, 256, 16, 8, 0, 0, 0, 1073741824 (every path on path lanes), 48, 1, 16, 24, 12, 4, 16. */
 typedef struct pt_tuning {
     int32_t trace_block;     /* threads per block of the trace kernel, multiple of 64 */
     int32_t shade_block;     /* threads per block of the shade kernel, multiple of 64 */
@@ -212,6 +213,12 @@ typedef struct pt_tuning {
                                   before the wave shades them together (1..64) */
     int32_t path_blocks_per_cu; /* path lanes: resident 256-thread blocks per compute unit, each lane
                                   taking pixel after pixel (-1: one lane per remaining pixel) */
+    int32_t path_first_samples; /* path lanes over every pixel of a call with more samples than this:
+                                  a first pass of this many samples per pixel, then the rest of the
+                                  samples with the pixels ordered by the cost of their first ones,
+                                  most expensive first (1..65536; -1: one pass) */
+    int32_t path_sort_group;    /* that ordering moves groups of this many consecutive pixels by their
+                                  summed cost, each group keeping its pixels' order (1..4096) */
 } pt_tuning;
 pt_status pt_renderer_set_tuning(pt_renderer *renderer, const pt_tuning *tuning);
 
@@ -245,7 +252,8 @@ pt_status pt_batch_stats(const pt_batch *batch, int32_t *samples, float *millise
 pt_status pt_batch_last_launch_stats(const pt_batch *batch, pt_launch_stats *stats);
 /* Fields used: trace_block, shade_block, blocks_per_cu, check_interval (bounce
  * iterations per graph replay, rounded down to even), drain_threshold (live
- * paths at which the tail kernel takes over; -1 never), lane_refill, handoff_steps. */
+ * paths at which the tail kernel takes over; -1 never), lane_refill, handoff_steps,
+ * path_shade_batch, path_blocks_per_cu, path_first_samples, path_sort_group. */
 pt_status pt_batch_get_tuning(const pt_batch *batch, pt_tuning *tuning);
 pt_status pt_batch_set_tuning(pt_batch *batch, const pt_tuning *tuning);
 /* Autotuner: coordinate descent over workgroup sizes, persistent blocks per

@@ -37,6 +37,31 @@ def test_batch_views_match_single_renderers(depth, direct):
 
 
 @pytest.mark.gpu
+def test_batch_split_path_lanes_match_single_renderers():
+    # 68544 pixels on 65536 lanes (one block per compute unit): the batch's
+    # path-lane launch of 6 samples runs in two passes (4 samples of every
+    # pixel, then the rest with groups of 50 pixels reordered by cost, the
+    # last group partial); the single renderers run one pass.  Images and ray
+    # counts must be equal.
+    scenes, cams = _scenes()
+    views = [(0, cams[0], 200, 150, 7), (2, cams[2], 180, 180, 3), (1, cams[1], 96, 64, 9)]
+    batch = ptrace.Batch(scenes[:3], views, direct_light=True, max_depth=8)
+    batch.set_tuning(path_blocks_per_cu=1, path_sort_group=50)
+    batch.add_samples(3)
+    batch.add_samples(6)
+    assert batch.last_launch_stats()["split_launches"] == 1
+    total = 0
+    for k, (si, cam, w, h, seed) in enumerate(views):
+        r = ptrace.Renderer(scenes[si], cam, w, h, seed, False, ptrace.DEBUG_SHADED, True, 8)
+        r.set_tuning(path_first_samples=-1)
+        r.add_samples(3)
+        r.add_samples(6)
+        assert np.array_equal(batch.image(k), r.image()), k
+        total += r.rays_traced()
+    assert batch.stats()["rays"] == total
+
+
+@pytest.mark.gpu
 def test_batch_tuning_and_autotuner_keep_images():
     scenes, cams = _scenes()
     views = [(0, cams[0], 40, 30, 7), (1, cams[1], 48, 32, 9), (2, cams[2], 36, 36, 3)]

@@ -368,28 +368,39 @@ def test_handoff_leaves_results_unchanged(steps, drain):
     assert int(ext.sum()) + int(sh.sum()) <= st["extend_rays"] + st["shadow_rays"]
 
 
-@pytest.mark.parametrize("scene_name,blocks,batch", [("icosphere", -1, 16), ("icosphere", 1, 1), ("cornell", 2, 64),
-                                                      ("cornell", -1, 16)])
-def test_path_lanes_match_wavefront(scene_name, blocks, batch):
+@pytest.mark.parametrize("scene_name,blocks,batch,first,group,split", [("icosphere", 1, 1, 2, 64, True),
+                                                                        ("icosphere", -1, 16, 4, 64, False),
+                                                                        ("cornell", 1, 64, 1, 1, True),
+                                                                        ("cornell", 2, 16, -1, 64, False),
+                                                                        ("icosphere", 1, 24, 4, 7, True),
+                                                                        ("cornell", 1, 24, 4, 4096, True)])
+def test_path_lanes_match_wavefront(scene_name, blocks, batch, first, group, split):
     # The path-lane kernel (every remaining pixel on a lane's state machine,
     # shading batched per wave, lanes taking pixel after pixel) must equal the
-    # wavefront bounce loop exactly: pixels, PCG states and ray counts.
+    # wavefront bounce loop exactly: pixels, PCG states and ray counts.  With
+    # more samples in a call than `first` and more pixels than lanes in flight
+    # (76800 pixels; one 256-lane block per compute unit), the launch runs in
+    # two passes: `first` samples of every pixel, then the rest with groups of
+    # `group` pixels reordered by cost (7 and 4096 leave a partial last
+    # group).  With one lane per pixel or 2 blocks per compute unit every
+    # pixel has a lane from the start: one pass.
     if scene_name == "icosphere":
-        scene, cam = ptrace.icosphere(40, 3, 256, 192)
-        w, h = 256, 192
+        scene, cam = ptrace.icosphere(40, 3, 320, 240)
     else:
-        scene, cam = ptrace.cornell(7, 160, 100)
-        w, h = 160, 100
+        scene, cam = ptrace.cornell(7, 320, 240)
+    w, h = 320, 240
     ref = ptrace.Renderer(scene, cam, w, h, 23, False, 0, True, 8)
     ref.set_tuning(drain_threshold=-1)
     ref.add_samples(3)
-    ref.add_samples(2)
+    ref.add_samples(6)
     r = ptrace.Renderer(scene, cam, w, h, 23, False, 0, True, 8)
-    r.set_tuning(drain_threshold=1 << 30, path_blocks_per_cu=blocks, path_shade_batch=batch)
+    r.set_tuning(drain_threshold=1 << 30, path_blocks_per_cu=blocks, path_shade_batch=batch, path_first_samples=first,
+                 path_sort_group=group)
     r.add_samples(3)
-    r.add_samples(2)
+    r.add_samples(6)
     st = r.last_launch_stats()
     assert st["trace_launches"] == 0 and st["drained_paths"] == w * h
+    assert st["split_launches"] == (1 if split else 0)
     (a, ga), (b, gb) = ref.read_state(), r.read_state()
     assert np.array_equal(a, b) and np.array_equal(ga, gb)
     assert ref.rays_traced() == r.rays_traced()
