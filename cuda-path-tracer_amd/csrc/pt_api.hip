@@ -700,7 +700,7 @@ struct pt_renderer {
     dev_buf<int4> hit;
     std::vector<std::unique_ptr<partition>> parts;
     dev_buf<float> aov;                  // first-hit albedo and normal sums (6 per pixel), when enabled
-    pt_tuning tuning{128, 256, 16, 8, 0, 0, 0, all_paths, 48, 1, 16, 24, 12, 4, 16};
+    pt_tuning tuning{128, 256, 16, 8, 0, 0, 0, all_paths, 48, 1, 16, 24, 12, 4, 16, 8};
     int cus = 1;
     pt_launch_stats last{};
 
@@ -727,7 +727,8 @@ struct pt_renderer {
     // stacks fit the 64 KiB of LDS a workgroup may hold; the wide traversal is
     // switched off if even 64 threads do not fit.
     pt::path_lanes path_lanes() const {
-        return pt::path_lanes{tuning.path_shade_batch, tuning.path_blocks_per_cu > 0 ? cus * tuning.path_blocks_per_cu : 0};
+        return pt::path_lanes{tuning.path_shade_batch, tuning.path_blocks_per_cu > 0 ? cus * tuning.path_blocks_per_cu : 0,
+                              tuning.path_trace_steps};
     }
     int trace_block() {
         frame.exact_traversal = tuning.exact_traversal;
@@ -1104,7 +1105,7 @@ struct pt_batch {
     partition part;
     hipStream_t stream = nullptr;
     hipEvent_t start = nullptr, stop = nullptr;
-    pt_tuning tuning{128, 256, 16, 8, 0, 0, 0, all_paths, 48, 1, 16, 24, 12, 4, 16};
+    pt_tuning tuning{128, 256, 16, 8, 0, 0, 0, all_paths, 48, 1, 16, 24, 12, 4, 16, 8};
     int cus = 1;
     int32_t samples = 0;
     uint64_t rays = 0;
@@ -1124,7 +1125,8 @@ struct pt_batch {
     }
 
     pt::path_lanes path_lanes() const {
-        return pt::path_lanes{tuning.path_shade_batch, tuning.path_blocks_per_cu > 0 ? cus * tuning.path_blocks_per_cu : 0};
+        return pt::path_lanes{tuning.path_shade_batch, tuning.path_blocks_per_cu > 0 ? cus * tuning.path_blocks_per_cu : 0,
+                              tuning.path_trace_steps};
     }
     int trace_block() {
         int block = tuning.trace_block;
@@ -1674,6 +1676,10 @@ pt_status pt_renderer_set_tuning(pt_renderer *renderer, const pt_tuning *tuning)
             if(tuning->path_sort_group < 1 || tuning->path_sort_group > 4096) fail(PT_EINVAL, "path sort group must be 1..4096");
             t.path_sort_group = tuning->path_sort_group;
         }
+        if(tuning->path_trace_steps) {
+            if(tuning->path_trace_steps < 1 || tuning->path_trace_steps > 64) fail(PT_EINVAL, "path trace steps must be 1..64");
+            t.path_trace_steps = tuning->path_trace_steps;
+        }
         renderer->tuning = t;
     });
 }
@@ -1775,6 +1781,10 @@ pt_status pt_batch_set_tuning(pt_batch *batch, const pt_tuning *tuning) {
             if(tuning->path_sort_group < 1 || tuning->path_sort_group > 4096) fail(PT_EINVAL, "path sort group must be 1..4096");
             t.path_sort_group = tuning->path_sort_group;
         }
+        if(tuning->path_trace_steps) {
+            if(tuning->path_trace_steps < 1 || tuning->path_trace_steps > 64) fail(PT_EINVAL, "path trace steps must be 1..64");
+            t.path_trace_steps = tuning->path_trace_steps;
+        }
         batch->tuning = t;
     });
 }
@@ -1793,6 +1803,7 @@ pt_status pt_batch_autotune(pt_batch *batch, int32_t probe_samples, int32_t pass
             {&pt_tuning::shade_block, {256, 512, 1024}},
             {&pt_tuning::drain_threshold, {std::max(1024, px / 32), std::max(2048, px / 16), std::max(4096, px / 8), std::max(8192, px / 4), all_paths}},
             {&pt_tuning::path_shade_batch, {16, 24, 32}},
+            {&pt_tuning::path_trace_steps, {1, 2, 4, 8}},
             {&pt_tuning::check_interval, {2, 4, 8, 16}},
         };
         int32_t count = 0;

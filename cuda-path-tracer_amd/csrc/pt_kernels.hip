@@ -280,7 +280,7 @@ struct wide_stack {
     int stride;
     int2 *spill;
     size_t spill_stride;
-    size_t lane_id;
+    uint32_t lane_id;   // grid-wide thread index (grids stay below 2^32 threads)
     __device__ __forceinline__ void store(int k, int32_t ref, float entry) {
         int2 v = make_int2(ref, __float_as_int(entry));
         if(k < wide_lds_entries) lds[k * stride] = v;
@@ -931,7 +931,7 @@ __global__ __launch_bounds__(256) void trace_kernel(scene_dev s, frame_params f,
     const bool brute = f.brute != 0;
     const bool wide = !brute && s.wroot_ref != empty_root && !f.exact_traversal;
     wide_stack wstk{reinterpret_cast<int2 *>(lds) + threadIdx.x, stride, p.spill,
-                    size_t(gridDim.x) * blockDim.x, size_t(blockIdx.x) * blockDim.x + threadIdx.x};
+                    size_t(gridDim.x) * blockDim.x, blockIdx.x * blockDim.x + threadIdx.x};
     int error = 0;
     unsigned long long n_box = 0, n_leaf = 0, n_sph = 0;
     // Each wave claims trace_chunk consecutive items per atomic and works
@@ -1017,7 +1017,7 @@ __global__ __launch_bounds__(256) void trace_lanes_kernel(scene_dev s, frame_par
     const saved_queries in = p.saved[parity], out = p.saved[parity ^ 1];
     const int lane = __lane_id();
     wide_stack wstk{reinterpret_cast<int2 *>(lds) + threadIdx.x, int(blockDim.x), p.spill,
-                    size_t(gridDim.x) * blockDim.x, size_t(blockIdx.x) * blockDim.x + threadIdx.x};
+                    size_t(gridDim.x) * blockDim.x, blockIdx.x * blockDim.x + threadIdx.x};
     int error = 0;
     unsigned long long n_box = 0, n_leaf = 0, n_sph = 0;
     int boxes = 0, prims = 0, spheres = 0;
@@ -1310,7 +1310,7 @@ __global__ __launch_bounds__(256) void drain_shadow_kernel(scene_dev s, frame_pa
     const bool wide = !brute && s.wroot_ref != empty_root && !f.exact_traversal;
     const int stride = int(blockDim.x);
     wide_stack wstk{reinterpret_cast<int2 *>(lds) + threadIdx.x, stride, p.spill,
-                    size_t(gridDim.x) * blockDim.x, size_t(blockIdx.x) * blockDim.x + threadIdx.x};
+                    size_t(gridDim.x) * blockDim.x, blockIdx.x * blockDim.x + threadIdx.x};
     int32_t *stk = lds + threadIdx.x;
     int error = 0;
     const uint32_t step = gridDim.x * blockDim.x;
@@ -1368,7 +1368,7 @@ __global__ __launch_bounds__(256) void drain_kernel(scene_dev s, frame_params f,
     const bool wide = !brute && s.wroot_ref != empty_root && !f.exact_traversal;
     const int stride = int(blockDim.x);
     wide_stack wstk{reinterpret_cast<int2 *>(lds) + threadIdx.x, stride, p.spill,
-                    size_t(gridDim.x) * blockDim.x, size_t(blockIdx.x) * blockDim.x + threadIdx.x};
+                    size_t(gridDim.x) * blockDim.x, blockIdx.x * blockDim.x + threadIdx.x};
     int32_t *stk = lds + threadIdx.x;
     int error = 0;
     unsigned long long ext = 0, shadow = 0;
@@ -1429,7 +1429,9 @@ __global__ __launch_bounds__(256) void drain_kernel(scene_dev s, frame_params f,
 // unblocked), then the next bounce or sample.
 // 4 waves per SIMD (at most 128 VGPRs, a few spilled): 2150 against 1868
 // Mrays/s at the compiler's own 140 VGPRs and 3 waves; 5 waves spill 106
-// VGPRs and fall to 1613.
+// VGPRs and fall to 1613.  A tracing lane takes up to `trace_steps` steps
+// before the wave looks again for lanes to shade and idle lanes to refill
+// (1080p, 64 spp: 1, 4 and 8 steps give 2440, 2554 and 2563 Mrays/s).
 //
 // Longest first (MODE).  A lane that takes an expensive pixel near the end of
 // the queue keeps the GPU waiting for all of that pixel's samples, since a
@@ -1447,14 +1449,21 @@ __global__ __launch_bounds__(256) void drain_kernel(scene_dev s, frame_params f,
 template<bool BATCH, int MODE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void path_lanes_kernel(scene_dev s, frame_params f, paths_dev p, loop_ctl *ctl,
                                                           int parity, const int32_t *queue, int shade_batch,
-                                                          int first_samples, uint32_t *costs, batch_dev bd) {
+                                                          int trace_steps, int first_samples, uint32_t *costs,
+                                                          batch_dev bd) {
     extern __shared__ int32_t lds[];
     const uint32_t n = ctl->n_ext[parity];
     const int lane = __lane_id();
     wide_stack wstk{reinterpret_cast<int2 *>(lds) + threadIdx.x, int(blockDim.x), p.spill,
-                    size_t(gridDim.x) * blockDim.x, size_t(blockIdx.x) * blockDim.x + threadIdx.x};
+                    size_t(gridDim.x) * blockDim.x, blockIdx.x * blockDim.x + threadIdx.x};
     int error = 0, boxes = 0, prims = 0, spheres = 0;
-    unsigned long long ext = 0, shadow = 0;
+    uint32_t ext = 0, shadow = 0;   // queries this lane began, flushed to the totals before 2^31
+    auto flush_rays = [&]() {
+        atomicAdd(&ctl->rays, (unsigned long long)ext + shadow);
+        atomicAdd(&ctl->ext_rays, (unsigned long long)ext);
+        atomicAdd(&ctl->shadow_rays, (unsigned long long)shadow);
+        ext = shadow = 0;
+    };
     enum : int { IDLE = 0, TRACE = 1, SHADE = 2 };
     int phase = IDLE;
     bool exhausted = false, cont_after = false;
@@ -1470,7 +1479,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void p
     auto begin_ext = [&]() {
         q.r = setup_ray(st.o.x, st.o.y, st.o.z, st.d.x, st.d.y, st.d.z);
         wide_begin(root, q, false, FLT_MAX);
-        ext++;
+        if(++ext >= (1u << 30)) flush_rays();
         phase = TRACE;
     };
     // After a bounce's queries: the next bounce, the next sample, or done.
@@ -1534,7 +1543,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void p
                 const bool cont = shade_bounce(s, f, pixel_lights<BATCH>(s, bd, pix), st, q.found, q.best, r, sq, has_shadow,
                                                p.aov ? p.aov + 6 * size_t(pix) : nullptr);
                 if(has_shadow) {
-                    shadow++;
+                    if(++shadow >= (1u << 30)) flush_rays();
                     contribution = sq.contribution;
                     cont_after = cont;
                     q.r = setup_ray(sq.o.x, sq.o.y, sq.o.z, sq.d.x, sq.d.y, sq.d.z);
@@ -1544,21 +1553,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void p
                     next_bounce(cont);
                 }
             }
-        } else if(phase == TRACE && wide_step<false>(s, q, wstk, boxes, prims, spheres, error)) {
-            if(!q.any) {
-                phase = SHADE;
-            } else {
-                if(!q.found) st.radiance = add(st.radiance, contribution);
-                next_bounce(cont_after);
+        } else {
+            // Up to trace_steps traversal steps per pass; a lane whose shadow
+            // query ends and whose next extension query begins keeps going.
+            for(int k = 0; k < trace_steps && phase == TRACE; k++) {
+                if(!wide_step<false>(s, q, wstk, boxes, prims, spheres, error)) continue;
+                if(!q.any) {
+                    phase = SHADE;
+                } else {
+                    if(!q.found) st.radiance = add(st.radiance, contribution);
+                    next_bounce(cont_after);
+                }
             }
         }
     }
     if(error) atomicExch(&ctl->failure, 1);
-    ext = wave_sum(ext); shadow = wave_sum(shadow);
-    if(lane == 0 && (ext || shadow)) {
-        atomicAdd(&ctl->rays, ext + shadow);
-        atomicAdd(&ctl->ext_rays, ext);
-        atomicAdd(&ctl->shadow_rays, shadow);
+    const unsigned long long ext_total = wave_sum((unsigned long long)ext), shadow_total = wave_sum((unsigned long long)shadow);
+    if(lane == 0 && (ext_total || shadow_total)) {
+        atomicAdd(&ctl->rays, ext_total + shadow_total);
+        atomicAdd(&ctl->ext_rays, ext_total);
+        atomicAdd(&ctl->shadow_rays, shadow_total);
     }
 }
 
@@ -1727,14 +1741,14 @@ void launch_drain(const scene_dev& s, const frame_params& f, const paths_dev& p,
 }
 template<int MODE>
 static void path_lanes_mode(const scene_dev& s, const frame_params& f, const paths_dev& p, loop_ctl *ctl, int parity,
-                            const int32_t *queue, int shade_batch, const path_pass& pass, dim3 grid, dim3 block,
-                            size_t lds, hipStream_t stream, const batch_dev *batch) {
+                            const int32_t *queue, int shade_batch, int trace_steps, const path_pass& pass, dim3 grid,
+                            dim3 block, size_t lds, hipStream_t stream, const batch_dev *batch) {
     if(batch)
         hipLaunchKernelGGL((path_lanes_kernel<true, MODE>), grid, block, lds, stream, s, f, p, ctl, parity, queue,
-                           shade_batch, pass.first_samples, pass.costs, *batch);
+                           shade_batch, trace_steps, pass.first_samples, pass.costs, *batch);
     else
         hipLaunchKernelGGL((path_lanes_kernel<false, MODE>), grid, block, lds, stream, s, f, p, ctl, parity, queue,
-                           shade_batch, pass.first_samples, pass.costs, batch_dev{});
+                           shade_batch, trace_steps, pass.first_samples, pass.costs, batch_dev{});
 }
 void launch_path_lanes(const scene_dev& s, const frame_params& f, const paths_dev& p, loop_ctl *ctl, int parity,
                        const int32_t *queue, uint32_t paths, int block, path_lanes lanes, path_pass pass,
@@ -1749,11 +1763,12 @@ void launch_path_lanes(const scene_dev& s, const frame_params& f, const paths_de
     const unsigned grid = unsigned((paths + uint32_t(block) - 1) / uint32_t(block));
     const unsigned g = lanes.blocks > 0 ? std::min(grid, unsigned(lanes.blocks)) : grid;
     const int batch_lanes = std::max(1, std::min(64, lanes.shade_batch));
+    const int steps = std::max(1, lanes.trace_steps);
     const dim3 gd{g}, bd{unsigned(block)};
     switch(pass.mode) {
-    case PATHS_FIRST: path_lanes_mode<PATHS_FIRST>(s, f, p, ctl, parity, queue, batch_lanes, pass, gd, bd, lds, stream, batch); break;
-    case PATHS_REST: path_lanes_mode<PATHS_REST>(s, f, p, ctl, parity, queue, batch_lanes, pass, gd, bd, lds, stream, batch); break;
-    default: path_lanes_mode<PATHS_WHOLE>(s, f, p, ctl, parity, queue, batch_lanes, pass, gd, bd, lds, stream, batch); break;
+    case PATHS_FIRST: path_lanes_mode<PATHS_FIRST>(s, f, p, ctl, parity, queue, batch_lanes, steps, pass, gd, bd, lds, stream, batch); break;
+    case PATHS_REST: path_lanes_mode<PATHS_REST>(s, f, p, ctl, parity, queue, batch_lanes, steps, pass, gd, bd, lds, stream, batch); break;
+    default: path_lanes_mode<PATHS_WHOLE>(s, f, p, ctl, parity, queue, batch_lanes, steps, pass, gd, bd, lds, stream, batch); break;
     }
 }
 // 8-bit key of a group's summed cost: exponent and top three mantissa bits of

@@ -108,7 +108,8 @@ class Tuning(ctypes.Structure):
                 ("exact_traversal", ctypes.c_int32), ("drain_threshold", ctypes.c_int32), ("lane_refill", ctypes.c_int32),
                 ("partitions", ctypes.c_int32), ("handoff_steps", ctypes.c_int32),
                 ("path_shade_batch", ctypes.c_int32), ("path_blocks_per_cu", ctypes.c_int32),
-                ("path_first_samples", ctypes.c_int32), ("path_sort_group", ctypes.c_int32)]
+                ("path_first_samples", ctypes.c_int32), ("path_sort_group", ctypes.c_int32),
+                ("path_trace_steps", ctypes.c_int32)]
 
 
 class BatchTrial(ctypes.Structure):
@@ -353,11 +354,12 @@ class Renderer:
 
     def set_tuning(self, trace_block=0, shade_block=0, blocks_per_cu=0, check_interval=0, time_kernels=False,
                    count_tests=False, exact_traversal=False, drain_threshold=0, lane_refill=0, partitions=0,
-                   handoff_steps=0, path_shade_batch=0, path_blocks_per_cu=0, path_first_samples=0, path_sort_group=0):
+                   handoff_steps=0, path_shade_batch=0, path_blocks_per_cu=0, path_first_samples=0, path_sort_group=0,
+                   path_trace_steps=0):
         t = Tuning(trace_block, shade_block, blocks_per_cu, check_interval, int(bool(time_kernels)),
                    int(bool(count_tests)), int(bool(exact_traversal)), int(drain_threshold), int(lane_refill),
                    int(partitions), int(handoff_steps), int(path_shade_batch), int(path_blocks_per_cu),
-                   int(path_first_samples), int(path_sort_group))
+                   int(path_first_samples), int(path_sort_group), int(path_trace_steps))
         _check(lib().pt_renderer_set_tuning(self._h, ctypes.byref(t)))
 
 

@@ -189,7 +189,7 @@ pt_status pt_renderer_enable_aovs(pt_renderer *renderer, int32_t enable);
 pt_status pt_renderer_read_aovs(const pt_renderer *renderer, float *albedo, float *normal, size_t count);
 
 /* Per-renderer tuning of the wavefront loop (0 keeps the current value).
- * Defaults: 128, 256, 16, 8, 0, 0, 0, 1073741824 (every path on path lanes), 48, 1, 16, 24, 12, 4, 16. */
+ * Defaults: 128, 256, 16, 8, 0, 0, 0, 1073741824 (every path on path lanes), 48, 1, 16, 24, 12, 4, 16, 8. */
 typedef struct pt_tuning {
     int32_t trace_block;     /* threads per block of the trace kernel, multiple of 64 */
     int32_t shade_block;     /* threads per block of the shade kernel, multiple of 64 */
@@ -219,6 +219,9 @@ typedef struct pt_tuning {
                                   most expensive first (1..65536; -1: one pass) */
     int32_t path_sort_group;    /* that ordering moves groups of this many consecutive pixels by their
                                   summed cost, each group keeping its pixels' order (1..4096) */
+    int32_t path_trace_steps;   /* path lanes: traversal steps a tracing lane takes per pass of its
+                                  state machine before the wave checks for shading and new pixels
+                                  (1..64) */
 } pt_tuning;
 pt_status pt_renderer_set_tuning(pt_renderer *renderer, const pt_tuning *tuning);
 
@@ -253,7 +256,7 @@ pt_status pt_batch_last_launch_stats(const pt_batch *batch, pt_launch_stats *sta
 /* Fields used: trace_block, shade_block, blocks_per_cu, check_interval (bounce
  * iterations per graph replay, rounded down to even), drain_threshold (live
  * paths at which the tail kernel takes over; -1 never), lane_refill, handoff_steps,
- * path_shade_batch, path_blocks_per_cu, path_first_samples, path_sort_group. */
+ * path_shade_batch, path_blocks_per_cu, path_first_samples, path_sort_group, path_trace_steps. */
 pt_status pt_batch_get_tuning(const pt_batch *batch, pt_tuning *tuning);
 pt_status pt_batch_set_tuning(pt_batch *batch, const pt_tuning *tuning);
 /* Autotuner: coordinate descent over workgroup sizes, persistent blocks per

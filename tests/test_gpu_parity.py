@@ -368,13 +368,13 @@ def test_handoff_leaves_results_unchanged(steps, drain):
     assert int(ext.sum()) + int(sh.sum()) <= st["extend_rays"] + st["shadow_rays"]
 
 
-@pytest.mark.parametrize("scene_name,blocks,batch,first,group,split", [("icosphere", 1, 1, 2, 64, True),
-                                                                        ("icosphere", -1, 16, 4, 64, False),
-                                                                        ("cornell", 1, 64, 1, 1, True),
-                                                                        ("cornell", 2, 16, -1, 64, False),
-                                                                        ("icosphere", 1, 24, 4, 7, True),
-                                                                        ("cornell", 1, 24, 4, 4096, True)])
-def test_path_lanes_match_wavefront(scene_name, blocks, batch, first, group, split):
+@pytest.mark.parametrize("scene_name,blocks,batch,first,group,steps,split", [("icosphere", 1, 1, 2, 64, 1, True),
+                                                                              ("icosphere", -1, 16, 4, 64, 8, False),
+                                                                              ("cornell", 1, 64, 1, 1, 3, True),
+                                                                              ("cornell", 2, 16, -1, 64, 1, False),
+                                                                              ("icosphere", 1, 24, 4, 7, 8, True),
+                                                                              ("cornell", 1, 24, 4, 4096, 64, True)])
+def test_path_lanes_match_wavefront(scene_name, blocks, batch, first, group, steps, split):
     # The path-lane kernel (every remaining pixel on a lane's state machine,
     # shading batched per wave, lanes taking pixel after pixel) must equal the
     # wavefront bounce loop exactly: pixels, PCG states and ray counts.  With
@@ -383,7 +383,8 @@ def test_path_lanes_match_wavefront(scene_name, blocks, batch, first, group, spl
     # two passes: `first` samples of every pixel, then the rest with groups of
     # `group` pixels reordered by cost (7 and 4096 leave a partial last
     # group).  With one lane per pixel or 2 blocks per compute unit every
-    # pixel has a lane from the start: one pass.
+    # pixel has a lane from the start: one pass.  `steps`: traversal steps a
+    # tracing lane takes per pass of its state machine.
     if scene_name == "icosphere":
         scene, cam = ptrace.icosphere(40, 3, 320, 240)
     else:
@@ -395,7 +396,7 @@ def test_path_lanes_match_wavefront(scene_name, blocks, batch, first, group, spl
     ref.add_samples(6)
     r = ptrace.Renderer(scene, cam, w, h, 23, False, 0, True, 8)
     r.set_tuning(drain_threshold=1 << 30, path_blocks_per_cu=blocks, path_shade_batch=batch, path_first_samples=first,
-                 path_sort_group=group)
+                 path_sort_group=group, path_trace_steps=steps)
     r.add_samples(3)
     r.add_samples(6)
     st = r.last_launch_stats()
