@@ -267,6 +267,39 @@ __device__ __forceinline__ bool slab_entry(float lox, float loy, float loz, floa
     return ok && high > low;
 }
 
+// The four child boxes of a wide node (SoA: child k in component k), each
+// exactly as slab_entry tests it.  A ray with no zero direction component
+// (almost every ray) takes a branch-free form: per axis the near plane is the
+// lower one unless 1/d < 0, a = (near - o) * (1/d), b likewise for the far
+// plane, low = a > low ? a : low and high = b < high ? b : high in x, y, z
+// order -- slab_entry's arithmetic without its per-axis d == 0 branches.
+__device__ __forceinline__ void slab_entry4(const float4& LX, const float4& LY, const float4& LZ, const float4& HX,
+                                            const float4& HY, const float4& HZ, const ray_in& r, float low,
+                                            float high, bool h[4], float e[4]) {
+    if(r.dx != 0 && r.dy != 0 && r.dz != 0) {
+        const bool sx = r.ix < 0, sy = r.iy < 0, sz = r.iz < 0;
+        const float lx[4] = {LX.x, LX.y, LX.z, LX.w}, ly[4] = {LY.x, LY.y, LY.z, LY.w}, lz[4] = {LZ.x, LZ.y, LZ.z, LZ.w};
+        const float hx[4] = {HX.x, HX.y, HX.z, HX.w}, hy[4] = {HY.x, HY.y, HY.z, HY.w}, hz[4] = {HZ.x, HZ.y, HZ.z, HZ.w};
+#pragma unroll
+        for(int k = 0; k < 4; k++) {
+            float lo = low, hi = high;
+            const float ax = ((sx ? hx[k] : lx[k]) - r.ox) * r.ix, bx = ((sx ? lx[k] : hx[k]) - r.ox) * r.ix;
+            lo = ax > lo ? ax : lo; hi = bx < hi ? bx : hi;
+            const float ay = ((sy ? hy[k] : ly[k]) - r.oy) * r.iy, by = ((sy ? ly[k] : hy[k]) - r.oy) * r.iy;
+            lo = ay > lo ? ay : lo; hi = by < hi ? by : hi;
+            const float az = ((sz ? hz[k] : lz[k]) - r.oz) * r.iz, bz = ((sz ? lz[k] : hz[k]) - r.oz) * r.iz;
+            lo = az > lo ? az : lo; hi = bz < hi ? bz : hi;
+            e[k] = lo;
+            h[k] = hi > lo;
+        }
+    } else {
+        h[0] = slab_entry(LX.x, LY.x, LZ.x, HX.x, HY.x, HZ.x, r, low, high, e[0]);
+        h[1] = slab_entry(LX.y, LY.y, LZ.y, HX.y, HY.y, HZ.y, r, low, high, e[1]);
+        h[2] = slab_entry(LX.z, LY.z, LZ.z, HX.z, HY.z, HZ.z, r, low, high, e[2]);
+        h[3] = slab_entry(LX.w, LY.w, LZ.w, HX.w, HY.w, HZ.w, r, low, high, e[3]);
+    }
+}
+
 // Closest (or any) hit over the 4-wide layout.  Children are visited nearest
 // first; a stacked child is dropped on pop when its entry distance is not
 // below the current limit, which is exactly the reference's pop-time box
@@ -308,10 +341,10 @@ __device__ bool traverse_wide(const scene_dev& s, int32_t root, const ray_in& r,
             const float limit = found ? next_toward(closest, high) : high;
             float e[4];
             int32_t c[4];
-            bool h0 = r0 != wide_empty && slab_entry(LX.x, LY.x, LZ.x, HX.x, HY.x, HZ.x, r, low, limit, e[0]);
-            bool h1 = r1 != wide_empty && slab_entry(LX.y, LY.y, LZ.y, HX.y, HY.y, HZ.y, r, low, limit, e[1]);
-            bool h2 = r2 != wide_empty && slab_entry(LX.z, LY.z, LZ.z, HX.z, HY.z, HZ.z, r, low, limit, e[2]);
-            bool h3 = r3 != wide_empty && slab_entry(LX.w, LY.w, LZ.w, HX.w, HY.w, HZ.w, r, low, limit, e[3]);
+            bool hh[4];
+            slab_entry4(LX, LY, LZ, HX, HY, HZ, r, low, limit, hh, e);
+            bool h0 = (r0 != wide_empty) & hh[0], h1 = (r1 != wide_empty) & hh[1];
+            bool h2 = (r2 != wide_empty) & hh[2], h3 = (r3 != wide_empty) & hh[3];
             if(STATS) boxes += (r0 != wide_empty) + (r1 != wide_empty) + (r2 != wide_empty) + (r3 != wide_empty);
             if(!h0) e[0] = INFINITY;
             if(!h1) e[1] = INFINITY;
@@ -399,10 +432,10 @@ __device__ __forceinline__ bool wide_step(const scene_dev& s, wide_query& q, wid
         const float limit = q.found ? next_toward(q.closest, q.high) : q.high;
         float e[4];
         int32_t c[4];
-        bool h0 = r0 != wide_empty && slab_entry(LX.x, LY.x, LZ.x, HX.x, HY.x, HZ.x, r, low, limit, e[0]);
-        bool h1 = r1 != wide_empty && slab_entry(LX.y, LY.y, LZ.y, HX.y, HY.y, HZ.y, r, low, limit, e[1]);
-        bool h2 = r2 != wide_empty && slab_entry(LX.z, LY.z, LZ.z, HX.z, HY.z, HZ.z, r, low, limit, e[2]);
-        bool h3 = r3 != wide_empty && slab_entry(LX.w, LY.w, LZ.w, HX.w, HY.w, HZ.w, r, low, limit, e[3]);
+        bool hh[4];
+        slab_entry4(LX, LY, LZ, HX, HY, HZ, r, low, limit, hh, e);
+        const bool h0 = (r0 != wide_empty) & hh[0], h1 = (r1 != wide_empty) & hh[1];
+        const bool h2 = (r2 != wide_empty) & hh[2], h3 = (r3 != wide_empty) & hh[3];
         if(STATS) boxes += (r0 != wide_empty) + (r1 != wide_empty) + (r2 != wide_empty) + (r3 != wide_empty);
         if(!h0) e[0] = INFINITY;
         if(!h1) e[1] = INFINITY;
