@@ -315,12 +315,23 @@ struct wide_stack {
     size_t spill_stride;
     uint32_t lane_id;   // grid-wide thread index (grids stay below 2^32 threads)
     __device__ __forceinline__ void store(int k, int32_t ref, float entry) {
-        int2 v = make_int2(ref, __float_as_int(entry));
-        if(k < wide_lds_entries) lds[k * stride] = v;
-        else spill[size_t(k - wide_lds_entries) * spill_stride + lane_id] = v;
+        if(k < wide_lds_entries) {
+            // one 64-bit LDS store (kept apart from the global one below, which
+            // the compiler would otherwise merge into a generic-address store)
+            *reinterpret_cast<unsigned long long *>(lds + k * stride) =
+                (unsigned long long)uint32_t(ref) | ((unsigned long long)__float_as_uint(entry) << 32);
+        } else {
+            int2 *slot = spill + size_t(k - wide_lds_entries) * spill_stride + lane_id;
+            slot->x = ref;
+            slot->y = __float_as_int(entry);
+        }
     }
     __device__ __forceinline__ int2 load(int k) const {
-        return k < wide_lds_entries ? lds[k * stride] : spill[size_t(k - wide_lds_entries) * spill_stride + lane_id];
+        if(k < wide_lds_entries) {
+            const unsigned long long v = *reinterpret_cast<const unsigned long long *>(lds + k * stride);
+            return make_int2(int32_t(uint32_t(v)), int32_t(uint32_t(v >> 32)));
+        }
+        return spill[size_t(k - wide_lds_entries) * spill_stride + lane_id];
     }
 };
 
@@ -354,7 +365,9 @@ __device__ bool traverse_wide(const scene_dev& s, int32_t root, const ray_in& r,
             const int nh = int(h0) + int(h1) + int(h2) + int(h3);
             if(nh) {
                 if(!ANY_HIT) {
-#define PT_CSWAP(i, j) if(e[j] < e[i]) { float te = e[i]; e[i] = e[j]; e[j] = te; int32_t tc = c[i]; c[i] = c[j]; c[j] = tc; }
+#define PT_CSWAP(i, j) { const bool sw = e[j] < e[i]; const float ei = sw ? e[j] : e[i], ej = sw ? e[i] : e[j]; \
+                               const int32_t ci = sw ? c[j] : c[i], cj = sw ? c[i] : c[j]; \
+                               e[i] = ei; e[j] = ej; c[i] = ci; c[j] = cj; }
                     PT_CSWAP(0, 1) PT_CSWAP(2, 3) PT_CSWAP(0, 2) PT_CSWAP(1, 3) PT_CSWAP(1, 2)
 #undef PT_CSWAP
                 } else {
@@ -366,7 +379,11 @@ __device__ bool traverse_wide(const scene_dev& s, int32_t root, const ray_in& r,
                     if(h3) { e[k] = e[3]; c[k] = c[3]; k++; }
                 }
                 if(count + nh - 1 > cap) { error = 1; return false; }
-                for(int k = nh - 1; k >= 1; k--) { stk.store(count, c[k], e[k]); count++; }
+                // c[nh-1] .. c[1] at count .. count + nh - 2 (c[k] at count + nh - 1 - k)
+                if(nh > 1) stk.store(count + nh - 2, c[1], e[1]);
+                if(nh > 2) stk.store(count + nh - 3, c[2], e[2]);
+                if(nh > 3) stk.store(count, c[3], e[3]);
+                count += nh - 1;
                 cur = c[0];
                 continue;
             }
@@ -444,12 +461,19 @@ __device__ __forceinline__ bool wide_step(const scene_dev& s, wide_query& q, wid
         c[0] = r0; c[1] = r1; c[2] = r2; c[3] = r3;
         const int nh = int(h0) + int(h1) + int(h2) + int(h3);
         if(nh) {
-            // nearest first (any-hit queries too: their order does not change the answer)
-#define PT_CSWAP(i, j) if(e[j] < e[i]) { float te = e[i]; e[i] = e[j]; e[j] = te; int32_t tc = c[i]; c[i] = c[j]; c[j] = tc; }
+            // nearest first (any-hit queries too: their order does not change the
+            // answer); swaps as selects, so the network has no branches
+#define PT_CSWAP(i, j) { const bool sw = e[j] < e[i]; const float ei = sw ? e[j] : e[i], ej = sw ? e[i] : e[j]; \
+                               const int32_t ci = sw ? c[j] : c[i], cj = sw ? c[i] : c[j]; \
+                               e[i] = ei; e[j] = ej; c[i] = ci; c[j] = cj; }
             PT_CSWAP(0, 1) PT_CSWAP(2, 3) PT_CSWAP(0, 2) PT_CSWAP(1, 3) PT_CSWAP(1, 2)
 #undef PT_CSWAP
             if(q.count + nh - 1 > s.wide_stack_capacity) { error = 1; return true; }
-            for(int k = nh - 1; k >= 1; k--) { stk.store(q.count, c[k], e[k]); q.count++; }
+            // c[nh-1] .. c[1] at count .. count + nh - 2 (c[k] at count + nh - 1 - k)
+            if(nh > 1) stk.store(q.count + nh - 2, c[1], e[1]);
+            if(nh > 2) stk.store(q.count + nh - 3, c[2], e[2]);
+            if(nh > 3) stk.store(q.count, c[3], e[3]);
+            q.count += nh - 1;
             q.cur = c[0];
             return false;
         }
