@@ -442,11 +442,12 @@ __device__ __forceinline__ bool wide_step(const scene_dev& s, wide_query& q, wid
                                           int& spheres, int& error) {
     const float low = 0.001f;
     const ray_in& r = q.r;
+    // The pruning limit, recomputed only when a hit moves the closest distance.
+    float limit = q.found ? next_toward(q.closest, q.high) : q.high;
     if(q.cur >= 0) {
         const float4 *nd = s.wnodes + 8 * size_t(q.cur);
         const float4 LX = nd[0], LY = nd[1], LZ = nd[2], HX = nd[3], HY = nd[4], HZ = nd[5], RF = nd[6];
         const int32_t r0 = __float_as_int(RF.x), r1 = __float_as_int(RF.y), r2 = __float_as_int(RF.z), r3 = __float_as_int(RF.w);
-        const float limit = q.found ? next_toward(q.closest, q.high) : q.high;
         float e[4];
         int32_t c[4];
         bool hh[4];
@@ -483,7 +484,6 @@ __device__ __forceinline__ bool wide_step(const scene_dev& s, wide_query& q, wid
         for(int j = 0; j < n; j++) {
             const int32_t k = first + j;
             const float4 A = s.leaves[3 * size_t(k)], B = s.leaves[3 * size_t(k) + 1], C = s.leaves[3 * size_t(k) + 2];
-            const float limit = q.found ? next_toward(q.closest, q.high) : q.high;
             float t, u = 0, vv = 0;
             bool hit;
             if(B.w < 0) { hit = sphere_test(A, r, low, limit, t); if(STATS) spheres++; }
@@ -493,6 +493,7 @@ __device__ __forceinline__ bool wide_step(const scene_dev& s, wide_query& q, wid
                 q.found = true; q.closest = t; q.best_rank = k;
                 q.best.t = t; q.best.u = u; q.best.v = vv; q.best.code = __float_as_int(C.w);
                 if(q.any) return true;
+                limit = next_toward(t, q.high);
             }
         }
     }
@@ -500,7 +501,7 @@ __device__ __forceinline__ bool wide_step(const scene_dev& s, wide_query& q, wid
         if(q.count == 0) return true;
         --q.count;
         const int2 top = stk.load(q.count);
-        if(q.found && !(__int_as_float(top.y) < next_toward(q.closest, q.high))) continue;
+        if(q.found && !(__int_as_float(top.y) < limit)) continue;
         q.cur = top.x;
         return false;
     }
