@@ -1614,7 +1614,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void p
         } else {
             // Up to trace_steps traversal steps per pass; a lane whose shadow
             // query ends and whose next extension query begins keeps going.
-            for(int k = 0; k < trace_steps && phase == TRACE; k++) {
+            // Each step takes the kind of work (wide node or leaf cluster) that
+            // most tracing lanes of the wave are at; the others wait a step, so
+            // a step issues one of the two code paths, not both.
+            for(int k = 0; k < trace_steps; k++) {
+                const uint64_t tr = __ballot(phase == TRACE);
+                if(!tr) break;
+                const uint64_t at_leaf = __ballot(phase == TRACE && q.cur < 0);
+                const bool leaves = 2 * __popcll(at_leaf) >= __popcll(tr);
+                if(phase != TRACE || (q.cur < 0) != leaves) continue;
                 if(!wide_step<false>(s, q, wstk, boxes, prims, spheres, error)) continue;
                 if(!q.any) {
                     phase = SHADE;
