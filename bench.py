@@ -17,8 +17,10 @@ Also reported:
                   algorithmic bytes of the frame / HIP-event time of the
                   path-lane work on the renderer's stream, against 8 TB/s.
                   Bytes per query = 32*box tests + 36*triangle tests +
-                  16*sphere tests + 48 (closest hit) or + 36 (shadow), from an
-                  untimed counting pass over the same rays.  traffic =
+                  16*sphere tests + 48 (closest hit) or + 36 (shadow), the
+                  tests being the reference-order traversal's trace_stats
+                  (branch nodes and leaves popped) from an untimed counting
+                  pass over the same rays.  traffic =
                   rocprofv3 FETCH_SIZE + WRITE_SIZE of that work (committed
                   summary under profiles/), scaled by a plain-copy calibration.
   cpu_baseline -- the CPU oracle (a C restatement of the reference's host
@@ -179,10 +181,12 @@ def main():
     log(f"upload {time.perf_counter() - t0:.2f}s")
 
     # Untimed counting pass: the same frame (same seed, reset) on the wavefront
-    # bounce loop with no tail kernel, so every query goes through the trace
-    # kernel and its box/triangle/sphere tests are counted.  The queries and
-    # their tests are the same whichever kernel traces them.
-    r.set_tuning(count_tests=True, drain_threshold=-1)
+    # bounce loop with no tail kernel and the reference-order binary traversal,
+    # so every query goes through the trace kernel and its box and primitive
+    # tests are the reference's own trace_stats (transport.h:43-85): branch
+    # nodes and leaves popped.  These counts depend on the scene and its flat
+    # hierarchy only, not on the layout or the schedule that is timed.
+    r.set_tuning(count_tests=True, drain_threshold=-1, exact_traversal=True)
     r.add_samples(args.spp)
     counts = r.last_launch_stats()
     ext_log, sh_log = r.iteration_log()

@@ -400,7 +400,7 @@ bool build_wide(const pt_scene_desc& s, const std::vector<float4>& ti, wide_layo
         double dx = double(b.maximum.x) - b.minimum.x, dy = double(b.maximum.y) - b.minimum.y, dz = double(b.maximum.z) - b.minimum.z;
         return dx * dy + dy * dz + dz * dx;
     };
-    // A cluster of count <= 4 leaves from depth-first position first; note that
+    // A cluster of count <= wide_leaf_max leaves from depth-first position first; note that
     // the first leaf alone encodes as -1, a valid reference.
     auto cluster_ref = [&](int32_t node) { return ~((first[size_t(node)] << 3) | (count[size_t(node)] - 1)); };
     bool failed = false;

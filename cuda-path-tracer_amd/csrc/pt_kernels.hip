@@ -1489,7 +1489,8 @@ __global__ __launch_bounds__(256) void drain_kernel(scene_dev s, frame_params f,
 // Mrays/s at the compiler's own 140 VGPRs and 3 waves; 5 waves spill 106
 // VGPRs and fall to 1613.  A tracing lane takes up to `trace_steps` steps
 // before the wave looks again for lanes to shade and idle lanes to refill
-// (1080p, 64 spp: 1, 4 and 8 steps give 2440, 2554 and 2563 Mrays/s).
+// (1080p, 64 spp: 1, 4 and 8 steps give 2440, 2554 and 2563 Mrays/s), each
+// step on the kind of work most tracing lanes are at (see below).
 //
 // Longest first (MODE).  A lane that takes an expensive pixel near the end of
 // the queue keeps the GPU waiting for all of that pixel's samples, since a
