@@ -264,22 +264,55 @@ const std::vector<float2>& host_trig_table() {
     return table;
 }
 
-// Per device, once per process: the host libm's cos/sin of all 2^24 angles,
-// uploaded whole (128 MB), so that the device reads the host's values.
-const float2 *device_trig_table() {
+struct trig_tables { const unsigned long long *bits; const uint32_t *rank; const float2 *exceptions; };
+
+// Per device, once per process: compare the device's double-rounded cos/sin
+// with the host libm for all 2^24 angles and keep only the differences.
+trig_tables device_trig_tables() {
     static std::mutex lock;
-    static std::map<int, const float2 *> cache;
+    static std::map<int, trig_tables> cache;
     int device = 0;
     HIP_CHECK(hipGetDevice(&device));
     std::lock_guard<std::mutex> guard(lock);
     auto it = cache.find(device);
     if(it != cache.end()) return it->second;
     const auto& host = host_trig_table();
-    float2 *table = nullptr;
-    HIP_CHECK(hipMalloc(reinterpret_cast<void **>(&table), trig_entries * sizeof(float2)));
-    HIP_CHECK(hipMemcpy(table, host.data(), trig_entries * sizeof(float2), hipMemcpyHostToDevice));
-    cache[device] = table;  // kept for the life of the process
-    return table;
+    std::vector<float2> dev(trig_entries);
+    {
+        dev_buf<float2> tmp(trig_entries);
+        pt::launch_trig_values(tmp.get(), nullptr);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipDeviceSynchronize());
+        HIP_CHECK(hipMemcpy(dev.data(), tmp.get(), trig_entries * sizeof(float2), hipMemcpyDeviceToHost));
+    }
+    const size_t words = trig_entries / 64;
+    std::vector<unsigned long long> bits(words, 0);
+    std::vector<uint32_t> rank(words, 0);
+    std::vector<float2> exceptions;
+    for(size_t w = 0; w < words; w++) {
+        rank[w] = uint32_t(exceptions.size());
+        for(size_t b = 0; b < 64; b++) {
+            size_t k = 64 * w + b;
+            const float2 h = host[k], d = dev[k];
+            if(std::memcmp(&h.x, &d.x, 4) != 0 || std::memcmp(&h.y, &d.y, 4) != 0) {
+                bits[w] |= 1ull << b;
+                exceptions.push_back(h);
+            }
+        }
+    }
+    if(exceptions.empty()) exceptions.push_back(make_float2(0, 0));
+    unsigned long long *db = nullptr;
+    uint32_t *dr = nullptr;
+    float2 *de = nullptr;
+    HIP_CHECK(hipMalloc(reinterpret_cast<void **>(&db), words * sizeof(unsigned long long)));
+    HIP_CHECK(hipMalloc(reinterpret_cast<void **>(&dr), words * sizeof(uint32_t)));
+    HIP_CHECK(hipMalloc(reinterpret_cast<void **>(&de), exceptions.size() * sizeof(float2)));
+    HIP_CHECK(hipMemcpy(db, bits.data(), words * sizeof(unsigned long long), hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(dr, rank.data(), words * sizeof(uint32_t), hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(de, exceptions.data(), exceptions.size() * sizeof(float2), hipMemcpyHostToDevice));
+    trig_tables t{db, dr, de};  // kept for the life of the process
+    cache[device] = t;
+    return t;
 }
 
 void require_device() {
@@ -508,7 +541,7 @@ struct device_scene {
         view.lights = lights.get(); view.light_count = int32_t(lights_host.size());
         view.light_area = lights_host.empty() ? 0 : lights_host.back().cumulative_area;
         view.stack_capacity = std::max(2, std::min(stack_limit, int(s.max_depth)));
-        view.trig_values = nullptr;
+        view.trig_bits = nullptr; view.trig_rank = nullptr; view.trig_exceptions = nullptr;
         view.log2_table = nullptr; view.log2_count = 0;
         view.wnodes = nullptr; view.leaves = nullptr; view.wroot_ref = pt::empty_root; view.wide_stack_capacity = 0;
         if(with_wide) {
@@ -522,7 +555,8 @@ struct device_scene {
             }
         }
         if(with_tables) {
-            view.trig_values = device_trig_table();
+            const trig_tables t = device_trig_tables();
+            view.trig_bits = t.bits; view.trig_rank = t.rank; view.trig_exceptions = t.exceptions;
         }
         if(with_tables && with_log2) {
             // log2f(1+n) of the host libm for every count a traversal can reach.

@@ -40,9 +40,13 @@ struct scene_dev {
     int32_t light_count;
     float light_area;
     // cos/sin of the 2^24 angles 2*float(pi)*k*2^-24 as the host libm's
-    // cosf/sinf return them (one float2 per k, 128 MB per device).
+    // cosf/sinf return them: computed on the device in double and rounded,
+    // except for the k flagged in trig_bits, whose host values are
+    // trig_exceptions[trig_rank[k/64] + (set bits of word k/64 below k)].
     // log2_table: log2f(1+n) of the host libm for n < log2_count.
-    const float2 *trig_values;
+    const unsigned long long *trig_bits;
+    const uint32_t *trig_rank;
+    const float2 *trig_exceptions;
     const float *log2_table;
     int32_t log2_count;
     int32_t stack_capacity;    // per-thread traversal stack entries (<= 64)
@@ -128,6 +132,8 @@ struct frame_params {
 };
 
 // ---- launchers (pt_kernels.hip) ----
+// Device cos/sin (double, rounded) of every angle index k in [0, 2^24).
+void launch_trig_values(float2 *out, hipStream_t stream);
 void launch_query(const scene_dev& s, const pt_ray *rays, pt_trace_result *out, int32_t *boxes, int32_t *prims,
                   int32_t count, bool brute, hipStream_t stream);
 // Batched rendering: many views, each of one scene of a merged scene set

@@ -618,8 +618,18 @@ __device__ __forceinline__ float mis_weight(float a, float b) {
     return a * a / (a * a + b * b);
 }
 // cosf/sinf of angle = 2*float(pi)*(k*2^-24) (light_sampling.h:20,47) as
-// the host libm computes them, read from the table of all 2^24 angles.
-__device__ __forceinline__ float2 libm_cos_sin(const scene_dev& s, uint32_t k) { return s.trig_values[k]; }
+// the host libm computes them: double evaluation rounded to float, replaced
+// by the stored host value for the k where the two differ.
+__device__ __noinline__ float2 device_cos_sin(uint32_t k) {
+    const float angle = (2 * PI_F) * (float(k) * 0x1p-24f);
+    return make_float2(float(cos(double(angle))), float(sin(double(angle))));
+}
+__device__ __forceinline__ float2 libm_cos_sin(const scene_dev& s, uint32_t k) {
+    const unsigned long long word = s.trig_bits[k >> 6];
+    const unsigned long long bit = 1ull << (k & 63u);
+    if(word & bit) return s.trig_exceptions[s.trig_rank[k >> 6] + uint32_t(__popcll(word & (bit - 1ull)))];
+    return device_cos_sin(k);
+}
 
 // light_sampling.h:18-25.
 __device__ __forceinline__ v3 cosine_direction(const scene_dev& s, v3 normal, rng_t& r) {
@@ -1683,7 +1693,15 @@ __global__ void pack_kernel(const float *sum, uint8_t *rgba, int64_t count, int3
 
 } // namespace
 
+__global__ void trig_values_kernel(float2 *out) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if(k < (1u << 24)) out[k] = device_cos_sin(k);
+}
+
 // ---------------- launchers ----------------
+void launch_trig_values(float2 *out, hipStream_t stream) {
+    hipLaunchKernelGGL(trig_values_kernel, dim3((1u << 24) / 256), dim3(256), 0, stream, out);
+}
 void launch_query(const scene_dev& s, const pt_ray *rays, pt_trace_result *out, int32_t *boxes, int32_t *prims,
                   int32_t count, bool brute, hipStream_t stream) {
     const int block = 256;
