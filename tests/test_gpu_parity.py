@@ -272,15 +272,26 @@ def test_state_roundtrip_resumes_exactly():
     assert np.array_equal(a.image(), c.image()) and a.rays_traced() == c.rays_traced()
 
 
-@pytest.mark.parametrize("maker", ["cornell", "icosphere", "soup", "asset_sphere"])
+@pytest.mark.parametrize("maker", ["cornell", "icosphere", "soup", "asset_sphere", "icosphere_dx0"])
 def test_wide_traversal_matches_reference_order(maker):
     # The shaded path's 4-wide nearest-first traversal must return the same
     # closest hit as the reference-order binary traversal on every query:
     # accumulation sums, PCG states and ray counts are compared bit for bit.
+    # "icosphere_dx0": every camera ray has a direction x component of exactly
+    # 0 (the camera's corner, horizontal and vertical vectors and its origin
+    # share x = 0), so primary rays take the box test's zero-component form
+    # and the secondary rays the branch-free one.
     if maker == "cornell":
         scene, cam = ptrace.cornell(5, 96, 64)
     elif maker == "icosphere":
         scene, cam = ptrace.icosphere(40, 3, 96, 64)
+    elif maker == "icosphere_dx0":
+        scene, cam = ptrace.icosphere(40, 3, 96, 64)
+        cam = np.array(cam, copy=True)
+        cam["origin"] = (0.0, 0.55, 3.1)
+        cam["lower_left_corner"] = (0.0, -0.55, 1.85)
+        cam["horizontal"] = (0.0, 0.0, 0.5)
+        cam["vertical"] = (0.0, 2.2, 0.0)
     elif maker == "soup":
         scene, cam = ptrace.soup(30000, 2, 96, 64)
     else:
