@@ -175,13 +175,13 @@ void launch_shade(const scene_dev& s, const frame_params& f, const paths_dev& p,
                   int grid, int block, hipStream_t stream, const batch_dev *batch = nullptr);
 void launch_regen(const frame_params& f, const paths_dev& p, loop_ctl *ctl, int parity, int grid, int block,
                   hipStream_t stream, const batch_dev *batch = nullptr);
-// Finishes the pending shadow queries and then every queued path in one
-// thread each; `shadows`/`paths` are the host's copies of the queue sizes.
-// Tail: shadow queries pending (and queries handed on by the last trace
-// launch), then every remaining path to its end in one thread.
-// Path lanes of the tail kernel: lanes of a wave waiting to shade before the
-// wave shades them, the resident grid (blocks 0: one lane per path) and the
-// traversal steps a tracing lane takes per pass of its state machine.
+// Finishes the pending shadow queries (and the queries handed on by the last
+// trace launch), then every queued path to its end: on path lanes with the
+// 4-wide traversal, otherwise one thread per path (drain_kernel).
+// `shadows`/`saved`/`paths` are the host's copies of the queue sizes.
+// Path lanes: lanes of a wave waiting to shade before the wave shades them,
+// the resident grid (blocks 0: one lane per path) and the traversal steps a
+// tracing lane takes per pass of its state machine.
 struct path_lanes { int shade_batch; int blocks; int trace_steps; };
 void launch_drain(const scene_dev& s, const frame_params& f, const paths_dev& p, loop_ctl *ctl, int parity,
                   uint32_t shadows, uint32_t saved, uint32_t paths, int block, path_lanes lanes, hipStream_t stream,

@@ -164,7 +164,7 @@ typedef struct pt_launch_stats {
     uint64_t box_tests;      /* filled when pt_tuning.count_tests is set */
     uint64_t triangle_tests;
     uint64_t sphere_tests;
-    uint64_t drained_paths;  /* paths finished by the tail kernel */
+    uint64_t drained_paths;  /* paths finished on path lanes (or the drain kernel) */
     float drain_ms;
     int32_t split_launches;  /* path-lane launches run in two passes (pt_tuning.path_first_samples) */
 } pt_launch_stats;
@@ -199,8 +199,9 @@ typedef struct pt_tuning {
     int32_t count_tests;     /* 1: count box/triangle/sphere tests (slower; for roofline bytes) */
     int32_t exact_traversal; /* 1: shaded path on the reference-order binary traversal instead of the
                                 4-wide nearest-first one (both return the same closest hit) */
-    int32_t drain_threshold; /* live paths at or below which the bounce loop hands every remaining
-                                pixel to one thread (-1: never) */
+    int32_t drain_threshold; /* live paths at or below which the wavefront bounce loop hands every
+                                remaining path to path lanes (the default, 2^30, does so from the
+                                start; -1: never) */
     int32_t lane_refill;     /* wide traversal with persistent lanes: idle lanes of a wave that
                                 trigger a claim of new queries (1..64; -1: one batch per wave) */
     int32_t partitions;      /* pixel partitions whose bounce loops run concurrently on their own
@@ -209,7 +210,7 @@ typedef struct pt_tuning {
                                 empty before it saves the queries still running and ends; the next
                                 bounce iteration resumes them first and shades their paths then
                                 (1..65536; -1: never) */
-    int32_t path_shade_batch;  /* path lanes (the tail kernel): lanes of a wave that wait to shade
+    int32_t path_shade_batch;  /* path lanes: lanes of a wave that wait to shade
                                   before the wave shades them together (1..64) */
     int32_t path_blocks_per_cu; /* path lanes: resident 256-thread blocks per compute unit, each lane
                                   taking pixel after pixel (-1: one lane per remaining pixel) */
@@ -255,7 +256,7 @@ pt_status pt_batch_stats(const pt_batch *batch, int32_t *samples, float *millise
 pt_status pt_batch_last_launch_stats(const pt_batch *batch, pt_launch_stats *stats);
 /* Fields used: trace_block, shade_block, blocks_per_cu, check_interval (bounce
  * iterations per graph replay, rounded down to even), drain_threshold (live
- * paths at which the tail kernel takes over; -1 never), lane_refill, handoff_steps,
+ * paths at which path lanes take over; -1 never), lane_refill, handoff_steps,
  * path_shade_batch, path_blocks_per_cu, path_first_samples, path_sort_group, path_trace_steps. */
 pt_status pt_batch_get_tuning(const pt_batch *batch, pt_tuning *tuning);
 pt_status pt_batch_set_tuning(pt_batch *batch, const pt_tuning *tuning);
