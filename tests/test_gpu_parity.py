@@ -329,6 +329,27 @@ def test_tail_drain_matches_wavefront_loop(threshold):
         assert r.last_launch_stats()["drained_paths"] > 0
 
 
+def test_partitions_split_path_lanes_match_one_pass():
+    # Two pixel partitions of about 92,000 pixels each on one 256-lane block per
+    # compute unit: each partition's path-lane launch of 6 samples runs in two
+    # passes of its own (4 samples, then the rest in cost order, with its own
+    # cost, key and queue buffers); the result must equal one partition in one
+    # pass.
+    scene, cam = ptrace.icosphere(30, 6, 460, 400)
+    ref = ptrace.Renderer(scene, cam, 460, 400, 13, False, 0, True, 8)
+    ref.set_tuning(partitions=1, path_first_samples=-1)
+    ref.add_samples(2)
+    ref.add_samples(6)
+    r = ptrace.Renderer(scene, cam, 460, 400, 13, False, 0, True, 8)
+    r.set_tuning(partitions=2, path_blocks_per_cu=1)
+    r.add_samples(2)
+    r.add_samples(6)
+    assert r.last_launch_stats()["split_launches"] == 2
+    (a, ga), (b, gb) = ref.read_state(), r.read_state()
+    assert np.array_equal(a, b) and np.array_equal(ga, gb)
+    assert ref.rays_traced() == r.rays_traced()
+
+
 @pytest.mark.parametrize("parts", [2, 3, 5])
 def test_pixel_partitions_match_single_loop(parts):
     # Concurrent pixel partitions (4096-pixel chunks dealt round-robin, the
