@@ -425,14 +425,17 @@ __device__ bool traverse_wide(const scene_dev& s, int32_t root, const ray_in& r,
 // take a new one while the rest of its wave is still traversing.
 struct wide_query {
     ray_in r;
-    float high, closest;
+    // limit: the pruning limit, the query's upper bound until the first hit,
+    // then nextafterf(closest, upper bound), which for 0 < closest < upper
+    // bound is the next float above closest.
+    float limit, closest;
     int32_t cur, count, best_rank;
     best_hit best;
     bool found, any;
 };
 
 __device__ __forceinline__ void wide_begin(int32_t root, wide_query& q, bool any, float high) {
-    q.high = high; q.closest = high; q.cur = root; q.count = 0; q.best_rank = -1;
+    q.limit = high; q.closest = high; q.cur = root; q.count = 0; q.best_rank = -1;
     q.found = false; q.any = any;
 }
 
@@ -442,8 +445,7 @@ __device__ __forceinline__ bool wide_step(const scene_dev& s, wide_query& q, wid
                                           int& spheres, int& error) {
     const float low = 0.001f;
     const ray_in& r = q.r;
-    // The pruning limit, recomputed only when a hit moves the closest distance.
-    float limit = q.found ? next_toward(q.closest, q.high) : q.high;
+    float limit = q.limit;
     if(q.cur >= 0) {
         const float4 *nd = s.wnodes + 8 * size_t(q.cur);
         const float4 LX = nd[0], LY = nd[1], LZ = nd[2], HX = nd[3], HY = nd[4], HZ = nd[5], RF = nd[6];
@@ -493,7 +495,9 @@ __device__ __forceinline__ bool wide_step(const scene_dev& s, wide_query& q, wid
                 q.found = true; q.closest = t; q.best_rank = k;
                 q.best.t = t; q.best.u = u; q.best.v = vv; q.best.code = __float_as_int(C.w);
                 if(q.any) return true;
-                limit = next_toward(t, q.high);
+                // next_toward(t, upper bound): a hit has 0.001 < t < limit
+                limit = __uint_as_float(__float_as_uint(t) + 1u);
+                q.limit = limit;
             }
         }
     }
@@ -1099,11 +1103,13 @@ __global__ __launch_bounds__(256) void trace_lanes_kernel(scene_dev s, frame_par
                     pix = any ? ~h.x : h.x;
                     const float4 o = any ? p.sh_o[pix] : p.ray_o[pix], d = any ? p.sh_d[pix] : p.ray_d[pix];
                     q.r = setup_ray(o.x, o.y, o.z, d.x, d.y, d.z);
-                    wide_begin(query_root<BATCH>(s, bd, pix), q, any, any ? o.w : FLT_MAX);
+                    const float high = any ? o.w : FLT_MAX;
+                    wide_begin(query_root<BATCH>(s, bd, pix), q, any, high);
                     q.cur = h.y; q.count = h.z; q.best_rank = h.w; q.found = h.w >= 0;
                     if(q.found) {
                         const int4 bb = in.best[item];
                         q.closest = __int_as_float(bb.x);
+                        q.limit = next_toward(q.closest, high);
                         q.best.t = __int_as_float(bb.x); q.best.u = __int_as_float(bb.y);
                         q.best.v = __int_as_float(bb.z); q.best.code = bb.w;
                     }
