@@ -280,15 +280,17 @@ __device__ __forceinline__ void slab_entry4(const float4& LX, const float4& LY, 
         const bool sx = r.ix < 0, sy = r.iy < 0, sz = r.iz < 0;
         const float lx[4] = {LX.x, LX.y, LX.z, LX.w}, ly[4] = {LY.x, LY.y, LY.z, LY.w}, lz[4] = {LZ.x, LZ.y, LZ.z, LZ.w};
         const float hx[4] = {HX.x, HX.y, HX.z, HX.w}, hy[4] = {HY.x, HY.y, HY.z, HY.w}, hz[4] = {HZ.x, HZ.y, HZ.z, HZ.w};
+        // fmaxf/fminf give what the ternaries give: a NaN plane distance (0
+        // times an infinite 1/d) is skipped by both, and the sign of a zero
+        // cannot matter since low starts at 0.001 and a zero high misses
+        // either way.  They let the compiler use 3-input min/max.
 #pragma unroll
         for(int k = 0; k < 4; k++) {
-            float lo = low, hi = high;
             const float ax = ((sx ? hx[k] : lx[k]) - r.ox) * r.ix, bx = ((sx ? lx[k] : hx[k]) - r.ox) * r.ix;
-            lo = ax > lo ? ax : lo; hi = bx < hi ? bx : hi;
             const float ay = ((sy ? hy[k] : ly[k]) - r.oy) * r.iy, by = ((sy ? ly[k] : hy[k]) - r.oy) * r.iy;
-            lo = ay > lo ? ay : lo; hi = by < hi ? by : hi;
             const float az = ((sz ? hz[k] : lz[k]) - r.oz) * r.iz, bz = ((sz ? lz[k] : hz[k]) - r.oz) * r.iz;
-            lo = az > lo ? az : lo; hi = bz < hi ? bz : hi;
+            const float lo = fmaxf(fmaxf(fmaxf(low, ax), ay), az);
+            const float hi = fminf(fminf(fminf(high, bx), by), bz);
             e[k] = lo;
             h[k] = hi > lo;
         }
