@@ -493,7 +493,9 @@ __device__ __forceinline__ bool wide_step(const scene_dev& s, wide_query& q, wid
             if(B.w < 0) { hit = sphere_test(A, r, low, limit, t); if(STATS) spheres++; }
             else hit = tri_test(A, B, C, r, low, limit, t, u, vv);
             if(STATS) prims++;
-            if(hit && (!q.found || t < q.closest || (t == q.closest && k > q.best_rank))) {
+            // A hit has t < limit, so t <= closest (before the first hit closest
+            // is the upper bound and best_rank -1): the later leaf wins a tie.
+            if(hit && (t < q.closest || k > q.best_rank)) {
                 q.found = true; q.closest = t; q.best_rank = k;
                 q.best.t = t; q.best.u = u; q.best.v = vv; q.best.code = __float_as_int(C.w);
                 if(q.any) return true;
@@ -507,7 +509,10 @@ __device__ __forceinline__ bool wide_step(const scene_dev& s, wide_query& q, wid
         if(q.count == 0) return true;
         --q.count;
         const int2 top = stk.load(q.count);
-        if(q.found && !(__int_as_float(top.y) < limit)) continue;
+        // Entries were pushed below the limit of their time; before the first
+        // hit the limit is the upper bound, so only entries pushed before a hit
+        // can fail this test.
+        if(!(__int_as_float(top.y) < limit)) continue;
         q.cur = top.x;
         return false;
     }
