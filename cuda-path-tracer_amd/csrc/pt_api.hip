@@ -700,7 +700,7 @@ struct pt_renderer {
     dev_buf<int4> hit;
     std::vector<std::unique_ptr<partition>> parts;
     dev_buf<float> aov;                  // first-hit albedo and normal sums (6 per pixel), when enabled
-    pt_tuning tuning{128, 256, 16, 8, 0, 0, 0, all_paths, 48, 1, 16, 32, 12, 4, 8, 8};
+    pt_tuning tuning{128, 256, 16, 8, 0, 0, 0, all_paths, 48, 1, 16, 32, 12, 4, 4, 8};
     int cus = 1;
     pt_launch_stats last{};
 
@@ -1105,7 +1105,7 @@ struct pt_batch {
     partition part;
     hipStream_t stream = nullptr;
     hipEvent_t start = nullptr, stop = nullptr;
-    pt_tuning tuning{128, 256, 16, 8, 0, 0, 0, all_paths, 48, 1, 16, 32, 12, 4, 8, 8};
+    pt_tuning tuning{128, 256, 16, 8, 0, 0, 0, all_paths, 48, 1, 16, 32, 12, 4, 4, 8};
     int cus = 1;
     int32_t samples = 0;
     uint64_t rays = 0;

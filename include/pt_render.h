@@ -189,7 +189,7 @@ pt_status pt_renderer_enable_aovs(pt_renderer *renderer, int32_t enable);
 pt_status pt_renderer_read_aovs(const pt_renderer *renderer, float *albedo, float *normal, size_t count);
 
 /* Per-renderer tuning of the wavefront loop (0 keeps the current value).
- * Defaults: 128, 256, 16, 8, 0, 0, 0, 1073741824 (every path on path lanes), 48, 1, 16, 32, 12, 4, 8, 8. */
+ * Defaults: 128, 256, 16, 8, 0, 0, 0, 1073741824 (every path on path lanes), 48, 1, 16, 32, 12, 4, 4, 8. */
 typedef struct pt_tuning {
     int32_t trace_block;     /* threads per block of the trace kernel, multiple of 64 */
     int32_t shade_block;     /* threads per block of the shade kernel, multiple of 64 */
