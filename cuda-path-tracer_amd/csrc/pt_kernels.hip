@@ -444,7 +444,7 @@ __device__ __forceinline__ void wide_begin(int32_t root, wide_query& q, bool any
 // Returns true when the query has finished.
 template<bool STATS>
 __device__ __forceinline__ bool wide_step(const scene_dev& s, wide_query& q, wide_stack& stk, int& boxes, int& prims,
-                                          int& spheres, int& error) {
+                                          int& spheres) {
     const float low = 0.001f;
     const ray_in& r = q.r;
     float limit = q.limit;
@@ -473,7 +473,10 @@ __device__ __forceinline__ bool wide_step(const scene_dev& s, wide_query& q, wid
                                e[i] = ei; e[j] = ej; c[i] = ci; c[j] = cj; }
             PT_CSWAP(0, 1) PT_CSWAP(2, 3) PT_CSWAP(0, 2) PT_CSWAP(1, 3) PT_CSWAP(1, 2)
 #undef PT_CSWAP
-            if(q.count + nh - 1 > s.wide_stack_capacity) { error = 1; return true; }
+            // No overflow test: at a node of depth d (the root at 1) the stack
+            // holds at most the 3 unvisited siblings of each of its d - 1
+            // ancestors, so after these pushes count <= 3d, below the
+            // capacity 3 * (deepest node's depth) + 1 the stack is sized for.
             // c[nh-1] .. c[1] at count .. count + nh - 2 (c[k] at count + nh - 1 - k)
             if(nh > 1) stk.store(q.count + nh - 2, c[1], e[1]);
             if(nh > 2) stk.store(q.count + nh - 3, c[2], e[2]);
@@ -1167,7 +1170,7 @@ __global__ __launch_bounds__(256) void trace_lanes_kernel(scene_dev s, frame_par
                 }
             }
         }
-        if(active && wide_step<COUNT>(s, q, wstk, boxes, prims, spheres, error)) {
+        if(active && wide_step<COUNT>(s, q, wstk, boxes, prims, spheres)) {
             active = false;
             if(!q.any) {
                 p.hit[pix] = q.found ? make_int4(__float_as_int(q.best.t), __float_as_int(q.best.u),
@@ -1637,7 +1640,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void p
                 const uint64_t at_leaf = __ballot(phase == TRACE && q.cur < 0);
                 const bool leaves = 2 * __popcll(at_leaf) >= __popcll(tr);
                 if(phase != TRACE || (q.cur < 0) != leaves) continue;
-                if(!wide_step<false>(s, q, wstk, boxes, prims, spheres, error)) continue;
+                if(!wide_step<false>(s, q, wstk, boxes, prims, spheres)) continue;
                 if(!q.any) {
                     phase = SHADE;
                 } else {
