@@ -243,62 +243,36 @@ __device__ bool traverse_brute(const scene_dev& s, const ray_in& r, float low, f
 }
 
 // ---------------- 4-wide nearest-first traversal (shaded path) ----------------
-// Same slab arithmetic as trace_box (transport.h:14-31) with the reference's
-// near/far selection by the sign of 1/d, returning the entry distance.
-__device__ __forceinline__ bool slab_entry(float lox, float loy, float loz, float hix, float hiy, float hiz,
-                                           const ray_in& r, float low, float high, float& entry) {
-    bool ok = true;
-    if(r.dx == 0) ok = ok && !(r.ox < lox || r.ox > hix);
-    else {
-        float a = ((r.ix < 0 ? hix : lox) - r.ox) * r.ix, b = ((r.ix < 0 ? lox : hix) - r.ox) * r.ix;
-        low = a > low ? a : low; high = b < high ? b : high;
-    }
-    if(r.dy == 0) ok = ok && !(r.oy < loy || r.oy > hiy);
-    else {
-        float a = ((r.iy < 0 ? hiy : loy) - r.oy) * r.iy, b = ((r.iy < 0 ? loy : hiy) - r.oy) * r.iy;
-        low = a > low ? a : low; high = b < high ? b : high;
-    }
-    if(r.dz == 0) ok = ok && !(r.oz < loz || r.oz > hiz);
-    else {
-        float a = ((r.iz < 0 ? hiz : loz) - r.oz) * r.iz, b = ((r.iz < 0 ? loz : hiz) - r.oz) * r.iz;
-        low = a > low ? a : low; high = b < high ? b : high;
-    }
-    entry = low;
-    return ok && high > low;
-}
-
-// The four child boxes of a wide node (SoA: child k in component k), each
-// exactly as slab_entry tests it.  A ray with no zero direction component
-// (almost every ray) takes a branch-free form: per axis the near plane is the
-// lower one unless 1/d < 0, a = (near - o) * (1/d), b likewise for the far
-// plane, low = a > low ? a : low and high = b < high ? b : high in x, y, z
-// order -- slab_entry's arithmetic without its per-axis d == 0 branches.
+// The four child boxes of a wide node (SoA: child k in component k), with
+// trace_box's arithmetic (transport.h:14-31) and the reference's near/far
+// choice by the sign of 1/d: per axis a = (near - o) * (1/d), b likewise for
+// the far plane, low = a > low ? a : low and high = b < high ? b : high in x,
+// y, z order, the box hit when high > low; returns the entry distance low.
+// fmaxf/fminf give what those ternaries give, since both skip a NaN plane
+// distance, and they let the compiler use 3-input min/max.  One form serves
+// every ray, d == 0 included, where trace_box instead rejects the box if
+// o < min or o > max and leaves low and high alone: 1/d is then +-inf, so
+// with o outside the slab one plane distance is +inf as the near one or
+// -inf as the far one (a miss), with o strictly inside they are -inf and
+// +inf (no bound), and with o on a plane that distance is 0 * inf = NaN
+// (skipped).  (min - o is never 0 for min != o: denormals are kept.)  The
+// sign of a zero distance cannot matter: low starts at 0.001 and a zero
+// high misses either way.
 __device__ __forceinline__ void slab_entry4(const float4& LX, const float4& LY, const float4& LZ, const float4& HX,
                                             const float4& HY, const float4& HZ, const ray_in& r, float low,
                                             float high, bool h[4], float e[4]) {
-    if(r.dx != 0 && r.dy != 0 && r.dz != 0) {
-        const bool sx = r.ix < 0, sy = r.iy < 0, sz = r.iz < 0;
-        const float lx[4] = {LX.x, LX.y, LX.z, LX.w}, ly[4] = {LY.x, LY.y, LY.z, LY.w}, lz[4] = {LZ.x, LZ.y, LZ.z, LZ.w};
-        const float hx[4] = {HX.x, HX.y, HX.z, HX.w}, hy[4] = {HY.x, HY.y, HY.z, HY.w}, hz[4] = {HZ.x, HZ.y, HZ.z, HZ.w};
-        // fmaxf/fminf give what the ternaries give: a NaN plane distance (0
-        // times an infinite 1/d) is skipped by both, and the sign of a zero
-        // cannot matter since low starts at 0.001 and a zero high misses
-        // either way.  They let the compiler use 3-input min/max.
+    const bool sx = r.ix < 0, sy = r.iy < 0, sz = r.iz < 0;
+    const float lx[4] = {LX.x, LX.y, LX.z, LX.w}, ly[4] = {LY.x, LY.y, LY.z, LY.w}, lz[4] = {LZ.x, LZ.y, LZ.z, LZ.w};
+    const float hx[4] = {HX.x, HX.y, HX.z, HX.w}, hy[4] = {HY.x, HY.y, HY.z, HY.w}, hz[4] = {HZ.x, HZ.y, HZ.z, HZ.w};
 #pragma unroll
-        for(int k = 0; k < 4; k++) {
-            const float ax = ((sx ? hx[k] : lx[k]) - r.ox) * r.ix, bx = ((sx ? lx[k] : hx[k]) - r.ox) * r.ix;
-            const float ay = ((sy ? hy[k] : ly[k]) - r.oy) * r.iy, by = ((sy ? ly[k] : hy[k]) - r.oy) * r.iy;
-            const float az = ((sz ? hz[k] : lz[k]) - r.oz) * r.iz, bz = ((sz ? lz[k] : hz[k]) - r.oz) * r.iz;
-            const float lo = fmaxf(fmaxf(fmaxf(low, ax), ay), az);
-            const float hi = fminf(fminf(fminf(high, bx), by), bz);
-            e[k] = lo;
-            h[k] = hi > lo;
-        }
-    } else {
-        h[0] = slab_entry(LX.x, LY.x, LZ.x, HX.x, HY.x, HZ.x, r, low, high, e[0]);
-        h[1] = slab_entry(LX.y, LY.y, LZ.y, HX.y, HY.y, HZ.y, r, low, high, e[1]);
-        h[2] = slab_entry(LX.z, LY.z, LZ.z, HX.z, HY.z, HZ.z, r, low, high, e[2]);
-        h[3] = slab_entry(LX.w, LY.w, LZ.w, HX.w, HY.w, HZ.w, r, low, high, e[3]);
+    for(int k = 0; k < 4; k++) {
+        const float ax = ((sx ? hx[k] : lx[k]) - r.ox) * r.ix, bx = ((sx ? lx[k] : hx[k]) - r.ox) * r.ix;
+        const float ay = ((sy ? hy[k] : ly[k]) - r.oy) * r.iy, by = ((sy ? ly[k] : hy[k]) - r.oy) * r.iy;
+        const float az = ((sz ? hz[k] : lz[k]) - r.oz) * r.iz, bz = ((sz ? lz[k] : hz[k]) - r.oz) * r.iz;
+        const float lo = fmaxf(fmaxf(fmaxf(low, ax), ay), az);
+        const float hi = fminf(fminf(fminf(high, bx), by), bz);
+        e[k] = lo;
+        h[k] = hi > lo;
     }
 }
 
