@@ -462,9 +462,18 @@ __device__ __forceinline__ bool wide_step(const scene_dev& s, wide_query& q, wid
     } else {
         const int32_t v = ~q.cur;
         const int32_t first = v >> 3, n = (v & 7) + 1;
-        for(int j = 0; j < n; j++) {
+        // A cluster holds at most two leaves: both records are loaded before
+        // the first test, so that their latencies overlap.
+        static_assert(wide_leaf_max <= 2, "the leaf step loads at most two leaves");
+        const float4 *L = s.leaves + 3 * size_t(first);
+        float4 rec[2][3];
+        rec[0][0] = L[0]; rec[0][1] = L[1]; rec[0][2] = L[2];
+        if(n > 1) { rec[1][0] = L[3]; rec[1][1] = L[4]; rec[1][2] = L[5]; }
+#pragma unroll
+        for(int j = 0; j < 2; j++) {
+            if(j >= n) break;
             const int32_t k = first + j;
-            const float4 A = s.leaves[3 * size_t(k)], B = s.leaves[3 * size_t(k) + 1], C = s.leaves[3 * size_t(k) + 2];
+            const float4 A = rec[j][0], B = rec[j][1], C = rec[j][2];
             float t, u = 0, vv = 0;
             bool hit;
             if(B.w < 0) { hit = sphere_test(A, r, low, limit, t); if(STATS) spheres++; }
