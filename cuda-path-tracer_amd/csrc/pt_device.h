@@ -66,6 +66,7 @@ struct scene_dev {
 constexpr int32_t wide_empty = INT32_MIN + 1;
 constexpr int wide_leaf_max = 2;   // leaves per leaf cluster (1, 2, 3, 4, 8 measured: 2 and 3 fastest)
 constexpr int wide_lds_entries = 12;  // per-lane stack entries kept in LDS
+constexpr int path_lane_block = 256;  // threads per path-lane block (the kernel's stack layout assumes it)
 
 constexpr int32_t empty_root = INT32_MIN;
 

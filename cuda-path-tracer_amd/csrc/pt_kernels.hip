@@ -283,18 +283,22 @@ __device__ __forceinline__ void slab_entry4(const float4& LX, const float4& LY, 
 // wins, the reference's tie rule (transport.h:67-68,79).
 // Traversal stack of the wide path: the first wide_lds_entries entries of a
 // lane live in LDS (entry k at lds[k*stride]), deeper ones in a global
-// overflow region laid out so that a wave's accesses are coalesced.
-struct wide_stack {
+// overflow region laid out so that a wave's accesses are coalesced.  STRIDE
+// is the block size when the kernel fixes it (the LDS address is then a
+// shift), 0 when it is known only at run time.
+template<int STRIDE>
+struct wide_stack_t {
     int2 *lds;
     int stride;
     int2 *spill;
     size_t spill_stride;
     uint32_t lane_id;   // grid-wide thread index (grids stay below 2^32 threads)
+    __device__ __forceinline__ int lds_stride() const { return STRIDE ? STRIDE : stride; }
     __device__ __forceinline__ void store(int k, int32_t ref, float entry) {
         if(k < wide_lds_entries) {
             // one 64-bit LDS store (kept apart from the global one below, which
             // the compiler would otherwise merge into a generic-address store)
-            *reinterpret_cast<unsigned long long *>(lds + k * stride) =
+            *reinterpret_cast<unsigned long long *>(lds + k * lds_stride()) =
                 (unsigned long long)uint32_t(ref) | ((unsigned long long)__float_as_uint(entry) << 32);
         } else {
             int2 *slot = spill + size_t(k - wide_lds_entries) * spill_stride + lane_id;
@@ -304,12 +308,13 @@ struct wide_stack {
     }
     __device__ __forceinline__ int2 load(int k) const {
         if(k < wide_lds_entries) {
-            const unsigned long long v = *reinterpret_cast<const unsigned long long *>(lds + k * stride);
+            const unsigned long long v = *reinterpret_cast<const unsigned long long *>(lds + k * lds_stride());
             return make_int2(int32_t(uint32_t(v)), int32_t(uint32_t(v >> 32)));
         }
         return spill[size_t(k - wide_lds_entries) * spill_stride + lane_id];
     }
 };
+using wide_stack = wide_stack_t<0>;
 
 template<bool ANY_HIT, bool STATS>
 __device__ bool traverse_wide(const scene_dev& s, int32_t root, const ray_in& r, float low, float high, wide_stack& stk,
@@ -416,8 +421,8 @@ __device__ __forceinline__ void wide_begin(int32_t root, wide_query& q, bool any
 }
 
 // Returns true when the query has finished.
-template<bool STATS>
-__device__ __forceinline__ bool wide_step(const scene_dev& s, wide_query& q, wide_stack& stk, int& boxes, int& prims,
+template<bool STATS, class STACK>
+__device__ __forceinline__ bool wide_step(const scene_dev& s, wide_query& q, STACK& stk, int& boxes, int& prims,
                                           int& spheres) {
     const float low = 0.001f;
     const ray_in& r = q.r;
@@ -1505,14 +1510,14 @@ __global__ __launch_bounds__(256) void drain_kernel(scene_dev s, frame_params f,
 // 16 and 64 pixels: 2348, 2378 and 2329; groups of 256 or more fall below
 // one pass (the ordering is then too coarse for the tail).
 template<bool BATCH, int MODE>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void path_lanes_kernel(scene_dev s, frame_params f, paths_dev p, loop_ctl *ctl,
+__global__ __launch_bounds__(path_lane_block) __attribute__((amdgpu_waves_per_eu(4))) void path_lanes_kernel(scene_dev s, frame_params f, paths_dev p, loop_ctl *ctl,
                                                           int parity, const int32_t *queue, int shade_batch,
                                                           int trace_steps, int first_samples, uint32_t *costs,
                                                           batch_dev bd) {
     extern __shared__ int32_t lds[];
     const uint32_t n = ctl->n_ext[parity];
     const int lane = __lane_id();
-    wide_stack wstk{reinterpret_cast<int2 *>(lds) + threadIdx.x, int(blockDim.x), p.spill,
+    wide_stack_t<path_lane_block> wstk{reinterpret_cast<int2 *>(lds) + threadIdx.x, int(blockDim.x), p.spill,
                     size_t(gridDim.x) * blockDim.x, blockIdx.x * blockDim.x + threadIdx.x};
     int error = 0, boxes = 0, prims = 0, spheres = 0;
     uint32_t ext = 0, shadow = 0;   // queries this lane began, flushed to the totals before 2^31
@@ -1822,6 +1827,7 @@ void launch_path_lanes(const scene_dev& s, const frame_params& f, const paths_de
     // The kernel keeps wide_lds_entries stack entries per lane in LDS; the
     // size below covers them only for the wide traversal.
     if(!wide_traversal(s, f)) throw std::logic_error("path lanes need the 4-wide traversal");
+    if(block != path_lane_block) throw std::logic_error("path lanes run blocks of path_lane_block threads");
     if(pass.mode == PATHS_FIRST && (!pass.costs || pass.first_samples <= 0))
         throw std::logic_error("first path-lane pass without a cost buffer");
     if(!paths) return;
