@@ -18,16 +18,20 @@
 
 #include <hipcub/hipcub.hpp>
 #include "pt_device.h"
+#include "pt_layout.h"
+#include "pt_libm.h"
 
 namespace {
 
 thread_local std::string g_last_error;
 
-struct status_error : std::runtime_error {
-    pt_status code;
-    status_error(pt_status c, const std::string& m) : std::runtime_error(m), code(c) {}
-};
-[[noreturn]] void fail(pt_status c, const std::string& m) { throw status_error(c, m); }
+using pt::status_error;
+using pt::fail;
+using pt::validate_scene;
+using pt::validate_camera;
+using pt::validate_mode;
+using pt::checked_pixel_count;
+using pt::finite3;
 
 void hip_check(hipError_t e, const char *what) {
     if(e != hipSuccess) fail(PT_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
@@ -75,164 +79,6 @@ private:
     T *ptr = nullptr;
     size_t count = 0;
 };
-
-// ---------------- validation (src/flat/flat_validate.h:10-83) ----------------
-constexpr int stack_limit = 64; // flat_stack_capacity, src/flat/flat_data.h:52
-
-bool finite3(const pt_vec3& v) { return std::isfinite(v.x) && std::isfinite(v.y) && std::isfinite(v.z); }
-float axis(const pt_vec3& v, int a) { return a == 0 ? v.x : a == 1 ? v.y : v.z; }
-bool valid_id(int32_t id, int64_t n) { return id >= 0 && int64_t(id) < n; }
-
-void validate_scene(const pt_scene_desc& s) {
-    auto require = [](bool ok) { if(!ok) fail(PT_ESCENE, "invalid flat scene data"); };
-    require(s.triangle_count >= 0 && s.sphere_count >= 0 && s.node_count >= 0 && s.material_count >= 0 &&
-            s.texture_count >= 0 && s.pixel_bytes >= 0);
-    require((s.triangles || !s.triangle_count) && (s.spheres || !s.sphere_count) && (s.nodes || !s.node_count) &&
-            (s.materials || !s.material_count) && (s.textures || !s.texture_count) && (s.pixels || !s.pixel_bytes));
-    require(s.node_count < int64_t(INT32_MAX) && s.triangle_count < int64_t(INT32_MAX / 2) && s.sphere_count < int64_t(INT32_MAX / 2));
-    for(int64_t i = 0; i < s.texture_count; i++) {
-        const auto& t = s.textures[i];
-        require(finite3(t.color) && finite3(t.scale) && finite3(t.offset));
-        require(t.type >= PT_CONSTANT && t.type <= PT_MAP);
-        if(t.type == PT_IMAGE) {
-            require(t.width > 0 && t.height > 0 && t.byte_offset >= 0);
-            uint64_t end = uint64_t(t.byte_offset) + 3 * uint64_t(t.width) * uint64_t(t.height);
-            require(end <= uint64_t(s.pixel_bytes));
-        } else if(t.type == PT_MAP) {
-            require(valid_id(t.image_id, s.texture_count));
-            require(s.textures[t.image_id].type == PT_IMAGE && (t.clamp == 0 || t.clamp == 1));
-        }
-    }
-    for(int64_t i = 0; i < s.material_count; i++) {
-        const auto& m = s.materials[i];
-        require(m.type >= PT_DIFFUSE && m.type <= PT_LIGHT && finite3(m.albedo));
-        require(std::isfinite(m.fuzz) && std::isfinite(m.ref_idx));
-        if(m.type == PT_DIFFUSE || m.type == PT_LIGHT) require(valid_id(m.texture_id, s.texture_count));
-        if(m.type == PT_GLASS) require(m.ref_idx > 0);
-    }
-    for(int64_t i = 0; i < s.triangle_count; i++) {
-        const auto& t = s.triangles[i];
-        require(valid_id(t.material_id, s.material_count) && (t.smooth == 0 || t.smooth == 1));
-        require(finite3(t.v0) && finite3(t.v1) && finite3(t.v2) && finite3(t.n0) && finite3(t.n1) && finite3(t.n2));
-        require(finite3(t.uv0) && finite3(t.uv1) && finite3(t.uv2));
-    }
-    for(int64_t i = 0; i < s.sphere_count; i++) {
-        const auto& p = s.spheres[i];
-        require(valid_id(p.material_id, s.material_count) && finite3(p.center) && std::isfinite(p.radius) && p.radius > 0);
-    }
-    if(s.root == -1) {
-        require(s.node_count == 0 && s.triangle_count == 0 && s.sphere_count == 0 && s.max_depth == 0);
-        return;
-    }
-    require(valid_id(s.root, s.node_count));
-    std::vector<int> heights(size_t(s.node_count), 0);
-    std::vector<char> active(size_t(s.node_count), 0);
-    int64_t visited = 0;
-    // Iterative post-order walk with the recursive checker's rules.
-    struct frame { int32_t id; int depth; int stage; int left_height; };
-    std::vector<frame> stack;
-    int result = 0;
-    auto enter = [&](int32_t id, int depth) -> bool {
-        require(valid_id(id, s.node_count) && depth <= stack_limit && !active[size_t(id)]);
-        if(heights[size_t(id)]) { require(depth + heights[size_t(id)] - 1 <= stack_limit); result = heights[size_t(id)]; return false; }
-        active[size_t(id)] = 1;
-        visited++;
-        const auto& n = s.nodes[id];
-        require(finite3(n.minimum) && finite3(n.maximum));
-        for(int a = 0; a < 3; a++) require(axis(n.minimum, a) <= axis(n.maximum, a));
-        stack.push_back({id, depth, 0, 0});
-        return true;
-    };
-    if(enter(s.root, 1)) {
-        while(!stack.empty()) {
-            frame& f = stack.back();
-            const auto& n = s.nodes[f.id];
-            if(n.type == PT_BRANCH) {
-                if(f.stage == 0) {
-                    f.stage = 1;
-                    int32_t child = n.left; int d = f.depth + 1;
-                    if(enter(child, d)) continue;
-                    // cached height returned in result
-                }
-                if(f.stage == 1) {
-                    f.left_height = result;
-                    f.stage = 2;
-                    if(n.left != n.right) {
-                        int32_t child = n.right; int d = f.depth + 1;
-                        if(enter(child, d)) continue;
-                    }
-                }
-                // stage 2: right height is in result (or equals left for an alias)
-                int right_height = n.left == n.right ? f.left_height : result;
-                int height = 1 + std::max(f.left_height, right_height);
-                for(int32_t child : {n.left, n.right}) {
-                    const auto& c = s.nodes[child];
-                    for(int a = 0; a < 3; a++)
-                        require(axis(n.minimum, a) <= axis(c.minimum, a) && axis(n.maximum, a) >= axis(c.maximum, a));
-                }
-                active[size_t(f.id)] = 0; heights[size_t(f.id)] = height; result = height;
-                stack.pop_back();
-            } else {
-                if(n.type == PT_TRIANGLE) {
-                    require(valid_id(n.primitive, s.triangle_count));
-                    const auto& t = s.triangles[n.primitive];
-                    for(const pt_vec3& p : {t.v0, t.v1, t.v2})
-                        for(int a = 0; a < 3; a++) require(axis(p, a) >= axis(n.minimum, a) && axis(p, a) <= axis(n.maximum, a));
-                } else if(n.type == PT_SPHERE) {
-                    require(valid_id(n.primitive, s.sphere_count));
-                    const auto& p = s.spheres[n.primitive];
-                    for(int a = 0; a < 3; a++)
-                        require(axis(p.center, a) - p.radius >= axis(n.minimum, a) && axis(p.center, a) + p.radius <= axis(n.maximum, a));
-                } else require(false);
-                active[size_t(f.id)] = 0; heights[size_t(f.id)] = 1; result = 1;
-                stack.pop_back();
-            }
-        }
-    }
-    require(result == s.max_depth && visited == s.node_count);
-}
-
-// src/renderer/render_validation.h:7-22
-size_t checked_pixel_count(int width, int height) {
-    if(width <= 0 || height <= 0 || uint64_t(width) * uint64_t(height) > uint64_t(INT32_MAX))
-        fail(PT_EINVAL, "image dimensions exceed supported pixel count");
-    return size_t(width) * size_t(height);
-}
-void validate_mode(int32_t mode) {
-    if(mode < PT_DEBUG_SHADED || mode > PT_DEBUG_INTERSECTIONS) fail(PT_EINVAL, "invalid render debug mode");
-}
-void validate_camera(const pt_flat_camera& c) {
-    for(const pt_vec3& v : {c.origin, c.lower_left_corner, c.horizontal, c.vertical, c.u, c.v})
-        if(!finite3(v)) fail(PT_EINVAL, "camera contains nonfinite values");
-    if(!std::isfinite(c.lens_radius) || c.lens_radius < 0 || !std::isfinite(c.time0) || !std::isfinite(c.time1) || c.time1 < c.time0)
-        fail(PT_EINVAL, "invalid camera lens/shutter");
-}
-
-// ---------------- light table (src/renderer/scene_view.h:10-28) ----------------
-std::vector<pt::light_rec> scene_lights(const pt_scene_desc& s) {
-    std::vector<pt::light_rec> lights;
-    double area = 0;
-    auto add = [&](int32_t type, int32_t id, double size) {
-        if(!(size > 0) || !std::isfinite(size)) return;
-        area += size;
-        if(!std::isfinite(area) || area > FLT_MAX) fail(PT_EINVAL, "light area exceeds range");
-        lights.push_back({type, id, float(area)});
-    };
-    for(int64_t i = 0; i < s.triangle_count; i++) {
-        const auto& t = s.triangles[i];
-        if(s.materials[t.material_id].type != PT_LIGHT) continue;
-        pt_vec3 e1{t.v1.x - t.v0.x, t.v1.y - t.v0.y, t.v1.z - t.v0.z}, e2{t.v2.x - t.v0.x, t.v2.y - t.v0.y, t.v2.z - t.v0.z};
-        pt_vec3 c{e1.y * e2.z - e1.z * e2.y, -(e1.x * e2.z - e1.z * e2.x), e1.x * e2.y - e1.y * e2.x};
-        float len = std::sqrt(c.x * c.x + c.y * c.y + c.z * c.z);
-        add(PT_TRIANGLE, int32_t(i), 0.5 * len);
-    }
-    for(int64_t i = 0; i < s.sphere_count; i++) {
-        const auto& p = s.spheres[i];
-        if(s.materials[p.material_id].type != PT_LIGHT) continue;
-        add(PT_SPHERE, int32_t(i), 4 * M_PI * double(p.radius) * p.radius);
-    }
-    return lights;
-}
 
 // ---------------- host-libm tables ----------------
 // cos/sin of every angle cosine_direction and sphere-light sampling can form:
@@ -315,147 +161,41 @@ trig_tables device_trig_tables() {
     return t;
 }
 
+// x^5 of the host libm's pow(double, double) over the table domain of
+// pt_libm.h, kept where it differs from the device's correctly rounded value
+// (about 1 input in 2000 with glibc 2.35); per device, once per process.
+struct pow5_device { const int32_t *keys; const double *values; int32_t count; };
+pow5_device device_pow5_table() {
+    static std::mutex lock;
+    static std::map<int, pow5_device> cache;
+    static std::vector<int32_t> keys;
+    static std::vector<double> values;
+    static std::once_flag once;
+    std::call_once(once, [] { pt::build_pow5_table(keys, values); });
+    int device = 0;
+    HIP_CHECK(hipGetDevice(&device));
+    std::lock_guard<std::mutex> guard(lock);
+    auto it = cache.find(device);
+    if(it != cache.end()) return it->second;
+    int32_t *dk = nullptr;
+    double *dv = nullptr;
+    const size_t n = std::max<size_t>(1, keys.size());
+    HIP_CHECK(hipMalloc(reinterpret_cast<void **>(&dk), n * sizeof(int32_t)));
+    HIP_CHECK(hipMalloc(reinterpret_cast<void **>(&dv), n * sizeof(double)));
+    if(!keys.empty()) {
+        HIP_CHECK(hipMemcpy(dk, keys.data(), keys.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+        HIP_CHECK(hipMemcpy(dv, values.data(), values.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    pow5_device d{dk, dv, int32_t(keys.size())};  // kept for the life of the process
+    cache[device] = d;
+    return d;
+}
+
 void require_device() {
     int count = 0;
     hipError_t e = hipGetDeviceCount(&count);
     if(e != hipSuccess) fail(PT_EDEVICE, std::string("HIP device unavailable: ") + hipGetErrorString(e));
     if(count < 1) fail(PT_EDEVICE, "HIP device unavailable: no HIP devices");
-}
-
-// ---------------- 4-wide layout for the shaded path ----------------
-// Collapses the flat binary tree: every wide node takes up to four of the
-// binary descendants (splitting the child of largest surface area first), and
-// every subtree of at most wide_leaf_max leaves becomes a leaf cluster.  Leaf
-// positions follow the depth-first, left-first leaf order of the flat tree, so
-// a subtree's leaves are one contiguous range and a leaf's position is the
-// order in which the reference's traversal would reach it (its tie rank).
-struct wide_layout {
-    std::vector<float4> nodes;   // 8 per node
-    std::vector<float4> leaves;  // 3 per leaf position
-    int32_t root_ref = pt::empty_root;
-    int depth = 0;
-};
-
-// Triangle test data: v0, e1 = v1-v0, e2 = v2-v0 and |e1 x e2| exactly as
-// flat_intersect.h:9-12 computes them, plus 1e-7f*|e1 x e2| (:16).
-std::vector<float4> triangle_test_records(const pt_scene_desc& s) {
-    std::vector<float4> ti(size_t(3 * s.triangle_count));
-    for(int64_t i = 0; i < s.triangle_count; i++) {
-        const auto& t = s.triangles[i];
-        float e1x = t.v1.x - t.v0.x, e1y = t.v1.y - t.v0.y, e1z = t.v1.z - t.v0.z;
-        float e2x = t.v2.x - t.v0.x, e2y = t.v2.y - t.v0.y, e2z = t.v2.z - t.v0.z;
-        float cx = e1y * e2z - e1z * e2y, cy = -(e1x * e2z - e1z * e2x), cz = e1x * e2y - e1y * e2x;
-        float nl = std::sqrt(cx * cx + cy * cy + cz * cz);
-        ti[size_t(3 * i)] = make_float4(t.v0.x, t.v0.y, t.v0.z, 0.0000001f * nl);
-        ti[size_t(3 * i + 1)] = make_float4(e1x, e1y, e1z, nl);
-        ti[size_t(3 * i + 2)] = make_float4(e2x, e2y, e2z, 0);
-    }
-    return ti;
-}
-
-bool build_wide(const pt_scene_desc& s, const std::vector<float4>& ti, wide_layout& out) {
-    if(s.root < 0) return false;
-    const int64_t n = s.node_count;
-    std::vector<int32_t> first(size_t(n), -1), count(size_t(n), 0);
-    std::vector<int32_t> leaf_code;
-    leaf_code.reserve(size_t(s.triangle_count + s.sphere_count));
-    // Iterative depth-first walk, left before right; a node reached twice (a
-    // shared subtree) disables the wide layout.
-    struct item { int32_t node; bool exit; };
-    std::vector<item> stack{{s.root, false}};
-    while(!stack.empty()) {
-        item it = stack.back(); stack.pop_back();
-        const auto& nd = s.nodes[it.node];
-        if(it.exit) { count[size_t(it.node)] = int32_t(leaf_code.size()) - first[size_t(it.node)]; continue; }
-        if(first[size_t(it.node)] >= 0) return false;
-        first[size_t(it.node)] = int32_t(leaf_code.size());
-        if(nd.type == PT_BRANCH) {
-            stack.push_back({it.node, true});
-            if(nd.left != nd.right) stack.push_back({nd.right, false});
-            stack.push_back({nd.left, false});
-        } else {
-            leaf_code.push_back(nd.primitive * 2 + (nd.type == PT_SPHERE ? 1 : 0));
-            count[size_t(it.node)] = 1;
-        }
-    }
-    if(leaf_code.size() >= (size_t(1) << 27)) return false;
-    out.leaves.resize(3 * leaf_code.size());
-    for(size_t k = 0; k < leaf_code.size(); k++) {
-        int32_t code = leaf_code[k], prim = code >> 1;
-        float cf; std::memcpy(&cf, &code, 4);
-        if(code & 1) {
-            const auto& p = s.spheres[prim];
-            out.leaves[3 * k] = make_float4(p.center.x, p.center.y, p.center.z, p.radius);
-            out.leaves[3 * k + 1] = make_float4(0, 0, 0, -1);
-            out.leaves[3 * k + 2] = make_float4(0, 0, 0, cf);
-        } else {
-            out.leaves[3 * k] = ti[size_t(3 * prim)];
-            out.leaves[3 * k + 1] = ti[size_t(3 * prim + 1)];
-            float4 c = ti[size_t(3 * prim + 2)];
-            out.leaves[3 * k + 2] = make_float4(c.x, c.y, c.z, cf);
-        }
-    }
-    auto area = [&](int32_t node) {
-        const auto& b = s.nodes[node];
-        double dx = double(b.maximum.x) - b.minimum.x, dy = double(b.maximum.y) - b.minimum.y, dz = double(b.maximum.z) - b.minimum.z;
-        return dx * dy + dy * dz + dz * dx;
-    };
-    // A cluster of count <= wide_leaf_max leaves from depth-first position first; note that
-    // the first leaf alone encodes as -1, a valid reference.
-    auto cluster_ref = [&](int32_t node) { return ~((first[size_t(node)] << 3) | (count[size_t(node)] - 1)); };
-    bool failed = false;
-    auto expandable = [&](int32_t node) { return s.nodes[node].type == PT_BRANCH && count[size_t(node)] > pt::wide_leaf_max; };
-    std::function<int32_t(int32_t, int)> make = [&](int32_t b, int depth) -> int32_t {
-        if(!expandable(b)) return cluster_ref(b);
-        out.depth = std::max(out.depth, depth);
-        std::vector<int32_t> kids;
-        const auto& nb = s.nodes[b];
-        kids.push_back(nb.left);
-        if(nb.left != nb.right) kids.push_back(nb.right);
-        while(kids.size() < 4) {
-            int best = -1; double best_area = -1;
-            for(size_t k = 0; k < kids.size(); k++)
-                if(expandable(kids[k]) && area(kids[k]) > best_area) { best = int(k); best_area = area(kids[k]); }
-            if(best < 0) break;
-            const auto& nk = s.nodes[kids[size_t(best)]];
-            std::vector<int32_t> repl{nk.left};
-            if(nk.left != nk.right) repl.push_back(nk.right);
-            kids.erase(kids.begin() + best);
-            kids.insert(kids.begin() + best, repl.begin(), repl.end());
-            if(kids.size() > 4) {  // cannot happen: one slot becomes at most two
-                failed = true;
-                return 0;
-            }
-        }
-        size_t w = out.nodes.size() / 8;
-        out.nodes.resize(out.nodes.size() + 8);
-        float lo[3][4], hi[3][4];
-        int32_t ref[4];
-        for(int k = 0; k < 4; k++) {
-            for(int a = 0; a < 3; a++) { lo[a][k] = INFINITY; hi[a][k] = -INFINITY; }
-            ref[k] = pt::wide_empty;
-        }
-        for(size_t k = 0; k < kids.size(); k++) {
-            const auto& c = s.nodes[kids[k]];
-            lo[0][k] = c.minimum.x; lo[1][k] = c.minimum.y; lo[2][k] = c.minimum.z;
-            hi[0][k] = c.maximum.x; hi[1][k] = c.maximum.y; hi[2][k] = c.maximum.z;
-            ref[k] = make(kids[k], depth + 1);
-            if(failed) return 0;
-        }
-        float4 *dst = &out.nodes[8 * w];
-        for(int a = 0; a < 3; a++) {
-            dst[a] = make_float4(lo[a][0], lo[a][1], lo[a][2], lo[a][3]);
-            dst[3 + a] = make_float4(hi[a][0], hi[a][1], hi[a][2], hi[a][3]);
-        }
-        float rf[4];
-        std::memcpy(rf, ref, sizeof rf);
-        dst[6] = make_float4(rf[0], rf[1], rf[2], rf[3]);
-        dst[7] = make_float4(0, 0, 0, 0);
-        return int32_t(w);
-    };
-    out.depth = 1;
-    out.root_ref = make(s.root, 1);
-    return !failed;
 }
 
 // ---------------- uploaded scene (cuda_renderer.cu:36-54) ----------------
@@ -472,91 +212,48 @@ struct device_scene {
     dev_buf<float4> wnodes, leaves;
     pt::scene_dev view{};
     int64_t branch_count = 0;
+    std::vector<int32_t> branch_ids;   // compact branch index of each flat node (binary layout), -1 for leaves
 
     device_scene(const pt_scene_desc& s, bool with_tables, bool with_wide = false, bool with_log2 = true) {
-        // Traversal layout: branch nodes renumbered in array order, children
-        // encoded as branch indices or leaf references.
-        std::vector<int32_t> branch_id(size_t(s.node_count), -1);
-        int64_t branches = 0;
-        for(int64_t i = 0; i < s.node_count; i++) if(s.nodes[i].type == PT_BRANCH) branch_id[size_t(i)] = int32_t(branches++);
-        branch_count = branches;
-        auto ref_of = [&](int32_t node) -> int32_t {
-            const auto& n = s.nodes[node];
-            if(n.type == PT_BRANCH) return branch_id[size_t(node)];
-            return pt::leaf_ref(n.primitive, n.type == PT_SPHERE);
-        };
-        std::vector<float4> bn(size_t(2 * branches));
-        for(int64_t i = 0; i < s.node_count; i++) {
-            const auto& n = s.nodes[i];
-            if(n.type != PT_BRANCH) continue;
-            size_t b = size_t(branch_id[size_t(i)]);
-            int32_t l = ref_of(n.left), r = ref_of(n.right);
-            float lf, rf;
-            std::memcpy(&lf, &l, 4); std::memcpy(&rf, &r, 4);
-            bn[2 * b] = make_float4(n.minimum.x, n.minimum.y, n.minimum.z, n.maximum.x);
-            bn[2 * b + 1] = make_float4(n.maximum.y, n.maximum.z, lf, rf);
-        }
-        std::vector<float4> ti = triangle_test_records(s);
-        // Shading records (see scene_dev::shade_tri).  The outward normal is
-        // cross(e1,e2) scaled by float(1.0/|e1 x e2|) as vec3::operator/=
-        // does (src/core/vec3.h:152-158, flat_intersect.h:28).
-        std::vector<float4> sh(size_t(5 * s.triangle_count));
-        for(int64_t i = 0; i < s.triangle_count; i++) {
-            const auto& t = s.triangles[i];
-            const float4 B = ti[size_t(3 * i + 1)], C = ti[size_t(3 * i + 2)];
-            float cx = B.y * C.z - B.z * C.y, cy = -(B.x * C.z - B.z * C.x), cz = B.x * C.y - B.y * C.x;
-            float k = float(1.0 / double(B.w));
-            float mf, sf;
-            std::memcpy(&mf, &t.material_id, 4);
-            std::memcpy(&sf, &t.smooth, 4);
-            sh[size_t(5 * i)] = make_float4(cx * k, cy * k, cz * k, mf);
-            sh[size_t(5 * i + 1)] = make_float4(t.n0.x, t.n0.y, t.n0.z, sf);
-            sh[size_t(5 * i + 2)] = make_float4(t.n1.x, t.n1.y, t.n1.z, t.uv0.x);
-            sh[size_t(5 * i + 3)] = make_float4(t.n2.x, t.n2.y, t.n2.z, t.uv0.y);
-            sh[size_t(5 * i + 4)] = make_float4(t.uv1.x, t.uv1.y, t.uv2.x, t.uv2.y);
-        }
-        std::vector<float4> si(size_t(s.sphere_count));
-        for(int64_t i = 0; i < s.sphere_count; i++) {
-            const auto& p = s.spheres[i];
-            si[size_t(i)] = make_float4(p.center.x, p.center.y, p.center.z, p.radius);
-        }
-        auto lights_host = scene_lights(s);
-        bnodes.upload(bn.data(), bn.size());
-        tri_isect.upload(ti.data(), ti.size());
-        sph_isect.upload(si.data(), si.size());
-        shade_tri.upload(sh.data(), sh.size());
+        const pt::scene_layout L = pt::build_layout(s, with_wide);
+        branch_ids = L.branch_id;
+        branch_count = int64_t(L.bnodes.size() / 2);
+        bnodes.upload(L.bnodes.data(), L.bnodes.size());
+        tri_isect.upload(L.tri_isect.data(), L.tri_isect.size());
+        sph_isect.upload(L.sph_isect.data(), L.sph_isect.size());
+        shade_tri.upload(L.shade_tri.data(), L.shade_tri.size());
         triangles.upload(s.triangles, size_t(s.triangle_count));
         spheres.upload(s.spheres, size_t(s.sphere_count));
         nodes.upload(s.nodes, size_t(s.node_count));
         materials.upload(s.materials, size_t(s.material_count));
         textures.upload(s.textures, size_t(s.texture_count));
         pixels.upload(s.pixels, size_t(s.pixel_bytes));
-        lights.upload(lights_host.data(), lights_host.size());
+        lights.upload(L.lights.data(), L.lights.size());
         view.bnodes = bnodes.get(); view.tri_isect = tri_isect.get(); view.sph_isect = sph_isect.get();
         view.shade_tri = shade_tri.get();
-        view.root_ref = s.root < 0 ? pt::empty_root : ref_of(s.root);
+        view.root_ref = L.root_ref;
         view.triangles = triangles.get(); view.spheres = spheres.get(); view.nodes = nodes.get();
         view.node_count = int32_t(s.node_count);
         view.materials = materials.get(); view.textures = textures.get(); view.pixels = pixels.get();
-        view.lights = lights.get(); view.light_count = int32_t(lights_host.size());
-        view.light_area = lights_host.empty() ? 0 : lights_host.back().cumulative_area;
-        view.stack_capacity = std::max(2, std::min(stack_limit, int(s.max_depth)));
+        view.lights = lights.get(); view.light_count = int32_t(L.lights.size());
+        view.light_area = L.lights.empty() ? 0 : L.lights.back().cumulative_area;
+        view.stack_capacity = L.stack_capacity;
         view.trig_bits = nullptr; view.trig_rank = nullptr; view.trig_exceptions = nullptr;
         view.log2_table = nullptr; view.log2_count = 0;
         view.wnodes = nullptr; view.leaves = nullptr; view.wroot_ref = pt::empty_root; view.wide_stack_capacity = 0;
-        if(with_wide) {
-            wide_layout w;
-            if(build_wide(s, ti, w)) {
-                wnodes.upload(w.nodes.data(), w.nodes.size());
-                leaves.upload(w.leaves.data(), w.leaves.size());
-                view.wnodes = wnodes.get(); view.leaves = leaves.get();
-                view.wroot_ref = w.root_ref;
-                view.wide_stack_capacity = 3 * w.depth + 1;
-            }
+        if(L.has_wide) {
+            wnodes.upload(L.wide.nodes.data(), L.wide.nodes.size());
+            leaves.upload(L.wide.leaves.data(), L.wide.leaves.size());
+            view.wnodes = wnodes.get(); view.leaves = leaves.get();
+            view.wroot_ref = L.wide.root_ref;
+            view.wide_stack_capacity = L.wide_stack_capacity;
         }
+        view.pow5_keys = nullptr; view.pow5_values = nullptr; view.pow5_count = 0;
         if(with_tables) {
             const trig_tables t = device_trig_tables();
             view.trig_bits = t.bits; view.trig_rank = t.rank; view.trig_exceptions = t.exceptions;
+            const pow5_device p = device_pow5_table();
+            view.pow5_keys = p.keys; view.pow5_values = p.values; view.pow5_count = p.count;
         }
         if(with_tables && with_log2) {
             // log2f(1+n) of the host libm for every count a traversal can reach.
@@ -931,6 +628,7 @@ struct pt_renderer {
             st.box_tests += h.box_tests;
             st.triangle_tests += h.triangle_tests;
             st.sphere_tests += h.sphere_tests;
+            st.retraced_queries += h.retraces;
             total_rays += h.rays;
             failure |= h.failure;
             if(tuning.time_kernels && part->drained) {
@@ -1020,7 +718,7 @@ struct merged_scenes {
     std::vector<pt_flat_material> materials;
     std::vector<pt_flat_texture> textures;
     std::vector<uint8_t> pixels;
-    std::vector<int64_t> tri_offset, sph_offset;
+    std::vector<int64_t> tri_offset, sph_offset, node_offset;
     pt_scene_desc desc() const {
         pt_scene_desc d{};
         d.triangles = triangles.data(); d.triangle_count = int64_t(triangles.size());
@@ -1045,6 +743,7 @@ merged_scenes merge_scenes(const pt_scene_desc *scenes, int32_t count) {
             fail(PT_EINVAL, "batch exceeds 32-bit indices");
         m.tri_offset.push_back(to);
         m.sph_offset.push_back(so);
+        m.node_offset.push_back(no);
         for(int64_t i = 0; i < s.triangle_count; i++) {
             pt_flat_triangle t = s.triangles[i];
             t.material_id += int32_t(mo);
@@ -1236,6 +935,7 @@ struct pt_batch {
         st.iterations = int32_t(h.iterations);
         st.extend_rays = h.ext_rays;
         st.shadow_rays = h.shadow_rays;
+        st.retraced_queries = h.retraces;
         float elapsed = 0;
         HIP_CHECK(hipEventElapsedTime(&elapsed, start, stop));
         milliseconds += elapsed;
@@ -1283,15 +983,17 @@ pt_batch *create_batch(const pt_scene_desc *scenes, int32_t scene_count, const p
     std::vector<float4> wn, lv;
     std::vector<pt::light_rec> lights;
     const size_t nscenes = static_cast<size_t>(scene_count);
-    std::vector<int32_t> roots(nscenes, pt::empty_root), light_offset(nscenes, 0), light_count(nscenes, 0);
+    std::vector<int32_t> roots(nscenes, pt::empty_root), broots(nscenes, pt::empty_root), light_offset(nscenes, 0),
+        light_count(nscenes, 0);
+    int binary_capacity = 2;
     std::vector<float> light_area(nscenes, 0.0f);
     int depth = 1;
     for(int32_t k = 0; k < scene_count; k++) {
         const pt_scene_desc& s = scenes[k];
-        std::vector<float4> ti = triangle_test_records(s);
-        wide_layout w;
+        std::vector<float4> ti = pt::triangle_test_records(s);
+        pt::wide_layout w;
         if(s.root >= 0) {
-            if(!build_wide(s, ti, w)) fail(PT_EINVAL, "batch scenes need tree hierarchies (no shared subtrees)");
+            if(!pt::build_wide(s, ti, w)) fail(PT_EINVAL, "batch scenes need tree hierarchies (no shared subtrees)");
             const int32_t node_offset = int32_t(wn.size() / 8), leaf_offset = int32_t(lv.size() / 3);
             if(lv.size() / 3 + w.leaves.size() / 3 >= (size_t(1) << 27)) fail(PT_EINVAL, "batch exceeds 2^27 leaves");
             for(size_t i = 0; i < w.nodes.size() / 8; i++) {
@@ -1310,11 +1012,17 @@ pt_batch *create_batch(const pt_scene_desc *scenes, int32_t scene_count, const p
                 std::memcpy(cf, &code, 4);
             }
             roots[size_t(k)] = rebase_ref(w.root_ref, node_offset, leaf_offset);
+            // The scene's root in the merged binary layout, for reference-order retraces.
+            const int32_t merged_root = int32_t(m.node_offset[size_t(k)]) + s.root;
+            const pt_flat_node& rn = md.nodes[merged_root];
+            broots[size_t(k)] = rn.type == PT_BRANCH ? b->scene->branch_ids[size_t(merged_root)]
+                                                     : pt::leaf_ref(rn.primitive, rn.type == PT_SPHERE);
+            binary_capacity = std::max(binary_capacity, std::min(pt::stack_limit, int(s.max_depth)));
             wn.insert(wn.end(), w.nodes.begin(), w.nodes.end());
             lv.insert(lv.end(), w.leaves.begin(), w.leaves.end());
             depth = std::max(depth, w.depth);
         }
-        std::vector<pt::light_rec> lk = scene_lights(s);
+        std::vector<pt::light_rec> lk = pt::scene_lights(s);
         light_offset[size_t(k)] = int32_t(lights.size());
         light_count[size_t(k)] = int32_t(lk.size());
         light_area[size_t(k)] = lk.empty() ? 0.0f : lk.back().cumulative_area;
@@ -1330,7 +1038,8 @@ pt_batch *create_batch(const pt_scene_desc *scenes, int32_t scene_count, const p
     sv.wnodes = b->wnodes.get(); sv.leaves = b->leaves.get();
     // Marks the wide layout as present; each query starts at its view's root.
     sv.wroot_ref = roots[0];
-    sv.wide_stack_capacity = 3 * depth + 1;
+    sv.stack_capacity = binary_capacity;
+    sv.wide_stack_capacity = std::max(3 * depth + 1, binary_capacity);
     sv.lights = b->lights.get(); sv.light_count = int32_t(lights.size());
     sv.light_area = 0;
 
@@ -1343,6 +1052,7 @@ pt_batch *create_batch(const pt_scene_desc *scenes, int32_t scene_count, const p
         r.camera = in.camera; r.width = in.width; r.height = in.height;
         r.pixel_offset = offset; r.seed = in.seed;
         r.wroot_ref = roots[size_t(in.scene)];
+        r.broot_ref = broots[size_t(in.scene)];
         r.light_offset = light_offset[size_t(in.scene)]; r.light_count = light_count[size_t(in.scene)];
         r.light_area = light_area[size_t(in.scene)];
         b->host_views.push_back(r);
@@ -1529,6 +1239,37 @@ pt_status pt_query_rays(const pt_scene_desc *scene, const pt_ray *rays, size_t c
 pt_status pt_query_rays_stats(const pt_scene_desc *scene, const pt_ray *rays, size_t count, int32_t brute,
                               pt_trace_result *results, int32_t *boxes, int32_t *primitives) {
     return guarded([&] { query(scene, rays, count, brute, results, boxes, primitives); });
+}
+
+pt_status pt_query_rays_wide(const pt_scene_desc *scene, const pt_ray *rays, size_t count, pt_trace_result *results,
+                             int32_t *retraced) {
+    return guarded([&] {
+        if(!scene) fail(PT_EINVAL, "null scene");
+        validate_scene(*scene);
+        require_device();
+        if(count == 0) return;
+        if(count > size_t(INT32_MAX)) fail(PT_EINVAL, "too many validation rays");
+        if(!rays || !results) fail(PT_EINVAL, "null ray or result buffer");
+        device_scene uploaded(*scene, false, true);
+        dev_buf<pt_ray> input; input.upload(rays, count);
+        dev_buf<pt_trace_result> output(count);
+        dev_buf<int32_t> flags(count);
+        if(uploaded.view.wroot_ref == pt::empty_root) {
+            // No 4-wide layout (an empty scene or shared subtrees): the shaded
+            // path then runs the reference-order traversal too.
+            pt::launch_query(uploaded.view, input.get(), output.get(), nullptr, nullptr, int32_t(count), false, nullptr);
+            HIP_CHECK(hipMemset(flags.get(), 0, count * sizeof(int32_t)));
+        } else {
+            const int block = 256;
+            const size_t threads = (count + block - 1) / block * block;
+            dev_buf<int2> spill(std::max<size_t>(1, threads * size_t(pt::wide_spill_entries(uploaded.view.wide_stack_capacity))));
+            pt::launch_wide_query(uploaded.view, input.get(), output.get(), flags.get(), spill.get(), int32_t(count), block, nullptr);
+        }
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipDeviceSynchronize());
+        HIP_CHECK(hipMemcpy(results, output.get(), count * sizeof(pt_trace_result), hipMemcpyDeviceToHost));
+        if(retraced) HIP_CHECK(hipMemcpy(retraced, flags.get(), count * sizeof(int32_t), hipMemcpyDeviceToHost));
+    });
 }
 
 pt_status pt_renderer_read_state(const pt_renderer *renderer, float *sum, uint64_t *rng_state, size_t pixels) {

@@ -49,6 +49,11 @@ struct scene_dev {
     const float2 *trig_exceptions;
     const float *log2_table;
     int32_t log2_count;
+    // x^5 of the host libm's pow where it differs from the correctly rounded
+    // value (pt_libm.h): table-domain indices, ascending, and host values.
+    const int32_t *pow5_keys;
+    const double *pow5_values;
+    int32_t pow5_count;
     int32_t stack_capacity;    // per-thread traversal stack entries (<= 64)
     // 4-wide layout for the shaded path (nearest-first traversal):
     //   wnodes: 8 float4 per node = lo.x[4] lo.y[4] lo.z[4] hi.x[4] hi.y[4] hi.z[4] ref[4] pad[4]
@@ -60,6 +65,9 @@ struct scene_dev {
     const float4 *wnodes;
     const float4 *leaves;
     int32_t wroot_ref;         // empty_root when the scene is empty or wide traversal is off
+    // Entries of the wide traversal's stack: 3 * (wide depth) + 1, and at
+    // least stack_capacity, since a reference-order retrace (pt_traverse.h)
+    // uses the same stack for its binary entries.
     int32_t wide_stack_capacity;
 };
 
@@ -120,6 +128,9 @@ struct loop_ctl {
     // Test counts of a counting pass (trace_stats of every query, split by
     // primitive kind): box tests, triangle tests, sphere tests.
     unsigned long long box_tests, triangle_tests, sphere_tests;
+    // Closest-hit queries of the 4-wide traversal traced again in the
+    // reference's order (order-sensitive cases, see pt_traverse.h).
+    unsigned long long retraces;
 };
 
 struct frame_params {
@@ -137,6 +148,11 @@ struct frame_params {
 void launch_trig_values(float2 *out, hipStream_t stream);
 void launch_query(const scene_dev& s, const pt_ray *rays, pt_trace_result *out, int32_t *boxes, int32_t *prims,
                   int32_t count, bool brute, hipStream_t stream);
+// Closest hit of each ray through the 4-wide traversal with its
+// reference-order retrace (requires the wide layout); `spill` holds
+// grid * block * wide_spill_entries entries.
+void launch_wide_query(const scene_dev& s, const pt_ray *rays, pt_trace_result *out, int32_t *retraced, int2 *spill,
+                       int32_t count, int block, hipStream_t stream);
 // Batched rendering: many views, each of one scene of a merged scene set
 // (nodes, primitives, materials and lights of all scenes concatenated; each
 // scene keeps its own hierarchy root and light range).  Pixels of all views
@@ -148,6 +164,7 @@ struct view_rec {
     int64_t pixel_offset;      // first pixel of the view in the batch
     uint64_t seed;
     int32_t wroot_ref;         // the view's scene's hierarchy root
+    int32_t broot_ref;         // the same scene's root in the binary layout (reference-order retraces)
     int32_t light_offset, light_count;
     float light_area;
 };

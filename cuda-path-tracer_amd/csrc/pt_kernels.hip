@@ -18,38 +18,13 @@
 #include <stdexcept>
 
 #include "pt_device.h"
+#include "pt_traverse.h"
+#include "pt_libm.h"
 
 namespace pt {
 namespace {
 
 constexpr float PI_F = float(M_PI);
-
-// ---------------- vec3 arithmetic (src/core/vec3.h:49-162) ----------------
-struct v3 { float x, y, z; };
-__device__ __forceinline__ v3 mk(float x, float y, float z) { return v3{x, y, z}; }
-__device__ __forceinline__ v3 mk(const pt_vec3& a) { return v3{a.x, a.y, a.z}; }
-__device__ __forceinline__ v3 add(v3 a, v3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
-__device__ __forceinline__ v3 sub(v3 a, v3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
-__device__ __forceinline__ v3 mul(v3 a, v3 b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
-// operator*(float,vec3) / operator*(vec3,float): t*e[i] (vec3.h:93-103)
-__device__ __forceinline__ v3 scale(float t, v3 v) { return mk(t * v.x, t * v.y, t * v.z); }
-// operator/(vec3,float): per-component division (vec3.h:97-99)
-__device__ __forceinline__ v3 divs(v3 v, float t) { return mk(v.x / t, v.y / t, v.z / t); }
-__device__ __forceinline__ v3 neg(v3 v) { return mk(-v.x, -v.y, -v.z); }
-__device__ __forceinline__ float dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-// vec3.h:109-115 (negated middle difference kept for signed zeros)
-__device__ __forceinline__ v3 cross(v3 a, v3 b) {
-    return mk(a.y * b.z - a.z * b.y, -(a.x * b.z - a.z * b.x), a.x * b.y - a.y * b.x);
-}
-__device__ __forceinline__ float length(v3 v) { return sqrtf(v.x * v.x + v.y * v.y + v.z * v.z); }
-__device__ __forceinline__ float squared_length(v3 v) { return v.x * v.x + v.y * v.y + v.z * v.z; }
-// operator/=(float): multiply by float(1.0/t) computed in double (vec3.h:152-158)
-__device__ __forceinline__ v3 div_assign(v3 v, float t) {
-    float k = float(1.0 / double(t));
-    return mk(v.x * k, v.y * k, v.z * k);
-}
-__device__ __forceinline__ v3 unit_vector(v3 v) { return divs(v, length(v)); }
-__device__ __forceinline__ bool finite3(float x) { return isfinite(x); }
 
 // ---------------- PCG-XSH-RR 64/32 (src/renderer/random.h:8-23) ----------------
 struct rng_t {
@@ -89,198 +64,6 @@ __device__ v3 sample_disk(rng_t& r) {
     return p;
 }
 
-// ---------------- rays ----------------
-struct ray_in {
-    float ox, oy, oz, dx, dy, dz;
-    float ix, iy, iz;   // 1.0f/d per axis (the value trace_box recomputes per node)
-    float dlen;         // r.direction().length() (flat_intersect.h:13)
-};
-__device__ __forceinline__ ray_in setup_ray(float ox, float oy, float oz, float dx, float dy, float dz) {
-    ray_in r;
-    r.ox = ox; r.oy = oy; r.oz = oz; r.dx = dx; r.dy = dy; r.dz = dz;
-    r.ix = 1.0f / dx; r.iy = 1.0f / dy; r.iz = 1.0f / dz;
-    r.dlen = sqrtf(dx * dx + dy * dy + dz * dz);
-    return r;
-}
-
-// nextafterf(x, toward) for finite x != toward.
-__device__ __forceinline__ float next_toward(float x, float toward) {
-    if(!(x < toward) && !(x > toward)) return toward;
-    uint32_t b = __float_as_uint(x);
-    if(x == 0.0f) return x < toward ? __uint_as_float(1u) : __uint_as_float(0x80000001u);
-    bool up = x < toward;
-    bool positive = !(b >> 31);
-    b = (up == positive) ? b + 1 : b - 1;
-    return __uint_as_float(b);
-}
-
-// ---------------- slab test (src/renderer/transport.h:14-31) ----------------
-// All three axes are evaluated with the reference's ternaries; low only grows
-// and high only shrinks, so the reference's early exits cannot change the result.
-__device__ __forceinline__ bool slab(float mn, float mx, float o, float d, float inv, float& low, float& high, bool& ok) {
-    if(d == 0) { ok = ok && !(o < mn || o > mx); return ok; }
-    float a = (mn - o) * inv;
-    float b = (mx - o) * inv;
-    if(inv < 0) { float s = a; a = b; b = s; }
-    low = a > low ? a : low;
-    high = b < high ? b : high;
-    return ok;
-}
-__device__ __forceinline__ bool box_test(const float4& A, const float4& B, const ray_in& r, float low, float high) {
-    bool ok = true;
-    slab(A.x, A.w, r.ox, r.dx, r.ix, low, high, ok);
-    slab(A.y, B.x, r.oy, r.dy, r.iy, low, high, ok);
-    slab(A.z, B.y, r.oz, r.dz, r.iz, low, high, ok);
-    return ok && high > low;
-}
-
-// ---------------- Moller-Trumbore (src/flat/flat_intersect.h:8-27) ----------------
-// A = {v0, 1e-7f*|n|}, B = {e1, |n|}, C = {e2, -}, with e1 = v1-v0, e2 = v2-v0, n = e1 x e2.
-__device__ __forceinline__ bool tri_test(const float4& A, const float4& B, const float4& C, const ray_in& r,
-                                         float t_min, float t_max, float& t, float& u, float& v) {
-    if(!(B.w > 0) || !(r.dlen > 0)) return false;
-    v3 d = mk(r.dx, r.dy, r.dz), e1 = mk(B.x, B.y, B.z), e2 = mk(C.x, C.y, C.z);
-    v3 pvec = cross(d, e2);
-    float det = dot(e1, pvec);
-    if(fabsf(det) <= A.w * r.dlen) return false;
-    float inv_det = 1.0f / det;
-    v3 tvec = sub(mk(r.ox, r.oy, r.oz), mk(A.x, A.y, A.z));
-    u = dot(tvec, pvec) * inv_det;
-    if(!(u >= 0 && u <= 1)) return false;
-    v3 qvec = cross(tvec, e1);
-    v = dot(d, qvec) * inv_det;
-    if(!(v >= 0 && u + v <= 1)) return false;
-    t = dot(e2, qvec) * inv_det;
-    return t > t_min && t < t_max;
-}
-
-// ---------------- sphere (src/flat/flat_intersect.h:49-58) ----------------
-__device__ __forceinline__ bool sphere_test(const float4& S, const ray_in& r, float t_min, float t_max, float& t) {
-    v3 d = mk(r.dx, r.dy, r.dz);
-    v3 oc = sub(mk(r.ox, r.oy, r.oz), mk(S.x, S.y, S.z));
-    float a = dot(d, d);
-    float b = dot(oc, d);
-    float c = dot(oc, oc) - S.w * S.w;
-    float disc = b * b - a * c;
-    if(!(disc > 0)) return false;
-    float root = sqrtf(disc);
-    t = (-b - root) / a;
-    if(!(t < t_max && t > t_min)) t = (-b + root) / a;
-    return t < t_max && t > t_min;
-}
-
-struct best_hit { float t, u, v; int32_t code; };
-
-// ---------------- closest / any hit (src/renderer/transport.h:43-85) ----------------
-// Depth-first, pop then test, right pushed before left, pruning limit
-// nextafterf(closest, high), later leaf wins exact ties.  The stack lives in
-// LDS with a stride of the block size (entry k of a lane at stk[k*stride]).
-template<bool ANY_HIT, bool STATS>
-__device__ bool traverse_bvh(const scene_dev& s, const ray_in& r, float low, float high, int32_t *stk, int stride,
-                             best_hit& best, int& boxes, int& prims, int& spheres, int& error) {
-    if(s.root_ref == empty_root) return false;
-    bool found = false;
-    float closest = high;
-    int count = 0;
-    stk[0] = s.root_ref; count = 1;
-    const int cap = s.stack_capacity;
-    while(count) {
-        --count;
-        int32_t ref = stk[count * stride];
-        float limit = found ? next_toward(closest, high) : high;
-        if(ref >= 0) {
-            if(STATS) boxes++;
-            float4 A = s.bnodes[2 * ref], B = s.bnodes[2 * ref + 1];
-            if(!box_test(A, B, r, low, limit)) continue;
-            int32_t left = __float_as_int(B.z), right = __float_as_int(B.w);
-            int needed = left == right ? 1 : 2;
-            if(count + needed > cap) { error = 1; return false; }
-            if(left != right) { stk[count * stride] = right; count++; }
-            stk[count * stride] = left; count++;
-        } else {
-            if(STATS) prims++;
-            int32_t code = ~ref;
-            int32_t prim = code >> 1;
-            float t, u = 0, v = 0;
-            bool hit;
-            if(STATS && (code & 1)) spheres++;
-            if(code & 1) hit = sphere_test(s.sph_isect[prim], r, low, limit, t);
-            else hit = tri_test(s.tri_isect[3 * prim], s.tri_isect[3 * prim + 1], s.tri_isect[3 * prim + 2], r, low, limit, t, u, v);
-            if(hit && (!found || t <= closest)) {
-                found = true; closest = t;
-                best.t = t; best.u = u; best.v = v; best.code = code;
-                if(ANY_HIT) return true;
-            }
-        }
-    }
-    return found;
-}
-
-// Brute mode (transport.h:48-60): every leaf in node-array order, no pruning.
-template<bool ANY_HIT>
-__device__ bool traverse_brute(const scene_dev& s, const ray_in& r, float low, float high, best_hit& best, int& prims,
-                               int& spheres) {
-    if(s.root_ref == empty_root) return false;
-    bool found = false;
-    float closest = high;
-    for(int32_t i = 0; i < s.node_count; i++) {
-        const pt_flat_node& n = s.nodes[i];
-        if(n.type == PT_BRANCH) continue;
-        prims++;
-        if(n.type == PT_SPHERE) spheres++;
-        float t, u = 0, v = 0;
-        bool hit;
-        int32_t code = n.primitive * 2 + (n.type == PT_SPHERE ? 1 : 0);
-        if(n.type == PT_SPHERE) hit = sphere_test(s.sph_isect[n.primitive], r, low, high, t);
-        else hit = tri_test(s.tri_isect[3 * n.primitive], s.tri_isect[3 * n.primitive + 1], s.tri_isect[3 * n.primitive + 2], r, low, high, t, u, v);
-        if(hit && (!found || t <= closest)) {
-            found = true; closest = t;
-            best.t = t; best.u = u; best.v = v; best.code = code;
-            if(ANY_HIT) return true;
-        }
-    }
-    return found;
-}
-
-// ---------------- 4-wide nearest-first traversal (shaded path) ----------------
-// The four child boxes of a wide node (SoA: child k in component k), with
-// trace_box's arithmetic (transport.h:14-31) and the reference's near/far
-// choice by the sign of 1/d: per axis a = (near - o) * (1/d), b likewise for
-// the far plane, low = a > low ? a : low and high = b < high ? b : high in x,
-// y, z order, the box hit when high > low; returns the entry distance low.
-// fmaxf/fminf give what those ternaries give, since both skip a NaN plane
-// distance, and they let the compiler use 3-input min/max.  One form serves
-// every ray, d == 0 included, where trace_box instead rejects the box if
-// o < min or o > max and leaves low and high alone: 1/d is then +-inf, so
-// with o outside the slab one plane distance is +inf as the near one or
-// -inf as the far one (a miss), with o strictly inside they are -inf and
-// +inf (no bound), and with o on a plane that distance is 0 * inf = NaN
-// (skipped).  (min - o is never 0 for min != o: denormals are kept.)  The
-// sign of a zero distance cannot matter: low starts at 0.001 and a zero
-// high misses either way.
-__device__ __forceinline__ void slab_entry4(const float4& LX, const float4& LY, const float4& LZ, const float4& HX,
-                                            const float4& HY, const float4& HZ, const ray_in& r, float low,
-                                            float high, bool h[4], float e[4]) {
-    const bool sx = r.ix < 0, sy = r.iy < 0, sz = r.iz < 0;
-    const float lx[4] = {LX.x, LX.y, LX.z, LX.w}, ly[4] = {LY.x, LY.y, LY.z, LY.w}, lz[4] = {LZ.x, LZ.y, LZ.z, LZ.w};
-    const float hx[4] = {HX.x, HX.y, HX.z, HX.w}, hy[4] = {HY.x, HY.y, HY.z, HY.w}, hz[4] = {HZ.x, HZ.y, HZ.z, HZ.w};
-#pragma unroll
-    for(int k = 0; k < 4; k++) {
-        const float ax = ((sx ? hx[k] : lx[k]) - r.ox) * r.ix, bx = ((sx ? lx[k] : hx[k]) - r.ox) * r.ix;
-        const float ay = ((sy ? hy[k] : ly[k]) - r.oy) * r.iy, by = ((sy ? ly[k] : hy[k]) - r.oy) * r.iy;
-        const float az = ((sz ? hz[k] : lz[k]) - r.oz) * r.iz, bz = ((sz ? lz[k] : hz[k]) - r.oz) * r.iz;
-        const float lo = fmaxf(fmaxf(fmaxf(low, ax), ay), az);
-        const float hi = fminf(fminf(fminf(high, bx), by), bz);
-        e[k] = lo;
-        h[k] = hi > lo;
-    }
-}
-
-// Closest (or any) hit over the 4-wide layout.  Children are visited nearest
-// first; a stacked child is dropped on pop when its entry distance is not
-// below the current limit, which is exactly the reference's pop-time box
-// test.  Among hits at the same distance the leaf of larger depth-first rank
-// wins, the reference's tie rule (transport.h:67-68,79).
 // Traversal stack of the wide path: the first wide_lds_entries entries of a
 // lane live in LDS (entry k at lds[k*stride]), deeper ones in a global
 // overflow region laid out so that a wave's accesses are coalesced.  STRIDE
@@ -314,216 +97,47 @@ struct wide_stack_t {
         return spill[size_t(k - wide_lds_entries) * spill_stride + lane_id];
     }
 };
+
 using wide_stack = wide_stack_t<0>;
 
-template<bool ANY_HIT, bool STATS>
-__device__ bool traverse_wide(const scene_dev& s, int32_t root, const ray_in& r, float low, float high, wide_stack& stk,
-                              best_hit& best, int& boxes, int& prims, int& spheres, int& error) {
-    bool found = false;
-    float closest = high;
-    int32_t best_rank = -1;
-    int32_t cur = root;
-    int count = 0;
-    const int cap = s.wide_stack_capacity;
-    for(;;) {
-        if(cur >= 0) {
-            const float4 *nd = s.wnodes + 8 * size_t(cur);
-            const float4 LX = nd[0], LY = nd[1], LZ = nd[2], HX = nd[3], HY = nd[4], HZ = nd[5], RF = nd[6];
-            const int32_t r0 = __float_as_int(RF.x), r1 = __float_as_int(RF.y), r2 = __float_as_int(RF.z), r3 = __float_as_int(RF.w);
-            const float limit = found ? next_toward(closest, high) : high;
-            float e[4];
-            int32_t c[4];
-            bool hh[4];
-            slab_entry4(LX, LY, LZ, HX, HY, HZ, r, low, limit, hh, e);
-            bool h0 = (r0 != wide_empty) & hh[0], h1 = (r1 != wide_empty) & hh[1];
-            bool h2 = (r2 != wide_empty) & hh[2], h3 = (r3 != wide_empty) & hh[3];
-            if(STATS) boxes += (r0 != wide_empty) + (r1 != wide_empty) + (r2 != wide_empty) + (r3 != wide_empty);
-            if(!h0) e[0] = INFINITY;
-            if(!h1) e[1] = INFINITY;
-            if(!h2) e[2] = INFINITY;
-            if(!h3) e[3] = INFINITY;
-            c[0] = r0; c[1] = r1; c[2] = r2; c[3] = r3;
-            const int nh = int(h0) + int(h1) + int(h2) + int(h3);
-            if(nh) {
-                if(!ANY_HIT) {
-#define PT_CSWAP(i, j) { const bool sw = e[j] < e[i]; const float ei = sw ? e[j] : e[i], ej = sw ? e[i] : e[j]; \
-                               const int32_t ci = sw ? c[j] : c[i], cj = sw ? c[i] : c[j]; \
-                               e[i] = ei; e[j] = ej; c[i] = ci; c[j] = cj; }
-                    PT_CSWAP(0, 1) PT_CSWAP(2, 3) PT_CSWAP(0, 2) PT_CSWAP(1, 3) PT_CSWAP(1, 2)
-#undef PT_CSWAP
-                } else {
-                    // any order: move the hit children to the front
-                    if(!h0) { if(h1) { e[0] = e[1]; c[0] = c[1]; h1 = false; } else if(h2) { e[0] = e[2]; c[0] = c[2]; h2 = false; } else { e[0] = e[3]; c[0] = c[3]; h3 = false; } }
-                    int k = 1;
-                    if(h1) { e[k] = e[1]; c[k] = c[1]; k++; }
-                    if(h2) { e[k] = e[2]; c[k] = c[2]; k++; }
-                    if(h3) { e[k] = e[3]; c[k] = c[3]; k++; }
-                }
-                if(count + nh - 1 > cap) { error = 1; return false; }
-                // c[nh-1] .. c[1] at count .. count + nh - 2 (c[k] at count + nh - 1 - k)
-                if(nh > 1) stk.store(count + nh - 2, c[1], e[1]);
-                if(nh > 2) stk.store(count + nh - 3, c[2], e[2]);
-                if(nh > 3) stk.store(count, c[3], e[3]);
-                count += nh - 1;
-                cur = c[0];
-                continue;
-            }
-        } else {
-            const int32_t v = ~cur;
-            const int32_t first = v >> 3, n = (v & 7) + 1;
-            for(int j = 0; j < n; j++) {
-                const int32_t k = first + j;
-                const float4 A = s.leaves[3 * size_t(k)], B = s.leaves[3 * size_t(k) + 1], C = s.leaves[3 * size_t(k) + 2];
-                const float limit = found ? next_toward(closest, high) : high;
-                float t, u = 0, vv = 0;
-                bool hit;
-                if(B.w < 0) { hit = sphere_test(A, r, low, limit, t); if(STATS) spheres++; }
-                else hit = tri_test(A, B, C, r, low, limit, t, u, vv);
-                if(STATS) prims++;
-                if(hit && (!found || t < closest || (t == closest && k > best_rank))) {
-                    found = true; closest = t; best_rank = k;
-                    best.t = t; best.u = u; best.v = vv; best.code = __float_as_int(C.w);
-                    if(ANY_HIT) return true;
-                }
-            }
-        }
-        for(;;) {
-            if(count == 0) return found;
-            --count;
-            const int2 top = stk.load(count);
-            const int32_t ref = top.x;
-            const float entry = __int_as_float(top.y);
-            if(found && !(entry < next_toward(closest, high))) continue;
-            cur = ref;
-            break;
-        }
-    }
-}
-
-// ---------------- resumable wide traversal (persistent lanes) ----------------
-// The same traversal as traverse_wide, cut into steps (one wide node or one
-// leaf cluster, then the pop) so that a lane whose query has finished can
-// take a new one while the rest of its wave is still traversing.
-struct wide_query {
-    ray_in r;
-    // limit: the pruning limit, the query's upper bound until the first hit,
-    // then nextafterf(closest, upper bound), which for 0 < closest < upper
-    // bound is the next float above closest.
-    float limit, closest;
-    int32_t cur, count, best_rank;
-    best_hit best;
-    bool found, any;
+// Binary traversal stack in LDS: entry k of a lane at p[k*stride].
+struct lds_stack {
+    int32_t *p;
+    int stride;
+    __device__ __forceinline__ void put(int k, int32_t v) { p[k * stride] = v; }
+    __device__ __forceinline__ int32_t get(int k) const { return p[k * stride]; }
 };
-
-__device__ __forceinline__ void wide_begin(int32_t root, wide_query& q, bool any, float high) {
-    q.limit = high; q.closest = high; q.cur = root; q.count = 0; q.best_rank = -1;
-    q.found = false; q.any = any;
-}
-
-// Returns true when the query has finished.
-template<bool STATS, class STACK>
-__device__ __forceinline__ bool wide_step(const scene_dev& s, wide_query& q, STACK& stk, int& boxes, int& prims,
-                                          int& spheres) {
-    const float low = 0.001f;
-    const ray_in& r = q.r;
-    float limit = q.limit;
-    if(q.cur >= 0) {
-        const float4 *nd = s.wnodes + 8 * size_t(q.cur);
-        const float4 LX = nd[0], LY = nd[1], LZ = nd[2], HX = nd[3], HY = nd[4], HZ = nd[5], RF = nd[6];
-        const int32_t r0 = __float_as_int(RF.x), r1 = __float_as_int(RF.y), r2 = __float_as_int(RF.z), r3 = __float_as_int(RF.w);
-        float e[4];
-        int32_t c[4];
-        bool hh[4];
-        slab_entry4(LX, LY, LZ, HX, HY, HZ, r, low, limit, hh, e);
-        const bool h0 = (r0 != wide_empty) & hh[0], h1 = (r1 != wide_empty) & hh[1];
-        const bool h2 = (r2 != wide_empty) & hh[2], h3 = (r3 != wide_empty) & hh[3];
-        if(STATS) boxes += (r0 != wide_empty) + (r1 != wide_empty) + (r2 != wide_empty) + (r3 != wide_empty);
-        if(!h0) e[0] = INFINITY;
-        if(!h1) e[1] = INFINITY;
-        if(!h2) e[2] = INFINITY;
-        if(!h3) e[3] = INFINITY;
-        c[0] = r0; c[1] = r1; c[2] = r2; c[3] = r3;
-        const int nh = int(h0) + int(h1) + int(h2) + int(h3);
-        if(nh) {
-            // nearest first (any-hit queries too: their order does not change the
-            // answer); swaps as selects, so the network has no branches
-#define PT_CSWAP(i, j) { const bool sw = e[j] < e[i]; const float ei = sw ? e[j] : e[i], ej = sw ? e[i] : e[j]; \
-                               const int32_t ci = sw ? c[j] : c[i], cj = sw ? c[i] : c[j]; \
-                               e[i] = ei; e[j] = ej; c[i] = ci; c[j] = cj; }
-            PT_CSWAP(0, 1) PT_CSWAP(2, 3) PT_CSWAP(0, 2) PT_CSWAP(1, 3) PT_CSWAP(1, 2)
-#undef PT_CSWAP
-            // No overflow test: at a node of depth d (the root at 1) the stack
-            // holds at most the 3 unvisited siblings of each of its d - 1
-            // ancestors, so after these pushes count <= 3d, below the
-            // capacity 3 * (deepest node's depth) + 1 the stack is sized for.
-            // c[nh-1] .. c[1] at count .. count + nh - 2 (c[k] at count + nh - 1 - k)
-            if(nh > 1) stk.store(q.count + nh - 2, c[1], e[1]);
-            if(nh > 2) stk.store(q.count + nh - 3, c[2], e[2]);
-            if(nh > 3) stk.store(q.count, c[3], e[3]);
-            q.count += nh - 1;
-            q.cur = c[0];
-            return false;
-        }
-    } else {
-        const int32_t v = ~q.cur;
-        const int32_t first = v >> 3, n = (v & 7) + 1;
-        // A cluster holds at most two leaves: both records are loaded before
-        // the first test, so that their latencies overlap.
-        static_assert(wide_leaf_max <= 2, "the leaf step loads at most two leaves");
-        const float4 *L = s.leaves + 3 * size_t(first);
-        float4 rec[2][3];
-        rec[0][0] = L[0]; rec[0][1] = L[1]; rec[0][2] = L[2];
-        if(n > 1) { rec[1][0] = L[3]; rec[1][1] = L[4]; rec[1][2] = L[5]; }
-#pragma unroll
-        for(int j = 0; j < 2; j++) {
-            if(j >= n) break;
-            const int32_t k = first + j;
-            const float4 A = rec[j][0], B = rec[j][1], C = rec[j][2];
-            float t, u = 0, vv = 0;
-            bool hit;
-            if(B.w < 0) { hit = sphere_test(A, r, low, limit, t); if(STATS) spheres++; }
-            else hit = tri_test(A, B, C, r, low, limit, t, u, vv);
-            if(STATS) prims++;
-            // A hit has t < limit, so t <= closest (before the first hit closest
-            // is the upper bound and best_rank -1): the later leaf wins a tie.
-            if(hit && (t < q.closest || k > q.best_rank)) {
-                q.found = true; q.closest = t; q.best_rank = k;
-                q.best.t = t; q.best.u = u; q.best.v = vv; q.best.code = __float_as_int(C.w);
-                if(q.any) return true;
-                // next_toward(t, upper bound): a hit has 0.001 < t < limit
-                limit = __uint_as_float(__float_as_uint(t) + 1u);
-                q.limit = limit;
-            }
-        }
-    }
-    for(;;) {
-        if(q.count == 0) return true;
-        --q.count;
-        const int2 top = stk.load(q.count);
-        // Entries were pushed below the limit of their time; before the first
-        // hit the limit is the upper bound, so only entries pushed before a hit
-        // can fail this test.
-        if(!(__int_as_float(top.y) < limit)) continue;
-        q.cur = top.x;
-        return false;
-    }
-}
 
 template<bool ANY_HIT, bool STATS>
 __device__ __forceinline__ bool trace(const scene_dev& s, bool brute, const ray_in& r, float low, float high,
                                       int32_t *stk, int stride, best_hit& best, int& boxes, int& prims, int& error) {
     int spheres = 0;
     if(brute) return traverse_brute<ANY_HIT>(s, r, low, high, best, prims, spheres);
-    return traverse_bvh<ANY_HIT, STATS>(s, r, low, high, stk, stride, best, boxes, prims, spheres, error);
+    lds_stack st{stk, stride};
+    return traverse_bvh<ANY_HIT, STATS>(s, s.root_ref, r, low, high, st, best, boxes, prims, spheres, error);
 }
 template<bool ANY_HIT>
 __device__ __forceinline__ bool trace_counted(const scene_dev& s, bool brute, const ray_in& r, float low, float high,
                                               int32_t *stk, int stride, best_hit& best, int& boxes, int& prims,
                                               int& spheres, int& error) {
     if(brute) return traverse_brute<ANY_HIT>(s, r, low, high, best, prims, spheres);
-    return traverse_bvh<ANY_HIT, true>(s, r, low, high, stk, stride, best, boxes, prims, spheres, error);
+    lds_stack st{stk, stride};
+    return traverse_bvh<ANY_HIT, true>(s, s.root_ref, r, low, high, st, best, boxes, prims, spheres, error);
 }
 
+// The reference-order retrace of a path-lane query (pt_traverse.h), out of
+// line: it is rare, and inlined its registers would add to those of the
+// whole state machine.
+template<int STRIDE>
+__device__ __noinline__ bool retrace_call(const scene_dev& s, int32_t broot, ray_in r, wide_stack_t<STRIDE> stk,
+                                          best_hit *best, int *error) {
+    best_hit b{};
+    int e = 0;
+    const bool found = exact_retrace(s, broot, r, stk, b, e);
+    *best = b;
+    *error |= e;
+    return found;
+}
 __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
     for(int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
     return v;
@@ -704,10 +318,13 @@ __device__ __forceinline__ float hit_light_pdf(const scene_dev& s, float light_a
 // ---------------- BSDF sampling (src/renderer/transport.h:113-155) ----------------
 __device__ __forceinline__ v3 reflect(v3 v, v3 n) { return sub(v, scale(2 * dot(v, n), n)); }
 
-// r0 + (1-r0)*pow(1-cosine, 5) in double, rounded to float; out of line so
-// its double-precision registers do not count against every shading path.
-__device__ __noinline__ float schlick_double(float r0, float cosine) {
-    return float(double(r0) + double(1 - r0) * pow(double(1 - cosine), 5.0));
+// r0 + (1-r0)*pow(1-cosine, 5) in double, rounded to float, with pow the host
+// libm's (transport.h:147; the host build promotes pow(float,int) to double);
+// out of line so its registers do not count against every shading path.
+__device__ __noinline__ float schlick_double(const int32_t *keys, const double *values, int32_t count, float r0,
+                                             float cosine) {
+    const pow5_table table{keys, values, count};
+    return float(double(r0) + double(1 - r0) * libm_pow5(table, 1 - cosine));
 }
 
 __device__ bool scatter(const scene_dev& s, const pt_flat_material& mat, v3 in_d, const hit_rec& rec, rng_t& r,
@@ -744,7 +361,7 @@ __device__ bool scatter(const scene_dev& s, const pt_flat_material& mat, v3 in_d
             float r0 = (1 - mat.ref_idx) / (1 + mat.ref_idx);
             r0 *= r0;
             // The host build promotes pow(float,int) to double (transport.h:147).
-            probability = schlick_double(r0, cosine);
+            probability = schlick_double(s.pow5_keys, s.pow5_values, s.pow5_count, r0, cosine);
         }
         out_d = r.uniform() < probability ? reflected : refracted;
         attenuation = mk(1, 1, 1);
@@ -837,16 +454,9 @@ __global__ void init_pixels_kernel(float *sum, uint64_t *rng, int64_t count, uin
     rng[i] = seed_state(seed, uint64_t(i));
 }
 
-// cuda_renderer.cu:111-120 (query_rays), with optional trace_stats output.
-__global__ __launch_bounds__(256) void query_kernel(scene_dev s, const pt_ray *rays, pt_trace_result *out,
-                                                    int32_t *boxes_out, int32_t *prims_out, int32_t count, int brute) {
-    extern __shared__ int32_t lds[];
-    int32_t i = int32_t(blockIdx.x) * int32_t(blockDim.x) + int32_t(threadIdx.x);
-    if(i >= count) return;
-    pt_ray ray = rays[i];
-    ray_in r = setup_ray(ray.origin.x, ray.origin.y, ray.origin.z, ray.direction.x, ray.direction.y, ray.direction.z);
-    best_hit b; int boxes = 0, prims = 0, error = 0;
-    bool hit = trace<false, true>(s, brute != 0, r, 0.001f, FLT_MAX, lds + threadIdx.x, blockDim.x, b, boxes, prims, error);
+// The trace_result of a closest-hit query (cuda_renderer.cu:116-119): the hit
+// record, or the empty record with hit = 0.
+__device__ pt_trace_result trace_result(const scene_dev& s, const ray_in& r, bool hit, const best_hit& b, int error) {
     pt_trace_result res;
     res.rec.t = 0; res.rec.u = 0; res.rec.v = 0;
     res.rec.p = pt_vec3{0, 0, 0}; res.rec.normal = pt_vec3{0, 0, 0};
@@ -859,9 +469,44 @@ __global__ __launch_bounds__(256) void query_kernel(scene_dev s, const pt_ray *r
     }
     res.hit = (hit && !error) ? 1 : 0;
     res.error = error;
-    out[i] = res;
+    return res;
+}
+
+// cuda_renderer.cu:111-120 (query_rays), with optional trace_stats output.
+__global__ __launch_bounds__(256) void query_kernel(scene_dev s, const pt_ray *rays, pt_trace_result *out,
+                                                    int32_t *boxes_out, int32_t *prims_out, int32_t count, int brute) {
+    extern __shared__ int32_t lds[];
+    int32_t i = int32_t(blockIdx.x) * int32_t(blockDim.x) + int32_t(threadIdx.x);
+    if(i >= count) return;
+    pt_ray ray = rays[i];
+    ray_in r = setup_ray(ray.origin.x, ray.origin.y, ray.origin.z, ray.direction.x, ray.direction.y, ray.direction.z);
+    best_hit b; int boxes = 0, prims = 0, error = 0;
+    bool hit = trace<false, true>(s, brute != 0, r, 0.001f, FLT_MAX, lds + threadIdx.x, blockDim.x, b, boxes, prims, error);
+    out[i] = trace_result(s, r, hit, b, error);
     if(boxes_out) boxes_out[i] = boxes;
     if(prims_out) prims_out[i] = prims;
+}
+
+// The same closest-hit query through the shaded path's traversal: 4-wide,
+// nearest first, with its reference-order retrace (verification entry
+// pt_query_rays_wide; its results equal query_kernel's).  retraced_out[i] is
+// 1 where the retrace ran.  `spill`: wide stack entries beyond LDS.
+__global__ __launch_bounds__(256) void wide_query_kernel(scene_dev s, const pt_ray *rays, pt_trace_result *out,
+                                                         int32_t *retraced_out, int2 *spill, int32_t count) {
+    extern __shared__ int32_t lds[];
+    const int32_t i = int32_t(blockIdx.x) * int32_t(blockDim.x) + int32_t(threadIdx.x);
+    if(i >= count) return;
+    wide_stack wstk{reinterpret_cast<int2 *>(lds) + threadIdx.x, int(blockDim.x), spill,
+                    size_t(gridDim.x) * blockDim.x, uint32_t(i)};
+    const pt_ray ray = rays[i];
+    wide_query q;
+    q.r = setup_ray(ray.origin.x, ray.origin.y, ray.origin.z, ray.direction.x, ray.direction.y, ray.direction.z);
+    wide_begin(s.wroot_ref, q, false, FLT_MAX);
+    int boxes = 0, prims = 0, spheres = 0, error = 0;
+    while(!wide_step<false>(s, q, wstk, boxes, prims, spheres)) {}
+    if(q.retrace) q.found = exact_retrace(s, s.root_ref, q.r, wstk, q.best, error);
+    out[i] = trace_result(s, q.r, q.found, q.best, error);
+    retraced_out[i] = q.retrace ? 1 : 0;
 }
 
 // Primary-only diagnostic views (transport.h:179-187): pixel-parallel, each
@@ -919,6 +564,12 @@ __device__ __forceinline__ int32_t query_root(const scene_dev& s, const batch_de
     if constexpr(BATCH) return b.views[b.pixel_view[pix]].wroot_ref;
     else return s.wroot_ref;
 }
+// The same scene's root in the binary layout (reference-order retraces).
+template<bool BATCH>
+__device__ __forceinline__ int32_t query_broot(const scene_dev& s, const batch_dev& b, int32_t pix) {
+    if constexpr(BATCH) return b.views[b.pixel_view[pix]].broot_ref;
+    else return s.root_ref;
+}
 template<bool BATCH>
 __device__ __forceinline__ light_set pixel_lights(const scene_dev& s, const batch_dev& b, int32_t pix) {
     if constexpr(BATCH) {
@@ -959,7 +610,7 @@ __global__ void start_kernel(scene_dev s, frame_params f, paths_dev p, loop_ctl 
         ctl->n_ext[0] = uint32_t(count); ctl->n_ext[1] = 0; ctl->n_shadow = 0; ctl->n_finish = 0; ctl->work = 0;
         ctl->n_saved[0] = 0; ctl->n_saved[1] = 0; ctl->n_waiting = 0;
         ctl->iterations = 0; ctl->ext_rays = 0; ctl->shadow_rays = 0;
-        ctl->box_tests = 0; ctl->triangle_tests = 0; ctl->sphere_tests = 0;
+        ctl->box_tests = 0; ctl->triangle_tests = 0; ctl->sphere_tests = 0; ctl->retraces = 0;
     }
     if(j >= count) return;
     const int64_t i = ((j / chunk) * parts + part) * chunk + j % chunk;
@@ -1016,7 +667,7 @@ __global__ __launch_bounds__(256) void trace_kernel(scene_dev s, frame_params f,
             ray_in r = setup_ray(o.x, o.y, o.z, d.x, d.y, d.z);
             best_hit b;
             bool hit;
-            if(wide) hit = traverse_wide<false, COUNT>(s, s.wroot_ref, r, 0.001f, FLT_MAX, wstk, b, boxes, prims, spheres, error);
+            if(wide) hit = traverse_wide<false, COUNT>(s, s.wroot_ref, s.root_ref, r, FLT_MAX, wstk, b, boxes, prims, spheres, error);
             else hit = COUNT ? trace_counted<false>(s, brute, r, 0.001f, FLT_MAX, stk, stride, b, boxes, prims, spheres, error)
                              : trace<false, false>(s, brute, r, 0.001f, FLT_MAX, stk, stride, b, boxes, prims, error);
             p.hit[pix] = hit ? make_int4(__float_as_int(b.t), __float_as_int(b.u), __float_as_int(b.v), b.code)
@@ -1027,7 +678,7 @@ __global__ __launch_bounds__(256) void trace_kernel(scene_dev s, frame_params f,
             ray_in r = setup_ray(o.x, o.y, o.z, d.x, d.y, d.z);
             best_hit b;
             bool blocked;
-            if(wide) blocked = traverse_wide<true, COUNT>(s, s.wroot_ref, r, 0.001f, o.w, wstk, b, boxes, prims, spheres, error);
+            if(wide) blocked = traverse_wide<true, COUNT>(s, s.wroot_ref, s.root_ref, r, o.w, wstk, b, boxes, prims, spheres, error);
             else blocked = COUNT ? trace_counted<true>(s, brute, r, 0.001f, o.w, stk, stride, b, boxes, prims, spheres, error)
                                  : trace<true, false>(s, brute, r, 0.001f, o.w, stk, stride, b, boxes, prims, error);
             if(!blocked) {
@@ -1103,16 +754,17 @@ __global__ __launch_bounds__(256) void trace_lanes_kernel(scene_dev s, frame_par
                     q.r = setup_ray(o.x, o.y, o.z, d.x, d.y, d.z);
                     const float high = any ? o.w : FLT_MAX;
                     wide_begin(query_root<BATCH>(s, bd, pix), q, any, high);
-                    q.cur = h.y; q.count = h.z; q.best_rank = h.w; q.found = h.w >= 0;
+                    // The saved stack ends with the node the query was at and its entry distance.
+                    q.count = h.z - 1; q.best_rank = h.w; q.found = h.w >= 0;
                     if(q.found) {
                         const int4 bb = in.best[item];
-                        q.closest = __int_as_float(bb.x);
-                        q.limit = next_toward(q.closest, high);
                         q.best.t = __int_as_float(bb.x); q.best.u = __int_as_float(bb.y);
                         q.best.v = __int_as_float(bb.z); q.best.code = bb.w;
+                        q.limit = next_toward(q.best.t, high);
                     }
                     const int2 *src = in.stack + size_t(item) * size_t(p.save_stride);
                     for(int k = 0; k < q.count; k++) { const int2 e = src[k]; wstk.store(k, e.x, __int_as_float(e.y)); }
+                    q.cur = src[q.count].x; q.entry = __int_as_float(src[q.count].y);
                     active = true;
                 } else if(item < ext_end) {
                     const int32_t v = queue[item - n_resume];
@@ -1147,11 +799,12 @@ __global__ __launch_bounds__(256) void trace_lanes_kernel(scene_dev s, frame_par
                 base = __shfl(base, 0);
                 const uint32_t slot = base + uint32_t(__popcll(hand & ((1ull << lane) - 1ull)));
                 if(active && slot < uint32_t(p.save_capacity)) {  // beyond the capacity a query runs on
-                    out.head[slot] = make_int4(q.any ? ~pix : pix, q.cur, q.count, q.found ? q.best_rank : -1);
+                    out.head[slot] = make_int4(q.any ? ~pix : pix, q.cur, q.count + 1, q.found ? q.best_rank : -1);
                     out.best[slot] = make_int4(__float_as_int(q.best.t), __float_as_int(q.best.u),
                                                __float_as_int(q.best.v), q.best.code);
                     int2 *dst = out.stack + size_t(slot) * size_t(p.save_stride);
                     for(int k = 0; k < q.count; k++) dst[k] = wstk.load(k);
+                    dst[q.count] = make_int2(q.cur, __float_as_int(q.entry));
                     if(q.any) p.sh_d[pix].w = 1.0f;
                     else p.ray_d[pix].w = 1.0f;
                     active = false;
@@ -1160,6 +813,10 @@ __global__ __launch_bounds__(256) void trace_lanes_kernel(scene_dev s, frame_par
         }
         if(active && wide_step<COUNT>(s, q, wstk, boxes, prims, spheres)) {
             active = false;
+            if(q.retrace) {
+                q.found = exact_retrace(s, query_broot<BATCH>(s, bd, pix), q.r, wstk, q.best, error);
+                atomicAdd(&ctl->retraces, 1ull);
+            }
             if(!q.any) {
                 p.hit[pix] = q.found ? make_int4(__float_as_int(q.best.t), __float_as_int(q.best.u),
                                                  __float_as_int(q.best.v), q.best.code)
@@ -1352,13 +1009,17 @@ __global__ __launch_bounds__(1024) void shade_kernel(scene_dev s, frame_params f
     }
 }
 
+// One query of a pixel of the drain kernels: 4-wide (root, with broot for a
+// reference-order retrace) or the reference-order / brute traversal.
 template<bool ANY_HIT>
-__device__ __forceinline__ bool trace_query(const scene_dev& s, int32_t root, bool brute, bool wide, const ray_in& r,
-                                            float high, int32_t *stk, int stride, wide_stack& wstk, best_hit& b,
-                                            int& error) {
+__device__ __forceinline__ bool trace_query(const scene_dev& s, int32_t root, int32_t broot, bool brute, bool wide,
+                                            const ray_in& r, float high, int32_t *stk, int stride, wide_stack& wstk,
+                                            best_hit& b, int& error) {
     int boxes = 0, prims = 0, spheres = 0;
-    if(wide) return traverse_wide<ANY_HIT, false>(s, root, r, 0.001f, high, wstk, b, boxes, prims, spheres, error);
-    return trace<ANY_HIT, false>(s, brute, r, 0.001f, high, stk, stride, b, boxes, prims, error);
+    if(wide) return traverse_wide<ANY_HIT, false>(s, root, broot, r, high, wstk, b, boxes, prims, spheres, error);
+    if(brute) return traverse_brute<ANY_HIT>(s, r, 0.001f, high, b, prims, spheres);
+    lds_stack st{stk, stride};
+    return traverse_bvh<ANY_HIT, false>(s, broot, r, 0.001f, high, st, b, boxes, prims, spheres, error);
 }
 
 // Tail of an add_samples call, part 1: the shadow queries still pending
@@ -1380,7 +1041,7 @@ __global__ __launch_bounds__(256) void drain_shadow_kernel(scene_dev s, frame_pa
         const int32_t pix = p.shadow_queue[item];
         const float4 o = p.sh_o[pix], d = p.sh_d[pix];
         best_hit b;
-        if(!trace_query<true>(s, query_root<BATCH>(s, bd, pix), brute, wide, setup_ray(o.x, o.y, o.z, d.x, d.y, d.z), o.w,
+        if(!trace_query<true>(s, query_root<BATCH>(s, bd, pix), query_broot<BATCH>(s, bd, pix), brute, wide, setup_ray(o.x, o.y, o.z, d.x, d.y, d.z), o.w,
                               stk, stride, wstk, b, error)) {
             float4 c = p.sh_c[pix], ra = p.rad[pix];
             ra.x = ra.x + c.x; ra.y = ra.y + c.y; ra.z = ra.z + c.z;
@@ -1397,7 +1058,7 @@ __global__ __launch_bounds__(256) void drain_shadow_kernel(scene_dev s, frame_pa
         const int32_t pix = ~v;
         const float4 o = p.sh_o[pix], d = p.sh_d[pix];
         best_hit b;
-        if(!trace_query<true>(s, query_root<BATCH>(s, bd, pix), brute, wide, setup_ray(o.x, o.y, o.z, d.x, d.y, d.z), o.w,
+        if(!trace_query<true>(s, query_root<BATCH>(s, bd, pix), query_broot<BATCH>(s, bd, pix), brute, wide, setup_ray(o.x, o.y, o.z, d.x, d.y, d.z), o.w,
                               stk, stride, wstk, b, error)) {
             float4 c = p.sh_c[pix], ra = p.rad[pix];
             ra.x = ra.x + c.x; ra.y = ra.y + c.y; ra.z = ra.z + c.z;
@@ -1440,12 +1101,12 @@ __global__ __launch_bounds__(256) void drain_kernel(scene_dev s, frame_params f,
         path_state st = load_path(p, pix);
         p.ray_d[pix].w = 0;
         rng_t r = pixel_rng(p.rng[pix], local_index<BATCH>(bd, pix));
-        const int32_t root = query_root<BATCH>(s, bd, pix);
+        const int32_t root = query_root<BATCH>(s, bd, pix), broot = query_broot<BATCH>(s, bd, pix);
         const light_set ls = pixel_lights<BATCH>(s, bd, pix);
         float *sum = p.sum + 3 * size_t(pix);
         for(;;) {
             best_hit b;
-            const bool hit = trace_query<false>(s, root, brute, wide, setup_ray(st.o.x, st.o.y, st.o.z, st.d.x, st.d.y, st.d.z),
+            const bool hit = trace_query<false>(s, root, broot, brute, wide, setup_ray(st.o.x, st.o.y, st.o.z, st.d.x, st.d.y, st.d.z),
                                                 FLT_MAX, stk, stride, wstk, b, error);
             ext++;
             shadow_query sq;
@@ -1454,7 +1115,7 @@ __global__ __launch_bounds__(256) void drain_kernel(scene_dev s, frame_params f,
             if(has_shadow) {
                 shadow++;
                 best_hit bb;
-                if(!trace_query<true>(s, root, brute, wide, setup_ray(sq.o.x, sq.o.y, sq.o.z, sq.d.x, sq.d.y, sq.d.z),
+                if(!trace_query<true>(s, root, broot, brute, wide, setup_ray(sq.o.x, sq.o.y, sq.o.z, sq.d.x, sq.d.y, sq.d.z),
                                       sq.t_max, stk, stride, wstk, bb, error))
                     st.radiance = add(st.radiance, sq.contribution);
             }
@@ -1527,7 +1188,7 @@ __global__ __launch_bounds__(path_lane_block) __attribute__((amdgpu_waves_per_eu
         atomicAdd(&ctl->shadow_rays, (unsigned long long)shadow);
         ext = shadow = 0;
     };
-    enum : int { IDLE = 0, TRACE = 1, SHADE = 2 };
+    enum : int { IDLE = 0, TRACE = 1, SHADE = 2, RETRACE = 3 };
     int phase = IDLE;
     bool exhausted = false, cont_after = false;
     int32_t pix = 0, root = 0;
@@ -1594,6 +1255,17 @@ __global__ __launch_bounds__(path_lane_block) __attribute__((amdgpu_waves_per_eu
             }
         }
         if constexpr(MODE == PATHS_FIRST) cost += phase != IDLE ? 1u : 0u;
+        // Rare: closest-hit queries that stopped at an order-sensitive prune
+        // are traced again in the reference's order, here rather than in the
+        // traversal steps below, so that the binary traversal's registers do
+        // not add to theirs.
+        if(__ballot(phase == RETRACE)) {
+            if(phase == RETRACE) {
+                q.found = retrace_call(s, query_broot<BATCH>(s, bd, pix), q.r, wstk, &q.best, &error);
+                phase = SHADE;
+                atomicAdd(&ctl->retraces, 1ull);
+            }
+        }
         const uint64_t tracing = __ballot(phase == TRACE), ready = __ballot(phase == SHADE);
         if((tracing | ready) == 0) {
             if(exhausted) break;
@@ -1629,7 +1301,9 @@ __global__ __launch_bounds__(path_lane_block) __attribute__((amdgpu_waves_per_eu
                 const bool leaves = 2 * __popcll(at_leaf) >= __popcll(tr);
                 if(phase != TRACE || (q.cur < 0) != leaves) continue;
                 if(!wide_step<false>(s, q, wstk, boxes, prims, spheres)) continue;
-                if(!q.any) {
+                if(q.retrace) {
+                    phase = RETRACE;
+                } else if(!q.any) {
                     phase = SHADE;
                 } else {
                     if(!q.found) st.radiance = add(st.radiance, contribution);
@@ -1712,6 +1386,12 @@ void launch_query(const scene_dev& s, const pt_ray *rays, pt_trace_result *out, 
     unsigned grid = unsigned((count + block - 1) / block);
     hipLaunchKernelGGL(query_kernel, dim3(grid), dim3(block), stack_bytes(s.stack_capacity, block), stream,
                        s, rays, out, boxes, prims, count, brute ? 1 : 0);
+}
+void launch_wide_query(const scene_dev& s, const pt_ray *rays, pt_trace_result *out, int32_t *retraced, int2 *spill,
+                       int32_t count, int block, hipStream_t stream) {
+    const unsigned grid = unsigned((count + block - 1) / block);
+    if(grid) hipLaunchKernelGGL(wide_query_kernel, dim3(grid), dim3(unsigned(block)), wide_stack_bytes(s.wide_stack_capacity, block),
+                                stream, s, rays, out, retraced, spill, count);
 }
 void launch_init_pixels(float *sum, uint64_t *rng, int64_t count, uint64_t seed, loop_ctl *ctl, hipStream_t stream) {
     unsigned grid = unsigned((count + 255) / 256);

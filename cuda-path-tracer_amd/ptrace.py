@@ -99,7 +99,8 @@ class LaunchStats(ctypes.Structure):
                 ("shade_ms", ctypes.c_float), ("extend_rays", ctypes.c_uint64), ("shadow_rays", ctypes.c_uint64),
                 ("regen_ms", ctypes.c_float),
                 ("box_tests", ctypes.c_uint64), ("triangle_tests", ctypes.c_uint64), ("sphere_tests", ctypes.c_uint64),
-                ("drained_paths", ctypes.c_uint64), ("drain_ms", ctypes.c_float), ("split_launches", ctypes.c_int32)]
+                ("drained_paths", ctypes.c_uint64), ("drain_ms", ctypes.c_float), ("split_launches", ctypes.c_int32),
+                ("retraced_queries", ctypes.c_uint64)]
 
 
 class Tuning(ctypes.Structure):
@@ -149,6 +150,8 @@ PT_FUNCTIONS = {
     "pt_query_rays": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int32, ctypes.c_void_p]),
     "pt_query_rays_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int32,
                                            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "pt_query_rays_wide": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p,
+                                          ctypes.c_void_p]),
     "pt_renderer_read_state": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]),
     "pt_renderer_write_state": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
                                                ctypes.c_int32, ctypes.c_uint64]),
@@ -450,6 +453,19 @@ def query_rays(scene, rays, brute=False, stats=False):
         return out, boxes, prims
     _check(lib().pt_query_rays(ctypes.byref(desc), _ptr(rays), len(rays), int(bool(brute)), _ptr(out)))
     return out
+
+
+def query_rays_wide(scene, rays):
+    """(results, retraced): the closest hits as the shaded path's 4-wide
+    traversal finds them (equal to query_rays'), and where its
+    reference-order retrace ran."""
+    scene = scene.normalized()
+    rays = np.ascontiguousarray(rays, RAY).reshape(-1)
+    out = np.zeros(len(rays), TRACE_RESULT)
+    flags = np.zeros(len(rays), np.int32)
+    desc = scene_desc(scene)
+    _check(lib().pt_query_rays_wide(ctypes.byref(desc), _ptr(rays), len(rays), _ptr(out), _ptr(flags)))
+    return out, flags.astype(bool)
 
 
 # ---------------------------------------------------------------- scenes

@@ -145,6 +145,13 @@ pt_status pt_query_rays(const pt_scene_desc *scene, const pt_ray *rays, size_t c
 pt_status pt_query_rays_stats(const pt_scene_desc *scene, const pt_ray *rays, size_t count, int32_t brute,
                               pt_trace_result *results, int32_t *boxes, int32_t *primitives);
 
+/* Closest hit of each ray as the shaded path finds it (no counterpart in the
+ * reference; for verification): the 4-wide nearest-first traversal with its
+ * reference-order retrace.  Results equal pt_query_rays' bit for bit;
+ * retraced[i] (optional) is 1 where the retrace ran.  count may be 0. */
+pt_status pt_query_rays_wide(const pt_scene_desc *scene, const pt_ray *rays, size_t count, pt_trace_result *results,
+                             int32_t *retraced);
+
 /* Raw accumulation state, the complete resumable state of a renderer
  * (cuda_renderer.cu:145-155): sum (3 floats per pixel) and PCG state. */
 pt_status pt_renderer_read_state(const pt_renderer *renderer, float *sum, uint64_t *rng_state, size_t pixels);
@@ -167,6 +174,9 @@ typedef struct pt_launch_stats {
     uint64_t drained_paths;  /* paths finished on path lanes (or the drain kernel) */
     float drain_ms;
     int32_t split_launches;  /* path-lane launches run in two passes (pt_tuning.path_first_samples) */
+    uint64_t retraced_queries; /* closest-hit queries of the 4-wide traversal traced again in the reference's
+                                  order (order-sensitive cases); counted by the path-lane and persistent-lane
+                                  trace kernels */
 } pt_launch_stats;
 pt_status pt_renderer_last_launch_stats(const pt_renderer *renderer, pt_launch_stats *stats);
 

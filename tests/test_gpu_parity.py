@@ -2,12 +2,13 @@
 
 Bars (written per test):
   * closest-hit records, traversal counts and the diagnostic views: bit-exact;
-  * sphere texture coordinates: within 1.2e-7 absolute (the reference's host
-    build calls glibc atan2f/asinf, the device evaluates atan2/asin in double
-    and rounds; one float ulp of an angle below pi, divided by 2*pi or pi);
-  * shaded images: bit-exact on every pixel whose path never evaluates the
-    double pow of the glass branch differently -- asserted as >= 99.9% of
-    pixels identical and relative L2 < 1e-6 overall; ray counts exact.
+  * sphere texture coordinates in hit records: within 1.2e-7 absolute (the
+    reference's host build calls glibc atan2f/asinf, the device evaluates
+    atan2/asin in double and rounds; one float ulp of an angle below pi,
+    divided by 2*pi or pi).  They feed image textures on spheres only, which
+    none of the shaded scenes below has;
+  * shaded images, accumulation sums, PCG states, RGBA bytes and ray counts:
+    bit-exact on every pixel.
 """
 import numpy as np
 import pytest
@@ -151,23 +152,16 @@ def test_render_matches_reference_fixture(name, w, h, seed, batches):
     for key in sorted(k for k in g if k.startswith("image_m")):
         mode, dl = int(key[7]), bool(int(key[-1]))
         r = _render(scene, cam, w, h, seed, batches, mode, dl)
-        img = r.image()
         assert r.rays_traced() == int(g[key.replace("image", "rays")]), key
-        if mode != 0:
-            assert np.array_equal(img, g[key]), key
-        else:
-            frac, rel = _image_close(img, g[key])
-            assert frac >= 0.999 and rel < 1e-6, (key, frac, rel)
+        assert np.array_equal(r.image(), g[key]), key
 
 
 def test_sphere_lens_scene_and_brute_mode():
     scene, cam, g = golden("spheres.npz")
     r = _render(scene, cam, 31, 21, 19, [16, 48], 0, False)
-    frac, rel = _image_close(r.image(), g["image_m0_d0"])
-    assert frac >= 0.999 and rel < 1e-6
+    assert np.array_equal(r.image(), g["image_m0_d0"])
     b = _render(scene, cam, 31, 21, 19, [64], 0, False, brute=True)
-    frac, rel = _image_close(b.image(), r.image())
-    assert frac >= 0.999 and rel < 1e-6
+    assert np.array_equal(b.image(), r.image()) and b.rays_traced() == r.rays_traced()
     for mode in range(1, 6):
         d = _render(scene, cam, 31, 21, 19, [64], mode, False)
         assert np.array_equal(d.image(), g[f"image_m{mode}_d0"]), mode
@@ -203,25 +197,26 @@ def test_config1_room_matches_oracle_state():
     o = oracle.Renderer(scene, cam, 320, 200, 7, False, 0, True, 8)
     o.add_samples(16, threads=16)
     gs, grng = r.read_state()
-    same = (gs == o.sums()).all(axis=1)
-    assert r.rays_traced() >= o.rays_traced() * 0.999
-    assert same.mean() >= 0.999, same.mean()
-    assert (grng == o.rng_state()).mean() >= 0.999
-    frac, rel = _image_close(r.image(), o.image())
-    assert rel < 1e-4
-    assert np.array_equal(r.rgba_image()[4 * np.nonzero(same)[0][:100]], o.rgba_image()[4 * np.nonzero(same)[0][:100]])
+    assert r.rays_traced() == o.rays_traced()
+    assert np.array_equal(gs, o.sums())
+    assert np.array_equal(grng, o.rng_state())
+    assert np.array_equal(r.image(), o.image())
+    assert np.array_equal(r.rgba_image(), o.rgba_image())
 
 
 def test_direct_lighting_only_per_pixel():
     # max_depth = 1: NEE at the first hit plus the BSDF-sampled light hit.
-    # Per-pixel radiance within 1e-5 relative of the oracle.
+    # The bar of the north star is 1e-5 relative per pixel (and exactly 0
+    # where the oracle is 0); the renderer meets it with equality.
     scene, cam = ptrace.cornell(11, 160, 100)
     r = _render(scene, cam, 160, 100, 3, [8], 0, True, max_depth=1)
     o = oracle.Renderer(scene, cam, 160, 100, 3, False, 0, True, 1)
     o.add_samples(8, threads=16)
     got, want = r.image().astype(np.float64), o.image().astype(np.float64)
     rel = np.abs(got - want) / np.maximum(np.abs(want), 1e-30)
-    assert (rel[want > 0] <= 1e-5).all()
+    assert (rel[want != 0] <= 1e-5).all()
+    assert (got[want == 0] == 0).all()
+    assert np.array_equal(r.image(), o.image())
     assert r.rays_traced() == o.rays_traced()
 
 
@@ -230,13 +225,9 @@ def test_rgba_and_image_division_match_oracle():
     r = _render(scene, cam, 40, 25, 7, [3], 0, True)
     o = oracle.Renderer(scene, cam, 40, 25, 7, False, 0, True, 50)
     o.add_samples(3)
-    same = (r.read_state()[0] == o.sums()).all(axis=1)
-    px = np.nonzero(same)[0]
-    assert len(px) > 900
-    gi, oi = r.image(), o.image()
-    assert np.array_equal(gi[px], oi[px])
-    gr, orgba = r.rgba_image().reshape(-1, 4), o.rgba_image().reshape(-1, 4)
-    assert np.array_equal(gr[px], orgba[px])
+    assert np.array_equal(r.read_state()[0], o.sums())
+    assert np.array_equal(r.image(), o.image())
+    assert np.array_equal(r.rgba_image(), o.rgba_image())
 
 
 def test_error_paths_and_ownership():
