@@ -87,11 +87,11 @@ def test_wide_query_on_benchmark_mesh():
 
 @pytest.mark.parametrize("seed", [0, 1])
 def test_render_of_order_sensitive_scene_matches_oracle(seed):
-    # Camera rays and bounces over coplanar triangles in tight boxes: the
-    # path-lane kernel retraces queries in the reference's order and the whole
-    # render equals the oracle.
-    scene, cam = tie_scenes.lit_grid_scene(6, 1e-7, seed)
-    w, h = 96, 72
+    # Camera rays landing on the shared edges of coplanar triangles in tight
+    # boxes, and their bounces: the path-lane kernel retraces queries in the
+    # reference's order and the whole render equals the oracle.
+    scene, cam = tie_scenes.fine_grid_scene(64, seed)
+    w, h = 96, 96
     r = ptrace.Renderer(scene, cam, w, h, 3 + seed, False, 0, True, 8)
     r.add_samples(3)
     r.add_samples(5)

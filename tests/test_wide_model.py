@@ -92,6 +92,25 @@ def test_fallback_makes_coplanar_edge_ties_follow_the_reference(seed):
     assert differ > 0
 
 
+def test_camera_rays_on_shared_edges():
+    # The render-side case: camera rays of fine_grid_scene land exactly on the
+    # grid's edges; without the fallback some pick another tied triangle.
+    differ = 0
+    for seed in range(3):
+        scene, cam = tie_scenes.fine_grid_scene(64, seed)
+        oracle.validate(scene)
+        rays = tie_scenes.camera_rays(cam, 96, 96, 50000, seed)
+        ref = _oracle(scene, rays)
+        assert (ref[:, 3] >= 0).all()
+        hits, stats, overflows = wm.trace(scene, rays, wm.WIDE)
+        assert _same(hits, ref).all()
+        assert stats[:, 3].sum() > 100
+        _check_stack(scene, stats, overflows)
+        bare, _, _ = wm.trace(scene, rays, wm.WIDE_NO_FALLBACK)
+        differ += int((~_same(bare, ref)).sum())
+    assert differ > 0
+
+
 def test_flat_leaf_boxes_do_not_hide_hits():
     # pad 0: branch boxes of coplanar triangles are flat too, so the reference
     # finds nothing through them; the 4-wide traversal must agree (a flat box

@@ -16,6 +16,8 @@ These scenes make such cases common:
   edge two triangles share and hit both at the same distance.
 - sphere_scene: spheres with tight leaf boxes (center +- radius, as the
   reference's own builder makes them) and grazing rays.
+- fine_grid_scene: the same kind of grid at the scale of float ulps, seen by a
+  camera whose rays land exactly on its edges, for renders.
 - chain_scene: a maximally unbalanced hierarchy (every branch has a leaf and a
   subtree), for stack occupancy.
 
@@ -95,28 +97,70 @@ def grid_scene(n=6, pad=1e-6, seed=0):
     return ptrace.FlatScene(t, np.zeros(0, ptrace.SPHERE), nodes, m, x, np.zeros(0, np.uint8), root, height)
 
 
-def lit_grid_scene(n=6, pad=1e-7, seed=0):
-    """grid_scene with a 2-triangle area light (emission 15) at z = 3 over the
-    grid, in the same hierarchy, and a camera looking down at the grid."""
-    base = grid_scene(n, pad, seed)
-    rng = np.random.default_rng(seed + 1000)
-    light = np.zeros(2, ptrace.TRIANGLE)
-    a, b, c, d = (1, 1, 3), (n - 1, 1, 3), (n - 1, n - 1, 3), (1, n - 1, 3)
-    light[0]["v0"], light[0]["v1"], light[0]["v2"] = a, c, b
-    light[1]["v0"], light[1]["v1"], light[1]["v2"] = a, d, c
-    light["material_id"] = 1
-    t = np.concatenate([base.triangles, light])
+def fine_grid_scene(n=64, seed=0):
+    """Camera rays that land on shared edges: a grid of n x n squares of side
+    2^-20 (8 float ulps at 1.0) in the plane z = 1, two triangles each, with
+    tight leaf boxes and branch boxes padded by one ulp (2^-23) in z; a 2-triangle area
+    light above it; a pinhole camera looking straight down from 1 + 3 * 2^-16
+    with a focus distance of 2^-16.  Every camera ray then has the same z
+    direction, hits the plane at t = 3, and its x and y directions lie on the
+    2^-23 lattice of numbers near 1, so a large share of the camera rays pass
+    exactly through the grid lines and diagonals; there a box's entry distance
+    (edge plane - origin) * (1/d) rounds to 3 or one ulp above it, so boxes
+    whose entry distance exceeds the hit distance of their triangle are
+    common."""
+    s = 2.0 ** -20
+    rng = np.random.default_rng(seed)
+    tris = []
+    for i in range(n):
+        for j in range(n):
+            a = (1 + i * s, 1 + j * s, 1.0)
+            b = (1 + (i + 1) * s, 1 + j * s, 1.0)
+            c = (1 + (i + 1) * s, 1 + (j + 1) * s, 1.0)
+            d = (1 + i * s, 1 + (j + 1) * s, 1.0)
+            tris += [(a, b, c), (a, c, d)] if rng.random() < 0.5 else [(a, b, d), (b, c, d)]
+    focus = 2.0 ** -16
+    height = 3 * focus
+    lo, hi = 1 - 8 * s, 1 + (n + 8) * s
+    light_z = 1 + 2 * height
+    tris += [((lo, lo, light_z), (hi, hi, light_z), (hi, lo, light_z)),
+             ((lo, lo, light_z), (lo, hi, light_z), (hi, hi, light_z))]
+    t = np.zeros(len(tris), ptrace.TRIANGLE)
+    for k, (p, q, r) in enumerate(tris):
+        t[k]["v0"], t[k]["v1"], t[k]["v2"] = p, q, r
+        t[k]["uv1"], t[k]["uv2"] = (1, 0, 0), (0, 1, 0)
+    t["material_id"][-2:] = 1
     v = np.stack([t["v0"], t["v1"], t["v2"]])
-    nodes, root, height = _tree((v.min(0), v.max(0)), ptrace.TRIANGLE_PRIM, rng, pad=pad, axes=(0, 1, 2),
-                                centers=v.mean(0))
+    nodes, root, depth = _tree((v.min(0), v.max(0)), ptrace.TRIANGLE_PRIM, rng, pad=2.0 ** -23, axes=(0, 1, 2),
+                               centers=v.mean(0))
     m = np.zeros(2, ptrace.MATERIAL)
     m["type"] = (ptrace.DIFFUSE, ptrace.LIGHT)
     m["texture_id"] = (0, 1)
     x = np.zeros(2, ptrace.TEXTURE)
     x["color"] = [(0.6, 0.5, 0.4), (15, 15, 15)]
-    scene = ptrace.FlatScene(t, np.zeros(0, ptrace.SPHERE), nodes, m, x, np.zeros(0, np.uint8), root, height)
-    camera = ptrace.make_camera((n / 2, -1.5, 2.5), (n / 2, n / 2, 0), (0, 0, 1), 60, 4 / 3, 0.0, 1.0)
+    scene = ptrace.FlatScene(t, np.zeros(0, ptrace.SPHERE), nodes, m, x, np.zeros(0, np.uint8), root, depth)
+    centre = 1 + n / 2 * s
+    camera = ptrace.make_camera((centre, centre, 1 + height), (centre, centre, 1.0), (0, 1, 0), 60, 1.0, 0.0, focus)
     return scene, camera
+
+
+def camera_rays(camera, width, height, count, seed):
+    """Jittered camera rays with the renderer's arithmetic (transport.h:157-162,
+    232-237 with a zero lens radius), in float32."""
+    rng = np.random.default_rng(seed)
+    f = np.float32
+    x = rng.integers(0, width, count).astype(f)
+    y = rng.integers(0, height, count).astype(f)
+    u = ((x + rng.integers(0, 1 << 24, count).astype(f) * f(2.0 ** -24)) / f(width)).astype(f)
+    v = ((y + rng.integers(0, 1 << 24, count).astype(f) * f(2.0 ** -24)) / f(height)).astype(f)
+    c = {k: np.asarray(camera[k], f) for k in ("origin", "lower_left_corner", "horizontal", "vertical")}
+    d = c["lower_left_corner"] + (u[:, None] * c["horizontal"]).astype(f)
+    d = (d + (v[:, None] * c["vertical"]).astype(f)).astype(f)
+    d = (d - c["origin"]).astype(f)
+    rays = np.zeros(count, ptrace.RAY)
+    rays["origin"] = c["origin"]
+    rays["direction"] = d
+    return rays
 
 
 def edge_rays(n=6, count=20000, seed=0):
