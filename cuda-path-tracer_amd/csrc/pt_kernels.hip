@@ -125,19 +125,6 @@ __device__ __forceinline__ bool trace_counted(const scene_dev& s, bool brute, co
     return traverse_bvh<ANY_HIT, true>(s, s.root_ref, r, low, high, st, best, boxes, prims, spheres, error);
 }
 
-// The reference-order retrace of a path-lane query (pt_traverse.h), out of
-// line: it is rare, and inlined its registers would add to those of the
-// whole state machine.
-template<int STRIDE>
-__device__ __noinline__ bool retrace_call(const scene_dev& s, int32_t broot, ray_in r, wide_stack_t<STRIDE> stk,
-                                          best_hit *best, int *error) {
-    best_hit b{};
-    int e = 0;
-    const bool found = exact_retrace(s, broot, r, stk, b, e);
-    *best = b;
-    *error |= e;
-    return found;
-}
 __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
     for(int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
     return v;
@@ -504,9 +491,9 @@ __global__ __launch_bounds__(256) void wide_query_kernel(scene_dev s, const pt_r
     wide_begin(s.wroot_ref, q, false, FLT_MAX);
     int boxes = 0, prims = 0, spheres = 0, error = 0;
     while(!wide_step<false>(s, q, wstk, boxes, prims, spheres)) {}
-    if(q.retrace) q.found = exact_retrace(s, s.root_ref, q.r, wstk, q.best, error);
+    if(needs_retrace(q)) q.found = exact_retrace(s, s.root_ref, q.r, wstk, q.best, error);
     out[i] = trace_result(s, q.r, q.found, q.best, error);
-    retraced_out[i] = q.retrace ? 1 : 0;
+    retraced_out[i] = needs_retrace(q) ? 1 : 0;
 }
 
 // Primary-only diagnostic views (transport.h:179-187): pixel-parallel, each
@@ -813,7 +800,7 @@ __global__ __launch_bounds__(256) void trace_lanes_kernel(scene_dev s, frame_par
         }
         if(active && wide_step<COUNT>(s, q, wstk, boxes, prims, spheres)) {
             active = false;
-            if(q.retrace) {
+            if(needs_retrace(q)) {
                 q.found = exact_retrace(s, query_broot<BATCH>(s, bd, pix), q.r, wstk, q.best, error);
                 atomicAdd(&ctl->retraces, 1ull);
             }
@@ -1255,13 +1242,14 @@ __global__ __launch_bounds__(path_lane_block) __attribute__((amdgpu_waves_per_eu
             }
         }
         if constexpr(MODE == PATHS_FIRST) cost += phase != IDLE ? 1u : 0u;
-        // Rare: closest-hit queries that stopped at an order-sensitive prune
+        // Rare: closest-hit queries that stopped at an order-sensitive case
         // are traced again in the reference's order, here rather than in the
         // traversal steps below, so that the binary traversal's registers do
-        // not add to theirs.
+        // not add to theirs.  (Inline: a function call anywhere in this
+        // kernel measured 30% slower, even where it never ran.)
         if(__ballot(phase == RETRACE)) {
             if(phase == RETRACE) {
-                q.found = retrace_call(s, query_broot<BATCH>(s, bd, pix), q.r, wstk, &q.best, &error);
+                q.found = exact_retrace(s, query_broot<BATCH>(s, bd, pix), q.r, wstk, q.best, error);
                 phase = SHADE;
                 atomicAdd(&ctl->retraces, 1ull);
             }
@@ -1301,7 +1289,7 @@ __global__ __launch_bounds__(path_lane_block) __attribute__((amdgpu_waves_per_eu
                 const bool leaves = 2 * __popcll(at_leaf) >= __popcll(tr);
                 if(phase != TRACE || (q.cur < 0) != leaves) continue;
                 if(!wide_step<false>(s, q, wstk, boxes, prims, spheres)) continue;
-                if(q.retrace) {
+                if(needs_retrace(q)) {
                     phase = RETRACE;
                 } else if(!q.any) {
                     phase = SHADE;

@@ -255,7 +255,7 @@ PT_HD void slab_entry4(const float4& LX, const float4& LY, const float4& LZ, con
 //   where one inside the band would be dropped.
 // Any-hit (shadow) queries need no such care: whether some primitive is hit
 // below the upper bound does not depend on the order.
-constexpr uint32_t wide_window_ulps = 64;
+constexpr uint32_t wide_window_ulps = 4;
 
 // Closest (or any) hit over the 4-wide layout.  Children are visited nearest
 // first; a stacked child is dropped on pop when its entry distance is not
@@ -279,13 +279,19 @@ struct wide_query {
     int32_t cur, count, best_rank;
     best_hit best;
     bool found, any;
-    bool retrace;  // stopped at an order-sensitive case: trace again in the reference's order
 };
+
+// q.cur of a query that stopped at an order-sensitive case, to be traced
+// again in the reference's order.  (Kept in an integer rather than a flag of
+// its own: a per-lane flag carried through the divergent traversal loop
+// costs scalar mask instructions at every join.)
+constexpr int32_t wide_retrace = INT32_MIN + 2;
+PT_HD bool needs_retrace(const wide_query& q) { return q.cur == wide_retrace; }
 
 PT_HD void wide_begin(int32_t root, wide_query& q, bool any, float high) {
     q.limit = high; q.entry = -INFINITY; q.cur = root; q.count = 0; q.best_rank = -1;
     q.best.t = high; q.best.code = -1;
-    q.found = false; q.any = any; q.retrace = false;
+    q.found = false; q.any = any;
 }
 
 // The widened limit: wide_window_ulps above the pruning limit once a closest
@@ -329,8 +335,9 @@ PT_HD bool wide_step(const scene_dev& s, wide_query& q, STACK& stk, int& boxes, 
                                e[i] = ei; e[j] = ej; c[i] = ci; c[j] = cj; }
             PT_CSWAP(0, 1) PT_CSWAP(2, 3) PT_CSWAP(0, 2) PT_CSWAP(1, 3) PT_CSWAP(1, 2)
 #undef PT_CSWAP
-            // The nearest child passed only the widened test: an order-sensitive prune.
-            if(!(e[0] < limit)) { q.retrace = true; return true; }
+            // The nearest child passed only the widened test: an order-sensitive
+            // prune, and the query stops for a retrace.
+            const bool band = !(e[0] < limit);
             // No overflow test: at a node of depth d (the root at 1) the stack
             // holds at most the 3 unvisited siblings of each of its d - 1
             // ancestors, so after these pushes count <= 3d, within the
@@ -340,9 +347,9 @@ PT_HD bool wide_step(const scene_dev& s, wide_query& q, STACK& stk, int& boxes, 
             if(nh > 2) stk.store(q.count + nh - 3, c[2], e[2]);
             if(nh > 3) stk.store(q.count, c[3], e[3]);
             q.count += nh - 1;
-            q.cur = c[0];
+            q.cur = band ? wide_retrace : c[0];
             q.entry = e[0];
-            return false;
+            return band;
         }
     } else {
         const int32_t v = ~q.cur;
@@ -366,7 +373,9 @@ PT_HD bool wide_step(const scene_dev& s, wide_query& q, STACK& stk, int& boxes, 
             if(STATS) prims++;
             // A hit in front of its own box's entry distance: the reference
             // may find it or not depending on the order (see above).
-            if(WINDOW && hit && !q.any && t < q.entry) { q.retrace = true; return true; }
+            const bool front = WINDOW && hit && !q.any && t < q.entry;
+            if(front) q.cur = wide_retrace;
+            hit = hit && !front;
             // A hit has t < limit, so t <= closest (before the first hit closest
             // is the upper bound and best_rank -1): the later leaf wins a tie.
             if(hit && (t < q.best.t || k > q.best_rank)) {
@@ -379,6 +388,7 @@ PT_HD bool wide_step(const scene_dev& s, wide_query& q, STACK& stk, int& boxes, 
             }
         }
     }
+    if(q.cur == wide_retrace) return true;
     for(;;) {
         if(q.count == 0) return true;
         --q.count;
@@ -387,13 +397,15 @@ PT_HD bool wide_step(const scene_dev& s, wide_query& q, STACK& stk, int& boxes, 
         // the first hit the limit is the upper bound, so only entries pushed
         // before a hit, or inside the widened band, can fail this test.
         const float entry = i2f(top.y);
-        if(!(entry < limit)) {
-            if(entry < widened<WINDOW>(q, limit)) { q.retrace = true; return true; }
-            continue;
+        if(entry < limit) {
+            q.cur = top.x;
+            q.entry = entry;
+            return false;
         }
-        q.cur = top.x;
-        q.entry = entry;
-        return false;
+        if(entry < widened<WINDOW>(q, limit)) {
+            q.cur = wide_retrace;
+            return true;
+        }
     }
 }
 
@@ -425,7 +437,7 @@ PT_HD bool traverse_wide(const scene_dev& s, int32_t root, int32_t broot, const 
     q.r = r;
     wide_begin(root, q, ANY_HIT, high);
     while(!wide_step<STATS, STACK, WINDOW>(s, q, stk, boxes, prims, spheres)) {}
-    if(q.retrace) return exact_retrace(s, broot, r, stk, best, error);
+    if(needs_retrace(q)) return exact_retrace(s, broot, r, stk, best, error);
     if(q.found) best = q.best;
     return q.found;
 }

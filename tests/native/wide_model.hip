@@ -115,7 +115,7 @@ int64_t wm_trace(const pt_scene_desc *scene, const pt_ray *rays, const float *hi
             else while(!pt::wide_step<true, model_stack, 0>(v, q, stk, boxes, prims, spheres)) {}
             found = q.found;
             b = q.best;
-            if(q.retrace) {
+            if(pt::needs_retrace(q)) {
                 retraced = true;
                 found = pt::exact_retrace(v, L.root_ref, r, stk, b, error);
             }

@@ -141,8 +141,20 @@ def test_benchmark_frame_shaded_full_size(benchmark_scene):
     assert np.array_equal(rng, o.rng_state())
 
 
+def _hip():
+    """The HIP runtime the library is linked against (the one a GL-interop
+    caller maps its pixel buffer with)."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    hip.hipMemset.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t]
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    hip.hipFree.argtypes = [ctypes.c_void_p]
+    return hip
+
+
 def test_write_rgba_into_caller_device_buffer_and_kernel_time():
-    import torch
+    import ctypes
     scene, cam = ptrace.cornell(7, 80, 50)
     r = ptrace.Renderer(scene, cam, 80, 50, 7, False, 0, True, 8)
     assert r.kernel_milliseconds() == 0
@@ -150,12 +162,20 @@ def test_write_rgba_into_caller_device_buffer_and_kernel_time():
     t1 = r.kernel_milliseconds()
     r.add_samples(4)
     assert 0 < t1 < r.kernel_milliseconds()
-    buf = torch.full((4 * 80 * 50,), 7, dtype=torch.uint8, device="cuda")
-    torch.cuda.synchronize()  # the renderer's stream is not ordered after torch's
-    r.write_rgba(buf.data_ptr(), buf.numel())
-    assert np.array_equal(buf.cpu().numpy(), r.rgba_image())
-    small = torch.zeros(4 * 80 * 50 - 1, dtype=torch.uint8, device="cuda")
-    with pytest.raises(ValueError):
-        r.write_rgba(small.data_ptr(), small.numel())
+    hip = _hip()
+    n = 4 * 80 * 50
+    ptr = ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(ptr), n) == 0
+    try:
+        assert hip.hipMemset(ptr, 7, n) == 0
+        assert hip.hipDeviceSynchronize() == 0  # the renderer's stream is not ordered after the null stream
+        r.write_rgba(ptr.value, n)
+        host = np.zeros(n, np.uint8)
+        assert hip.hipMemcpy(host.ctypes.data, ptr, n, 2) == 0  # device to host
+        assert np.array_equal(host, r.rgba_image())
+        with pytest.raises(ValueError):
+            r.write_rgba(ptr.value, n - 1)
+    finally:
+        hip.hipFree(ptr)
     r.reset(cam)
     assert r.kernel_milliseconds() == 0
