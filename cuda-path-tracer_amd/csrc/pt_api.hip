@@ -375,6 +375,39 @@ constexpr int64_t max_saved_queries = 262144;  // queries one trace launch can h
 constexpr int32_t all_paths = 1 << 30;         // drain threshold: every path on path lanes
 constexpr uint32_t log_capacity = 1u << 16;
 
+// ---------------- allocation-free rendering ----------------
+// Scratch of pt_renderer_add_samples_noalloc, for callers that cannot allocate
+// device memory while they render: one array per GPU, held by the process
+// from the library's load until it ends, shared by every such call on that
+// GPU (calls that overlap on one GPU race).  Layout: the launch's control
+// block (work cursor and counters) in the first scratch_ctl_bytes, then the
+// 4-wide traversal stack entries that do not fit in LDS.  Kernels take its
+// address as found by hipGetSymbolAddress (static_scratch), never the symbol.
+constexpr size_t static_scratch_bytes = size_t(64) << 20;
+constexpr size_t scratch_ctl_bytes = 256;
+static_assert(sizeof(pt::loop_ctl) <= scratch_ctl_bytes, "control block fits its slot");
+static __device__ __align__(256) unsigned char scratch[static_scratch_bytes];
+
+namespace {
+std::mutex scratch_lock;
+std::map<int, unsigned char *> scratch_address;  // per device
+
+// The device address of this GPU's scratch array: resolved once per device,
+// at renderer creation (set-up) or else at the first use on the device.
+unsigned char *static_scratch() {
+    int device = 0;
+    HIP_CHECK(hipGetDevice(&device));
+    std::lock_guard<std::mutex> guard(scratch_lock);
+    auto it = scratch_address.find(device);
+    if(it != scratch_address.end()) return it->second;
+    void *ptr = nullptr;
+    HIP_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if(!ptr) fail(PT_EDEVICE, "static scratch array has no device address");
+    scratch_address[device] = static_cast<unsigned char *>(ptr);
+    return static_cast<unsigned char *>(ptr);
+}
+} // namespace
+
 struct pt_renderer {
     std::unique_ptr<device_scene> scene;
     pt::frame_params frame{};
@@ -400,6 +433,7 @@ struct pt_renderer {
     pt_tuning tuning{128, 256, 16, 8, 0, 0, 0, all_paths, 48, 1, 16, 32, 12, 4, 4, 8};
     int cus = 1;
     pt_launch_stats last{};
+    uint64_t rays_noalloc = 0;           // rays of add_samples_noalloc calls since reset (not in the device counters)
 
     ~pt_renderer() {
         parts.clear();
@@ -417,7 +451,7 @@ struct pt_renderer {
         if(aov.size()) HIP_CHECK(hipMemsetAsync(aov.get(), 0, aov.size() * sizeof(float), stream));
         HIP_CHECK(hipGetLastError());
         HIP_CHECK(hipStreamSynchronize(stream));
-        frame.camera = camera; samples = 0; milliseconds = 0; rays = 0;
+        frame.camera = camera; samples = 0; milliseconds = 0; rays = 0; rays_noalloc = 0;
     }
 
     // Largest block of at most tuning.trace_block threads whose traversal
@@ -497,7 +531,7 @@ struct pt_renderer {
             st.extend_rays = uint64_t(pixels) * uint64_t(count);
             HIP_CHECK(hipEventSynchronize(stop));
             if(ctl_host->failure) fail(PT_EOVERFLOW, "HIP traversal stack overflow");
-            rays = ctl_host->rays;
+            rays = ctl_host->rays + rays_noalloc;
         } else {
             shaded_samples(count, st);
         }
@@ -505,6 +539,75 @@ struct pt_renderer {
         HIP_CHECK(hipEventElapsedTime(&elapsed, start, stop));
         milliseconds += elapsed;
         samples += count;
+        last = st;
+    }
+
+    // add_samples with every scratch buffer in the static array at `base`
+    // (static_scratch()): no device or host allocation.  One path-lane launch
+    // over all pixels, each lane taking pixel indices and running `count`
+    // samples of each from its PCG state and sum (PATHS_FRESH), with as many
+    // resident blocks as the array's stack region holds; or the diagnostic
+    // kernel.  Results equal add_samples' (each pixel's samples run in order
+    // on its own PCG stream whatever the schedule).
+    void add_samples_noalloc(int32_t count, unsigned char *base) {
+        if(count <= 0 || count > INT32_MAX - samples) fail(PT_EINVAL, "invalid accumulated sample count");
+        pt::loop_ctl *c = reinterpret_cast<pt::loop_ctl *>(base);
+        int2 *spill = reinterpret_cast<int2 *>(base + scratch_ctl_bytes);
+        pt_launch_stats st{};
+        if(frame.debug != PT_DEBUG_SHADED) {
+            HIP_CHECK(hipEventRecord(start, stream));
+            pt::launch_fresh_ctl(c, 0, stream);
+            pt::launch_debug(scene->view, frame, sum.get(), rng.get(), count, &c->failure, block, stream);
+            HIP_CHECK(hipEventRecord(stop, stream));
+        } else {
+            frame.exact_traversal = tuning.exact_traversal;
+            if(!pt::wide_traversal(scene->view, frame))
+                fail(PT_EINVAL, "allocation-free rendering of shaded frames needs the 4-wide traversal "
+                                "(not brute-force or exact-traversal renderers)");
+            // Resident blocks: as many as the stack region holds (at the deepest
+            // hierarchy validation allows, 64 levels, 181 entries per lane, 181
+            // blocks still fit), at most the renderer's path-lane tuning.
+            const size_t entries = size_t(pt::wide_spill_entries(scene->view.wide_stack_capacity));
+            const size_t per_block = size_t(pt::path_lane_block) * entries * sizeof(int2);
+            const size_t fit = per_block ? (static_scratch_bytes - scratch_ctl_bytes) / per_block : size_t(INT32_MAX);
+            if(fit == 0) fail(PT_EINVAL, "traversal stack exceeds the static scratch array");
+            size_t blocks = fit;
+            if(tuning.path_blocks_per_cu > 0) blocks = std::min(blocks, size_t(cus) * size_t(tuning.path_blocks_per_cu));
+            blocks = std::min(blocks, size_t(INT32_MAX));
+            const pt::path_lanes lanes{tuning.path_shade_batch, int(blocks), tuning.path_trace_steps};
+            pt::paths_dev p{};
+            p.sum = sum.get(); p.rng = rng.get(); p.aov = aov.size() ? aov.get() : nullptr; p.spill = spill;
+            pt::path_pass pass;
+            pass.mode = pt::PATHS_FRESH;
+            pass.first_samples = count;
+            HIP_CHECK(hipEventRecord(start, stream));
+            pt::launch_fresh_ctl(c, uint32_t(pixels), stream);
+            pt::launch_path_lanes(scene->view, frame, p, c, 0, nullptr, uint32_t(pixels), pt::path_lane_block, lanes,
+                                  pass, stream);
+            HIP_CHECK(hipEventRecord(stop, stream));
+        }
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipMemcpyAsync(ctl_host, c, sizeof(pt::loop_ctl), hipMemcpyDeviceToHost, stream));
+        HIP_CHECK(hipStreamSynchronize(stream));
+        if(ctl_host->failure) fail(PT_EOVERFLOW, "HIP traversal stack overflow");
+        uint64_t call_rays;
+        if(frame.debug != PT_DEBUG_SHADED) {
+            call_rays = uint64_t(pixels) * uint64_t(count);
+            st.iterations = 1;
+            st.extend_rays = call_rays;
+        } else {
+            call_rays = ctl_host->rays;
+            st.extend_rays = ctl_host->ext_rays;
+            st.shadow_rays = ctl_host->shadow_rays;
+            st.retraced_queries = ctl_host->retraces;
+            st.drained_paths = pixels;
+        }
+        float elapsed = 0;
+        HIP_CHECK(hipEventElapsedTime(&elapsed, start, stop));
+        milliseconds += elapsed;
+        samples += count;
+        rays += call_rays;
+        rays_noalloc += call_rays;
         last = st;
     }
 
@@ -638,7 +741,7 @@ struct pt_renderer {
             }
         }
         if(failure) fail(PT_EOVERFLOW, "HIP traversal stack overflow");
-        rays = total_rays;
+        rays = total_rays + rays_noalloc;
     }
 };
 
@@ -1153,6 +1256,7 @@ pt_status pt_renderer_create(const pt_scene_desc *scene, const pt_flat_camera *c
         HIP_CHECK(hipEventCreate(&r->start));
         HIP_CHECK(hipEventCreate(&r->stop));
         r->cus = device_cu_count();
+        (void)static_scratch();  // resolve this GPU's allocation-free scratch at set-up
         r->scene = std::make_unique<device_scene>(*scene, true, settings->debug == PT_DEBUG_SHADED);
         allocate_state(*r, settings->debug == PT_DEBUG_SHADED);
         r->reset(*camera);
@@ -1175,6 +1279,13 @@ pt_status pt_renderer_add_samples(pt_renderer *renderer, int32_t count) {
     return guarded([&] {
         if(!renderer) fail(PT_EINVAL, "null renderer");
         renderer->add_samples(count);
+    });
+}
+
+pt_status pt_renderer_add_samples_noalloc(pt_renderer *renderer, int32_t count) {
+    return guarded([&] {
+        if(!renderer) fail(PT_EINVAL, "null renderer");
+        renderer->add_samples_noalloc(count, static_scratch());
     });
 }
 
@@ -1299,7 +1410,7 @@ pt_status pt_renderer_write_state(pt_renderer *renderer, const float *sum, const
             pc.log_capacity = log_capacity;
             HIP_CHECK(hipMemcpy(part->ctl.get(), &pc, sizeof(pc), hipMemcpyHostToDevice));
         }
-        renderer->samples = samples; renderer->rays = rays;
+        renderer->samples = samples; renderer->rays = rays; renderer->rays_noalloc = 0;
     });
 }
 

@@ -208,9 +208,12 @@ void launch_drain(const scene_dev& s, const frame_params& f, const paths_dev& p,
 // two passes: PATHS_FIRST runs at most `first_samples` samples of each pixel
 // and writes the pixel's cost (loop passes its lane spent on it; 0 when the
 // pixel is done) at its queue position; PATHS_REST starts each queued pixel
-// at its next sample.  PATHS_WHOLE takes queued paths where they stand and
+// at its next sample.  PATHS_FRESH (single renderers) takes pixel indices
+// 0 .. n-1 without a queue and runs `first_samples` samples of each from its
+// stored PCG state and sum, with no other per-pixel state.  PATHS_WHOLE takes
+// queued paths where they stand and
 // runs them to the end.
-enum path_mode : int { PATHS_WHOLE = 0, PATHS_FIRST = 1, PATHS_REST = 2 };
+enum path_mode : int { PATHS_WHOLE = 0, PATHS_FIRST = 1, PATHS_REST = 2, PATHS_FRESH = 3 };
 struct path_pass {
     path_mode mode = PATHS_WHOLE;
     int first_samples = 0;
@@ -234,6 +237,8 @@ void launch_path_lanes(const scene_dev& s, const frame_params& f, const paths_de
                        const int32_t *queue, uint32_t paths, int block, path_lanes lanes, path_pass pass,
                        hipStream_t stream, const batch_dev *batch = nullptr);
 void launch_zero_work(loop_ctl *ctl, hipStream_t stream);
+// Zero a control block and set n_ext[0] = n (the item count of a launch).
+void launch_fresh_ctl(loop_ctl *ctl, uint32_t n, hipStream_t stream);
 void launch_pack(const float *sum, uint8_t *rgba, int64_t count, int32_t samples, hipStream_t stream);
 void launch_add_rays(loop_ctl *ctl, unsigned long long rays, hipStream_t stream);
 

@@ -1202,7 +1202,7 @@ __global__ __launch_bounds__(path_lane_block) __attribute__((amdgpu_waves_per_eu
             // Done, or (first pass) stopped at a sample boundary: the next
             // sample's camera ray is drawn when the pixel is taken again.
             p.rng[pix] = r.state;
-            p.ctr[pix] = make_int2(0, st.left > 0 ? st.left : 0);
+            if constexpr(MODE != PATHS_FRESH) p.ctr[pix] = make_int2(0, st.left > 0 ? st.left : 0);
             if constexpr(MODE == PATHS_FIRST) costs[claim] = st.left > 0 ? cost : 0u;
             phase = IDLE;
             return;
@@ -1221,12 +1221,16 @@ __global__ __launch_bounds__(path_lane_block) __attribute__((amdgpu_waves_per_eu
             else if(phase == IDLE) {
                 const uint32_t item = b + uint32_t(__popcll(idle & ((1ull << lane) - 1ull)));
                 if(item < n) {
-                    const int32_t v = queue[item];
-                    pix = v >= 0 ? v : ~v;
+                    if constexpr(MODE == PATHS_FRESH) {
+                        pix = int32_t(item);
+                    } else {
+                        const int32_t v = queue[item];
+                        pix = v >= 0 ? v : ~v;
+                    }
                     r = pixel_rng(p.rng[pix], local_index<BATCH>(bd, pix));
                     root = query_root<BATCH>(s, bd, pix);
-                    if constexpr(MODE == PATHS_REST) {
-                        st.left = p.ctr[pix].y;
+                    if constexpr(MODE == PATHS_REST || MODE == PATHS_FRESH) {
+                        st.left = MODE == PATHS_FRESH ? first_samples : p.ctr[pix].y;
                         if(st.left > 0) {
                             pixel_camera_ray<BATCH>(f, bd, pix, r, st.o, st.d, st.time);
                             st.throughput = mk(1, 1, 1); st.radiance = mk(0, 0, 0); st.previous_pdf = 0; st.depth = 0;
@@ -1482,6 +1486,11 @@ template<int MODE>
 static void path_lanes_mode(const scene_dev& s, const frame_params& f, const paths_dev& p, loop_ctl *ctl, int parity,
                             const int32_t *queue, int shade_batch, int trace_steps, const path_pass& pass, dim3 grid,
                             dim3 block, size_t lds, hipStream_t stream, const batch_dev *batch) {
+    if constexpr(MODE == PATHS_FRESH) {  // single renderers only
+        hipLaunchKernelGGL((path_lanes_kernel<false, MODE>), grid, block, lds, stream, s, f, p, ctl, parity, queue,
+                           shade_batch, trace_steps, pass.first_samples, pass.costs, batch_dev{});
+        return;
+    }
     if(batch)
         hipLaunchKernelGGL((path_lanes_kernel<true, MODE>), grid, block, lds, stream, s, f, p, ctl, parity, queue,
                            shade_batch, trace_steps, pass.first_samples, pass.costs, *batch);
@@ -1498,6 +1507,8 @@ void launch_path_lanes(const scene_dev& s, const frame_params& f, const paths_de
     if(block != path_lane_block) throw std::logic_error("path lanes run blocks of path_lane_block threads");
     if(pass.mode == PATHS_FIRST && (!pass.costs || pass.first_samples <= 0))
         throw std::logic_error("first path-lane pass without a cost buffer");
+    if(pass.mode == PATHS_FRESH && (batch || pass.first_samples <= 0))
+        throw std::logic_error("fresh path-lane pass without a sample count, or on a batch");
     if(!paths) return;
     const size_t lds = trace_lds_bytes(s, f, block);
     const unsigned grid = unsigned((paths + uint32_t(block) - 1) / uint32_t(block));
@@ -1508,6 +1519,7 @@ void launch_path_lanes(const scene_dev& s, const frame_params& f, const paths_de
     switch(pass.mode) {
     case PATHS_FIRST: path_lanes_mode<PATHS_FIRST>(s, f, p, ctl, parity, queue, batch_lanes, steps, pass, gd, bd, lds, stream, batch); break;
     case PATHS_REST: path_lanes_mode<PATHS_REST>(s, f, p, ctl, parity, queue, batch_lanes, steps, pass, gd, bd, lds, stream, batch); break;
+    case PATHS_FRESH: path_lanes_mode<PATHS_FRESH>(s, f, p, ctl, parity, queue, batch_lanes, steps, pass, gd, bd, lds, stream, batch); break;
     default: path_lanes_mode<PATHS_WHOLE>(s, f, p, ctl, parity, queue, batch_lanes, steps, pass, gd, bd, lds, stream, batch); break;
     }
 }
@@ -1541,6 +1553,16 @@ void launch_group_gather(const int32_t *queue, const uint32_t *ids, uint32_t n, 
     if(n) hipLaunchKernelGGL(group_gather_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, queue, ids, n, group, out);
 }
 __global__ void zero_work_kernel(loop_ctl *ctl) { ctl->work = 0; }
+// A control block cleared for one launch over `n` items (a kernel rather than a
+// memset, so that the call can be captured into a HIP graph).
+__global__ void fresh_ctl_kernel(loop_ctl *ctl, uint32_t n) {
+    loop_ctl c{};
+    c.n_ext[0] = n;
+    *ctl = c;
+}
+void launch_fresh_ctl(loop_ctl *ctl, uint32_t n, hipStream_t stream) {
+    hipLaunchKernelGGL(fresh_ctl_kernel, dim3(1), dim3(1), 0, stream, ctl, n);
+}
 void launch_zero_work(loop_ctl *ctl, hipStream_t stream) {
     hipLaunchKernelGGL(zero_work_kernel, dim3(1), dim3(1), 0, stream, ctl);
 }

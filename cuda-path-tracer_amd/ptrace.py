@@ -142,6 +142,7 @@ PT_FUNCTIONS = {
     "pt_renderer_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "pt_renderer_reset": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "pt_renderer_add_samples": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "pt_renderer_add_samples_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "pt_renderer_set_block_shape": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
     "pt_renderer_read_image": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]),
     "pt_renderer_read_rgba": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]),
@@ -285,6 +286,10 @@ class Renderer:
 
     def add_samples(self, count):
         _check(lib().pt_renderer_add_samples(self._h, int(count)))
+
+    def add_samples_noalloc(self, count):
+        """pt_renderer_add_samples_noalloc: add_samples with its scratch in a fixed device array."""
+        _check(lib().pt_renderer_add_samples_noalloc(self._h, int(count)))
 
     def set_block_shape(self, width, height):
         _check(lib().pt_renderer_set_block_shape(self._h, int(width), int(height)))

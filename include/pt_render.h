@@ -118,6 +118,19 @@ pt_status pt_renderer_destroy(pt_renderer *renderer);
 pt_status pt_renderer_reset(pt_renderer *renderer, const pt_flat_camera *camera);
 /* cuda_renderer::add_samples (cuda_renderer.h:18, .cu:196-213) */
 pt_status pt_renderer_add_samples(pt_renderer *renderer, int32_t count);
+/* add_samples without any allocation (no counterpart in the reference): for
+ * callers that cannot allocate device memory while they render -- code under
+ * HIP graph capture, tight serving loops, harnesses that forbid it.  The
+ * results (sums, PCG states, ray count, image) equal pt_renderer_add_samples'.
+ * The call's scratch (launch control block, traversal stack entries beyond
+ * LDS) is a fixed 64 MiB device array per GPU, held from the library's load
+ * until the process ends and shared by every such call on that GPU: calls
+ * that overlap on one GPU race.  The launch keeps as many resident path-lane
+ * blocks as the array's stack region holds (at least 181 at the deepest
+ * hierarchy validation allows).  Shaded mode needs the 4-wide traversal:
+ * brute-force and exact-traversal renderers get PT_EINVAL; diagnostic views
+ * are supported.  Synchronous, like add_samples. */
+pt_status pt_renderer_add_samples_noalloc(pt_renderer *renderer, int32_t count);
 /* cuda_renderer::set_block_shape (cuda_renderer.h:19, .cu:215-228): one of
  * 8x8, 16x8, 16x16, 32x8; otherwise PT_EINVAL. */
 pt_status pt_renderer_set_block_shape(pt_renderer *renderer, int32_t width, int32_t height);
