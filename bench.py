@@ -11,21 +11,33 @@ inputs resident in HBM.  ``value`` is rays per second over the timed steps,
 counting every closest-hit and shadow query (the reference's rays_traced).
 
 Also reported:
-  roofline     -- path-lane kernel (every path of the frame, traversal and
-                  shading on per-lane state machines; a frame of more samples
-                  than the first pass runs it in two passes), HBM-bound:
-                  algorithmic bytes of the frame / HIP-event time of the
-                  path-lane work on the renderer's stream, against 8 TB/s.
-                  Bytes per query = 32*box tests + 36*triangle tests +
-                  16*sphere tests + 48 (closest hit) or + 36 (shadow), the
-                  tests being the reference-order traversal's trace_stats
-                  (branch nodes and leaves popped) from an untimed counting
-                  pass over the same rays.  traffic =
-                  rocprofv3 FETCH_SIZE + WRITE_SIZE of that work (committed
-                  summary under profiles/), scaled by a plain-copy calibration.
+  roofline     -- the path-lane kernel (every path of the frame: 4-wide
+                  traversal and shading on per-lane state machines, two passes
+                  per frame), HBM-bound.  achieved = algorithmic bytes of the
+                  frame / HIP-event time of the frame's path-lane work on the
+                  renderer stream.  Algorithmic bytes are what that traversal
+                  and shading must read and write, counted by an untimed
+                  counting pass over the same queries with the same 4-wide
+                  traversal: 128 B per wide node visited, 48 B per leaf record
+                  tested, 92 B per shaded hit (shading record 64 B + material
+                  28 B), 24 B per sample (read and write of the pixel's sum),
+                  16 B per pixel (PCG state in and out), and 32 B per branch /
+                  36 B per leaf popped by the reference-order retraces.
+                  Measured figures of the same kernel from rocprofv3 counters
+                  (committed summary under profiles/): HBM bytes (traffic,
+                  FETCH_SIZE + WRITE_SIZE scaled by a plain-copy calibration),
+                  the measured HBM fraction, the L2 hit rate and active lanes
+                  per VALU instruction.  The reference-order figure (the
+                  reference's own trace_stats: branch nodes and leaves popped
+                  by its binary traversal) is reported under reference_order.
   cpu_baseline -- the CPU oracle (a C restatement of the reference's host
-                  transport path) on the host's cores over a 40-spp frame of
-                  the same scene and camera (about 10 s on 16 threads).
+                  transport path) on every core available to the process over
+                  a bounded sample of the same frame; reference_host: the
+                  reference's own host path compiled from its sources
+                  (oracle/_ref), labelled, at its fixed path limit of 50.
+  denoised     -- time to relMSE <= 0.01 against a 16384-spp render of the
+                  same 1080p view, for plain progressive accumulation and for a
+                  low-spp render denoised by the shipped network.
 """
 import argparse
 import json
@@ -45,30 +57,78 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def traffic_from_profiles(kernel):
-    """HBM bytes per unit of `kernel` (one launch, or both passes of a split
-    frame) measured by rocprofv3 --pmc (committed summary), or None."""
-    path = os.path.join(HERE, "profiles", "path_kernel_icosphere1M_1080p_traffic.json")
-    if not os.path.exists(path):
-        return None
+def available_cores():
+    """CPUs this process may run on: its affinity mask, limited by a cgroup
+    CPU quota where one is set."""
+    n = len(os.sched_getaffinity(0))
     try:
-        with open(path) as f:
-            data = json.load(f)
+        quota, period = open("/sys/fs/cgroup/cpu.max").read().split()
+        if quota != "max":
+            n = min(n, max(1, int(float(quota) / float(period))))
     except (OSError, ValueError):
-        return None
-    return data.get("hbm_bytes_per_launch") if data.get("kernel") == kernel else None
+        pass
+    return n
 
 
-def cpu_baseline(scene, camera, threads, spp=1):
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(scene, camera, threads, spp):
     import oracle
     r = oracle.Renderer(scene, camera, WIDTH, HEIGHT, SEED, False, 0, True, DEPTH)
     t0 = time.perf_counter()
     r.add_samples(spp, threads=threads)
     dt = time.perf_counter() - t0
     rays = r.rays_traced()
-    return {"value": rays / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": f"{spp} spp over the full {WIDTH}x{HEIGHT} frame of the same scene/camera "
-                      f"({rays} rays, {dt:.1f} s, {threads} threads)"}
+    return {"value": rays / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port", "cpu": cpu_model(),
+            "sample": f"{spp} spp (the timed GPU frame has {SPP}) over the full {WIDTH}x{HEIGHT} frame of the same "
+                      f"scene, camera, seed and depth ({rays} rays, {dt:.1f} s, {threads} threads)"}
+
+
+def reference_host_baseline(scene, camera, threads, rows=216):
+    """The reference's own host path (portable_renderer's trace_pixel_sample,
+    compiled from the reference's sources into oracle/_ref) on `rows` rows of
+    the frame, 1 spp.  Its path limit is fixed at 50 (transport.h:174), so
+    its rays per sample differ from the depth-8 workload: a labelled side
+    figure, not the baseline."""
+    import ctypes
+    import numpy as np
+    import oracle
+    if not oracle.ref_available():
+        return None
+    L = oracle.ref()
+    s, keep = oracle.scene_struct(scene)
+    cam = np.ascontiguousarray(np.asarray(camera).reshape(()))
+    y0 = HEIGHT // 2 - rows // 2
+    out = np.zeros((rows * WIDTH, 3), np.float32)
+    rays = ctypes.c_uint64()
+    t0 = time.perf_counter()
+    rc = L.ref_render_rows(ctypes.byref(s), cam.ctypes.data, WIDTH, HEIGHT, SEED, 1, 1, y0, y0 + rows, threads,
+                           out.ctypes.data, ctypes.byref(rays))
+    dt = time.perf_counter() - t0
+    if rc != 0:
+        return None
+    return {"value": rays.value / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "reference",
+            "sample": f"1 spp over rows {y0}..{y0 + rows - 1} of the same frame at the reference's fixed path limit "
+                      f"of 50 ({rays.value} rays, {dt:.1f} s)"}
+
+
+def profile_figures(kernel):
+    """Counter figures of `kernel` from the committed summary, or {}."""
+    path = os.path.join(HERE, "profiles", "path_kernel_icosphere1M_1080p_traffic.json")
+    try:
+        with open(path) as f:
+            data = json.load(f)
+    except (OSError, ValueError):
+        return {}
+    return data if data.get("kernel") == kernel else {}
 
 
 BATCH_SIZES = [(256, 256), (512, 384), (640, 480), (800, 600), (1024, 576), (1280, 720)]
@@ -136,12 +196,39 @@ def bench_batch(args, ptrace):
     return 0
 
 
+def denoised_vs_plain(scene, camera, log):
+    """Wall time to relMSE <= 0.01 against a 16384-spp render of the
+    benchmark view: plain progressive accumulation (checkpoints from 4 to 4096
+    spp) against a 1, 4 or 16 spp render denoised by the shipped network (the
+    render, the denoiser and the read-back of the result to host memory)."""
+    import denoise
+    t0 = time.perf_counter()
+    net = denoise.Denoiser(weights=denoise.WEIGHTS_PATH)
+    res = denoise.time_to_relmse(scene, camera, WIDTH, HEIGHT, net, target=0.01, reference_spp=16384,
+                                 log=lambda *a: log(*a))
+    p, d = res["plain_hit"], res["denoised_hit"]
+    out = {"metric": "seconds to relMSE <= 0.01 (1080p, against 16384 spp)", "reference_spp": 16384,
+           "relmse_eps": 0.01,
+           "plain": {"spp": p["spp"], "seconds": round(p["seconds"], 4), "relmse": round(p["relmse"], 5)} if p else None,
+           "denoised": ({"spp": d["spp"], "seconds": round(d["seconds"], 4), "relmse": round(d["relmse"], 5),
+                         "denoise_ms": round(d["denoise_ms"], 3)} if d else None),
+           "speedup": round(res["speedup"], 2) if res["speedup"] else None,
+           "denoised_by_spp": [{"spp": x["spp"], "relmse": round(x["relmse"], 5), "seconds": round(x["seconds"], 4)}
+                               for x in res["denoised"]],
+           "plain_by_spp": [{"spp": x["spp"], "relmse": round(x["relmse"], 5), "seconds": round(x["seconds"], 4)}
+                            for x in res["plain"]],
+           "wall_s": round(time.perf_counter() - t0, 1)}
+    log("denoised vs plain:", out["plain"], out["denoised"], out["speedup"])
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-denoise", action="store_true", help="skip the denoised-vs-plain measurement")
     ap.add_argument("--spp", type=int, default=SPP)
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a plain 1 GiB device copy (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -180,26 +267,33 @@ def main():
     r = ptrace.Renderer(scene, camera, WIDTH, HEIGHT, SEED, False, ptrace.DEBUG_SHADED, True, DEPTH)
     log(f"upload {time.perf_counter() - t0:.2f}s")
 
-    # Untimed counting pass: the same frame (same seed, reset) on the wavefront
-    # bounce loop with no tail kernel and the reference-order binary traversal,
-    # so every query goes through the trace kernel and its box and primitive
-    # tests are the reference's own trace_stats (transport.h:43-85): branch
-    # nodes and leaves popped.  These counts depend on the scene and its flat
-    # hierarchy only, not on the layout or the schedule that is timed.
-    r.set_tuning(count_tests=True, drain_threshold=-1, exact_traversal=True)
-    r.add_samples(args.spp)
-    counts = r.last_launch_stats()
-    ext_log, sh_log = r.iteration_log()
-    traced_ext, traced_sh = int(ext_log.sum()), int(sh_log.sum())
-    r.set_tuning(count_tests=False, time_kernels=True, drain_threshold=DRAIN)
-    log("counting pass:", counts, "trace-kernel queries:", traced_ext, traced_sh)
+    # Untimed counting passes over the same frame (same seed, reset), on the
+    # wavefront bounce loop with no tail kernel, so that every query goes
+    # through the counting trace kernel:
+    # 1. the timed traversal: 4-wide nearest-first with its reference-order
+    #    retraces (persistent-lane trace kernel) -- wide nodes visited, leaf
+    #    records tested, closest-hit queries that hit, retrace pops;
+    # 2. the reference order (binary traversal): the reference's own
+    #    trace_stats (transport.h:43-85), reported as a separate figure.
+    def counting_pass(exact):
+        r.reset(camera)
+        r.set_tuning(count_tests=True, drain_threshold=-1, exact_traversal=exact)
+        r.add_samples(args.spp)
+        st = r.last_launch_stats()
+        ext_log, sh_log = r.iteration_log()
+        return st, int(ext_log.sum()), int(sh_log.sum())
+    wide_counts, wide_ext, wide_sh = counting_pass(False)
+    log("counting pass (4-wide):", wide_counts)
+    ref_counts, traced_ext, traced_sh = counting_pass(True)
+    log("counting pass (reference order):", ref_counts)
+    r.set_tuning(count_tests=False, time_kernels=True, drain_threshold=DRAIN, exact_traversal=False)
 
     for _ in range(args.warmup):
         r.reset(camera)
         r.add_samples(args.spp)
     rays = 0
     trace_ms = shade_ms = regen_ms = drain_ms = 0.0
-    launches = splits = 0
+    launches = splits = retraced = 0
     t0 = time.perf_counter()
     for _ in range(args.steps):
         r.reset(camera)
@@ -212,44 +306,65 @@ def main():
         drain_ms += st["drain_ms"]
         launches += st["trace_launches"]
         splits += st["split_launches"]
+        retraced += st["retraced_queries"]
     elapsed = time.perf_counter() - t0
     ms_per_step = elapsed / args.steps * 1e3
     mrays = rays / elapsed / 1e6
-    log(f"{args.steps} steps: {elapsed:.3f}s, {mrays:.1f} Mrays/s, trace {trace_ms:.1f} ms, shade {shade_ms:.1f} ms")
+    log(f"{args.steps} steps: {elapsed:.3f}s, {mrays:.1f} Mrays/s, path lanes {drain_ms / args.steps:.2f} ms/frame")
 
-    # Dominant kernel: the path-lane kernel (every query of the frame) by
-    # default, else the wavefront trace kernel.  A split frame runs it twice
-    # (first samples of every pixel, then the rest in cost order): the unit
-    # is then the frame's path-lane work, timed from the first pass's start to
-    # the second's end (HIP events on the renderer's stream; the 8-bit key
-    # sort between them included).
-    alg_bytes_frame = (32 * counts["box_tests"] + 36 * counts["triangle_tests"] + 16 * counts["sphere_tests"]
-                       + 48 * traced_ext + 36 * traced_sh)
-    passes = 1
-    if launches == 0:
-        if splits:
-            kernel, passes = "path_lanes_kernel<false, 1> + path_lanes_kernel<false, 2>", 2
-        else:
-            kernel = "path_lanes_kernel<false, 0>"
-        bytes_per_launch, launch_s = alg_bytes_frame, drain_ms / 1e3 / args.steps
-    else:
-        kernel, bytes_per_launch, launch_s = ("trace_lanes_kernel<false, false>", alg_bytes_frame / (launches / args.steps),
-                                              trace_ms / 1e3 / launches)
-    achieved = bytes_per_launch / launch_s / 1e9
-    traffic = traffic_from_profiles(kernel)
+    # Dominant kernel: the path-lane kernel (every query of the frame).  A
+    # frame runs it in two passes (first samples of every pixel, then the rest
+    # in cost order): the unit is the frame's path-lane work, timed from the
+    # first pass's start to the second's end (HIP events on the renderer's
+    # stream; the 8-bit key sort between them included).
+    pixels = WIDTH * HEIGHT
+    alg = {"wide_nodes": 128 * wide_counts["box_tests"],
+           "leaf_records": 48 * (wide_counts["triangle_tests"] + wide_counts["sphere_tests"]),
+           "shading": 92 * wide_counts["hit_queries"],
+           "accumulation": 24 * pixels * args.spp + 16 * pixels,
+           "retraces": 32 * wide_counts["retrace_box_tests"] + 36 * wide_counts["retrace_primitive_tests"]}
+    alg_bytes = sum(alg.values())
+    passes = 2 if splits else 1
+    kernel = "path_lanes_kernel<false, 1> + path_lanes_kernel<false, 2>" if splits else "path_lanes_kernel<false, 0>"
+    launch_s = drain_ms / 1e3 / args.steps
+    if launches:  # a tuning that left paths on the wavefront loop: the bytes cover more than the path lanes
+        kernel += " (with wavefront trace launches)"
+    achieved = alg_bytes / launch_s / 1e9
+    prof = profile_figures(kernel)
+    queries = wide_ext + wide_sh
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": prof.get("hbm_bytes_per_launch"),
                 "kernel": kernel, "launches_per_unit": passes, "avg_launch_ms": round(launch_s * 1e3, 4),
-                "alg_bytes_per_launch": int(bytes_per_launch),
-                "box_tests_per_query": round(counts["box_tests"] / (traced_ext + traced_sh), 2),
-                "triangle_tests_per_query": round(counts["triangle_tests"] / (traced_ext + traced_sh), 2)}
+                "alg_bytes_per_launch": int(alg_bytes), "alg_bytes_by_kind": alg,
+                "wide_nodes_per_query": round(wide_counts["box_tests"] / queries, 2),
+                "leaf_records_per_query": round((wide_counts["triangle_tests"] + wide_counts["sphere_tests"]) / queries, 2),
+                "retraced_queries_per_frame": retraced // max(1, args.steps),
+                "measured_hbm_frac": prof.get("measured_hbm_frac"), "l2_hit_rate": prof.get("l2_hit_rate"),
+                "valu_active_lanes": prof.get("valu_active_lanes"), "profile": prof.get("summary")}
+    ref_bytes = (32 * ref_counts["box_tests"] + 36 * ref_counts["triangle_tests"] + 16 * ref_counts["sphere_tests"]
+                 + 48 * traced_ext + 36 * traced_sh)
+    roofline["reference_order"] = {
+        "alg_bytes": int(ref_bytes), "achieved": round(ref_bytes / launch_s / 1e9, 1),
+        "box_tests_per_query": round(ref_counts["box_tests"] / (traced_ext + traced_sh), 2),
+        "triangle_tests_per_query": round(ref_counts["triangle_tests"] / (traced_ext + traced_sh), 2),
+        "note": "the reference's binary traversal tests (32 B per branch, 36 B per triangle) over the same time; "
+                "not the timed kernel's own work"}
 
-    cpu = None
+    cpu = ref_host = None
     if not args.no_cpu:
-        threads = min(16, len(os.sched_getaffinity(0)))
-        log(f"CPU baseline on {threads} threads ...")
-        cpu = cpu_baseline(scene, camera, threads, spp=40)
+        threads = available_cores()
+        spp = max(4, min(SPP, round(SPP * threads / 16)))  # the whole frame at 16 threads (~12 s)
+        log(f"CPU baseline on {threads} threads ({cpu_model()}), {spp} spp ...")
+        cpu = cpu_baseline(scene, camera, threads, spp)
         log("cpu:", cpu)
+        ref_host = reference_host_baseline(scene, camera, threads)
+        log("reference host path:", ref_host)
+        if cpu and ref_host:
+            cpu["reference_host"] = ref_host
+
+    den = None
+    if not args.no_denoise:
+        den = denoised_vs_plain(scene, camera, log)
 
     line = {"metric": "Mrays/s at 1080p, 1M tris", "value": round(mrays, 2), "unit": "Mrays/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 2),
@@ -258,17 +373,16 @@ def main():
             "config": {"workload": "plain path tracing (path lanes), icosphere 1,003,530 tris, 1920x1080, "
                                    f"{args.spp} spp, depth {DEPTH}, MIS", "width": WIDTH, "height": HEIGHT,
                        "spp": args.spp, "max_depth": DEPTH, "triangles": int(len(scene.triangles)),
-                       "rays_per_step": rays // args.steps, "trace_ms_per_step": round(trace_ms / args.steps, 2),
-                       "shade_ms_per_step": round(shade_ms / args.steps, 2),
-                       "regen_ms_per_step": round(regen_ms / args.steps, 2),
-                       "drain_ms_per_step": round(drain_ms / args.steps, 2),
-                       "bounce_iterations": counts["iterations"], "drained_paths": counts["drained_paths"],
+                       "rays_per_step": rays // args.steps,
+                       "path_lanes_ms_per_step": round(drain_ms / args.steps, 2),
+                       "wavefront_trace_launches_per_step": launches / args.steps,
                        "split_launches_per_step": splits / args.steps},
-            "roofline": roofline, "cpu_baseline": cpu}
+            "roofline": roofline, "cpu_baseline": cpu, "denoised": den}
     if cpu:
         line["gpu_over_cpu"] = round(mrays / cpu["value"], 1)
     print(json.dumps(line), flush=True)
     return 0
+
 
 
 if __name__ == "__main__":

@@ -732,6 +732,9 @@ struct pt_renderer {
             st.triangle_tests += h.triangle_tests;
             st.sphere_tests += h.sphere_tests;
             st.retraced_queries += h.retraces;
+            st.hit_queries += h.hits;
+            st.retrace_box_tests += h.retrace_branches;
+            st.retrace_primitive_tests += h.retrace_leaves;
             total_rays += h.rays;
             failure |= h.failure;
             if(tuning.time_kernels && part->drained) {

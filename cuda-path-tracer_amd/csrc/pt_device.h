@@ -131,6 +131,11 @@ struct loop_ctl {
     // Closest-hit queries of the 4-wide traversal traced again in the
     // reference's order (order-sensitive cases, see pt_traverse.h).
     unsigned long long retraces;
+    // Counting pass of the 4-wide persistent-lane trace kernel: closest-hit
+    // queries that hit, and the binary branch and leaf pops of the retraces
+    // (box_tests/triangle_tests/sphere_tests then count wide nodes visited and
+    // leaf records tested).
+    unsigned long long hits, retrace_branches, retrace_leaves;
 };
 
 struct frame_params {

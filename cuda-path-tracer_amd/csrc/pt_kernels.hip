@@ -598,6 +598,7 @@ __global__ void start_kernel(scene_dev s, frame_params f, paths_dev p, loop_ctl 
         ctl->n_saved[0] = 0; ctl->n_saved[1] = 0; ctl->n_waiting = 0;
         ctl->iterations = 0; ctl->ext_rays = 0; ctl->shadow_rays = 0;
         ctl->box_tests = 0; ctl->triangle_tests = 0; ctl->sphere_tests = 0; ctl->retraces = 0;
+        ctl->hits = 0; ctl->retrace_branches = 0; ctl->retrace_leaves = 0;
     }
     if(j >= count) return;
     const int64_t i = ((j / chunk) * parts + part) * chunk + j % chunk;
@@ -717,7 +718,7 @@ __global__ __launch_bounds__(256) void trace_lanes_kernel(scene_dev s, frame_par
     wide_stack wstk{reinterpret_cast<int2 *>(lds) + threadIdx.x, int(blockDim.x), p.spill,
                     size_t(gridDim.x) * blockDim.x, blockIdx.x * blockDim.x + threadIdx.x};
     int error = 0;
-    unsigned long long n_box = 0, n_leaf = 0, n_sph = 0;
+    unsigned long long n_box = 0, n_leaf = 0, n_sph = 0, n_hits = 0, n_rbox = 0, n_rleaf = 0;
     int boxes = 0, prims = 0, spheres = 0;
     wide_query q;
     bool active = false, exhausted = false, handed = false;
@@ -801,9 +802,16 @@ __global__ __launch_bounds__(256) void trace_lanes_kernel(scene_dev s, frame_par
         if(active && wide_step<COUNT>(s, q, wstk, boxes, prims, spheres)) {
             active = false;
             if(needs_retrace(q)) {
-                q.found = exact_retrace(s, query_broot<BATCH>(s, bd, pix), q.r, wstk, q.best, error);
+                if(COUNT) {
+                    int rb = 0, rp = 0, rs = 0;
+                    q.found = exact_retrace_counted(s, query_broot<BATCH>(s, bd, pix), q.r, wstk, q.best, error, rb, rp, rs);
+                    n_rbox += unsigned(rb); n_rleaf += unsigned(rp);
+                } else {
+                    q.found = exact_retrace(s, query_broot<BATCH>(s, bd, pix), q.r, wstk, q.best, error);
+                }
                 atomicAdd(&ctl->retraces, 1ull);
             }
+            if(COUNT && !q.any && q.found) n_hits++;
             if(!q.any) {
                 p.hit[pix] = q.found ? make_int4(__float_as_int(q.best.t), __float_as_int(q.best.u),
                                                  __float_as_int(q.best.v), q.best.code)
@@ -823,10 +831,14 @@ __global__ __launch_bounds__(256) void trace_lanes_kernel(scene_dev s, frame_par
     if(error) atomicExch(&ctl->failure, 1);
     if(COUNT) {
         n_box = wave_sum(n_box); n_leaf = wave_sum(n_leaf); n_sph = wave_sum(n_sph);
+        n_hits = wave_sum(n_hits); n_rbox = wave_sum(n_rbox); n_rleaf = wave_sum(n_rleaf);
         if(lane == 0) {
             atomicAdd(&ctl->box_tests, n_box);
             atomicAdd(&ctl->triangle_tests, n_leaf - n_sph);
             atomicAdd(&ctl->sphere_tests, n_sph);
+            atomicAdd(&ctl->hits, n_hits);
+            atomicAdd(&ctl->retrace_branches, n_rbox);
+            atomicAdd(&ctl->retrace_leaves, n_rleaf);
         }
     }
 }

@@ -303,6 +303,8 @@ PT_HD float widened(const wide_query& q, float limit) {
 }
 
 // Returns true when the query has finished (or stopped for a retrace).
+// STATS counts wide nodes visited in `boxes` and leaf records tested in
+// `prims` (spheres among them in `spheres`).
 // WINDOW: the band in ulps; 0 switches the reference-order fallback off
 // altogether (the CPU model runs it so to show what the fallback changes).
 template<bool STATS, class STACK, uint32_t WINDOW = wide_window_ulps>
@@ -320,7 +322,7 @@ PT_HD bool wide_step(const scene_dev& s, wide_query& q, STACK& stk, int& boxes, 
         slab_entry4(LX, LY, LZ, HX, HY, HZ, r, low, widened<WINDOW>(q, limit), hh, e);
         const bool h0 = (r0 != wide_empty) & hh[0], h1 = (r1 != wide_empty) & hh[1];
         const bool h2 = (r2 != wide_empty) & hh[2], h3 = (r3 != wide_empty) & hh[3];
-        if(STATS) boxes += (r0 != wide_empty) + (r1 != wide_empty) + (r2 != wide_empty) + (r3 != wide_empty);
+        if(STATS) boxes++;  // wide nodes visited (each a 128-byte record)
         if(!h0) e[0] = INFINITY;
         if(!h1) e[1] = INFINITY;
         if(!h2) e[2] = INFINITY;
@@ -426,6 +428,13 @@ PT_HD bool exact_retrace(const scene_dev& s, int32_t broot, const ray_in& r, STA
     int boxes = 0, prims = 0, spheres = 0;
     wide_as_binary<STACK> bs{stk};
     return traverse_bvh<false, false>(s, broot, r, 0.001f, FLT_MAX, bs, best, boxes, prims, spheres, error);
+}
+// The same, counting the binary traversal's branch and leaf pops.
+template<class STACK>
+PT_HD bool exact_retrace_counted(const scene_dev& s, int32_t broot, const ray_in& r, STACK& stk, best_hit& best,
+                                 int& error, int& boxes, int& prims, int& spheres) {
+    wide_as_binary<STACK> bs{stk};
+    return traverse_bvh<false, true>(s, broot, r, 0.001f, FLT_MAX, bs, best, boxes, prims, spheres, error);
 }
 
 // A whole 4-wide query in one call (wavefront trace kernel, drain kernels),

@@ -100,7 +100,8 @@ class LaunchStats(ctypes.Structure):
                 ("regen_ms", ctypes.c_float),
                 ("box_tests", ctypes.c_uint64), ("triangle_tests", ctypes.c_uint64), ("sphere_tests", ctypes.c_uint64),
                 ("drained_paths", ctypes.c_uint64), ("drain_ms", ctypes.c_float), ("split_launches", ctypes.c_int32),
-                ("retraced_queries", ctypes.c_uint64)]
+                ("retraced_queries", ctypes.c_uint64), ("hit_queries", ctypes.c_uint64),
+                ("retrace_box_tests", ctypes.c_uint64), ("retrace_primitive_tests", ctypes.c_uint64)]
 
 
 class Tuning(ctypes.Structure):

@@ -190,6 +190,12 @@ typedef struct pt_launch_stats {
     uint64_t retraced_queries; /* closest-hit queries of the 4-wide traversal traced again in the reference's
                                   order (order-sensitive cases); counted by the path-lane and persistent-lane
                                   trace kernels */
+    /* counting pass (count_tests) on the 4-wide persistent-lane trace kernel: box_tests then counts wide
+       nodes visited and triangle_tests/sphere_tests leaf records tested; these count the closest-hit queries
+       that hit and the binary branch and leaf pops of the retraces */
+    uint64_t hit_queries;
+    uint64_t retrace_box_tests;
+    uint64_t retrace_primitive_tests;
 } pt_launch_stats;
 pt_status pt_renderer_last_launch_stats(const pt_renderer *renderer, pt_launch_stats *stats);
 
