@@ -66,10 +66,17 @@ def build_scene():
 
 
 def build_cli():
-    """pt_render: the command-line renderer over both libraries."""
+    """pt_render: the command-line renderer over both libraries.  Without
+    --obj it loads models/blender_sphere/sphere.obj from the asset directory
+    named by PT_ASSET_DIR at build time ("assets", relative to the working
+    directory, when unset), as the reference's RAY_TRACER_ASSET_DIR."""
     out = os.path.join(HERE, "pt_render")
     tmp = out + ".tmp"
-    _run(["g++", "-O2", "-std=c++17", "-Wall", "-Wextra", "-I", INCLUDE, os.path.join(CSRC, "pt_cli.cpp"),
+    asset_dir = os.environ.get("PT_ASSET_DIR", "assets")
+    if '"' in asset_dir or "\\" in asset_dir:
+        raise ValueError("PT_ASSET_DIR must not contain quotes or backslashes")
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-Wextra", "-I", INCLUDE, f'-DPT_ASSET_DIR="{asset_dir}"',
+          os.path.join(CSRC, "pt_cli.cpp"),
           "-L", HERE, "-lptscene", "-lptrace", "-Wl,-rpath,$ORIGIN", "-o", tmp])
     os.replace(tmp, out)
     return out

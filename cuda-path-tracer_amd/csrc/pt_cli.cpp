@@ -59,18 +59,16 @@ float parse_float(const char *value) {
     return parsed;
 }
 
-std::string executable_dir() {
-    std::vector<char> buf(4096);
-    ssize_t n = readlink("/proc/self/exe", buf.data(), buf.size() - 1);
-    if(n <= 0) return ".";
-    std::string path(buf.data(), size_t(n));
-    size_t slash = path.rfind('/');
-    return slash == std::string::npos ? "." : path.substr(0, slash);
-}
+// The asset directory the default OBJ file is read from, fixed when the CLI
+// is built (render_options.h:19 uses the reference's RAY_TRACER_ASSET_DIR);
+// the build sets it from its PT_ASSET_DIR environment variable.
+#ifndef PT_ASSET_DIR
+#define PT_ASSET_DIR "assets"
+#endif
 
 options parse_options(int argc, char **argv) {
     options o;
-    o.obj_file = executable_dir() + "/../tests/golden/assets/models/blender_sphere/sphere.obj";
+    o.obj_file = PT_ASSET_DIR "/models/blender_sphere/sphere.obj";
     bool has_transform = false, has_obj = false;
     for(int i = 1; i < argc; i++) {
         std::string a = argv[i];

@@ -107,7 +107,9 @@ void inflate_block(bit_reader& in, const huffman& lit, const huffman& dist, std:
 std::vector<uint8_t> inflate(const uint8_t *data, size_t size, size_t expected) {
     bit_reader in(data, size);
     std::vector<uint8_t> out;
-    out.reserve(expected);
+    // DEFLATE expands at most 1032-fold (258-byte copies in 2 bits): no more
+    // is reserved than the stream can produce.
+    out.reserve(std::min(expected, size * 1032 + 1024));
     huffman fixed_lit, fixed_dist;
     bool fixed_ready = false;
     int last = 0;
@@ -244,6 +246,9 @@ std::vector<uint8_t> decode_png(const std::vector<uint8_t>& file, int& width, in
     bool depth_ok = depth == 8 || (depth == 16 && color != 3) ||
                     ((depth == 1 || depth == 2 || depth == 4) && (color == 0 || color == 3));
     if(!depth_ok) corrupt("bad PNG bit depth");
+    // stb_image's size limit for PNG: at most 2^30 bytes of samples per
+    // channel count, in its integer form (it rejects larger files as too large)
+    if((1u << 30) / w / uint32_t(channels) < h) corrupt("PNG image too large");
     if(color == 3 && (palette.empty() || palette.size() % 3 || palette.size() > 768)) corrupt("bad PNG palette");
     if(idat.size() < 2) corrupt("missing PNG image data");
     if((idat[0] & 15) != 8 || ((idat[0] << 8) | idat[1]) % 31 || (idat[1] & 32)) corrupt("bad zlib header");
@@ -793,27 +798,32 @@ private:
     }
 
     // Integer IDCT (jidctint "islow" with 12-bit constants; stb's rounding).
+    // The sums and products wrap modulo 2^32, as stb's int arithmetic does on
+    // the targets it is built for, but without undefined behaviour: extreme
+    // coefficients of a malformed file can overflow 32 bits.
     static int fix(float x) { return int(double(x * 4096) + 0.5); }
     static uint8_t clamp255(int x) { return uint8_t(x < 0 ? 0 : x > 255 ? 255 : x); }
-    struct idct_terms { int t0, t1, t2, t3, x0, x1, x2, x3; };
+    using w32 = uint32_t;
+    static int shifted(w32 x, int bits) { return int32_t(x) >> bits; }
+    struct idct_terms { w32 t0, t1, t2, t3, x0, x1, x2, x3; };
     static idct_terms idct_1d(int s0, int s1, int s2, int s3, int s4, int s5, int s6, int s7) {
-        static const int c0541 = fix(0.5411961f), cm1847 = fix(-1.847759065f), c0765 = fix(0.765366865f);
-        static const int c1175 = fix(1.175875602f), c0298 = fix(0.298631336f), c2053 = fix(2.053119869f);
-        static const int c3072 = fix(3.072711026f), c1501 = fix(1.501321110f), cm0899 = fix(-0.899976223f);
-        static const int cm2562 = fix(-2.562915447f), cm1961 = fix(-1.961570560f), cm0390 = fix(-0.390180644f);
+        static const w32 c0541 = w32(fix(0.5411961f)), cm1847 = w32(fix(-1.847759065f)), c0765 = w32(fix(0.765366865f));
+        static const w32 c1175 = w32(fix(1.175875602f)), c0298 = w32(fix(0.298631336f)), c2053 = w32(fix(2.053119869f));
+        static const w32 c3072 = w32(fix(3.072711026f)), c1501 = w32(fix(1.501321110f)), cm0899 = w32(fix(-0.899976223f));
+        static const w32 cm2562 = w32(fix(-2.562915447f)), cm1961 = w32(fix(-1.961570560f)), cm0390 = w32(fix(-0.390180644f));
         idct_terms o;
-        int p2 = s2, p3 = s6;
-        int p1 = (p2 + p3) * c0541;
-        int t2 = p1 + p3 * cm1847, t3 = p1 + p2 * c0765;
-        p2 = s0; p3 = s4;
-        int t0 = (p2 + p3) * 4096, t1 = (p2 - p3) * 4096;
+        w32 p2 = w32(s2), p3 = w32(s6);
+        w32 p1 = (p2 + p3) * c0541;
+        w32 t2 = p1 + p3 * cm1847, t3 = p1 + p2 * c0765;
+        p2 = w32(s0); p3 = w32(s4);
+        w32 t0 = (p2 + p3) * 4096u, t1 = (p2 - p3) * 4096u;
         o.x0 = t0 + t3; o.x3 = t0 - t3; o.x1 = t1 + t2; o.x2 = t1 - t2;
-        t0 = s7; t1 = s5; t2 = s3; t3 = s1;
+        t0 = w32(s7); t1 = w32(s5); t2 = w32(s3); t3 = w32(s1);
         p3 = t0 + t2;
-        int p4 = t1 + t3;
+        w32 p4 = t1 + t3;
         p1 = t0 + t3;
         p2 = t1 + t2;
-        const int p5 = (p3 + p4) * c1175;
+        const w32 p5 = (p3 + p4) * c1175;
         t0 = t0 * c0298; t1 = t1 * c2053; t2 = t2 * c3072; t3 = t3 * c1501;
         p1 = p5 + p1 * cm0899; p2 = p5 + p2 * cm2562;
         p3 = p3 * cm1961; p4 = p4 * cm0390;
@@ -832,21 +842,21 @@ private:
             }
             idct_terms o = idct_1d(c[0], c[8], c[16], c[24], c[32], c[40], c[48], c[56]);
             o.x0 += 512; o.x1 += 512; o.x2 += 512; o.x3 += 512;
-            v[i] = (o.x0 + o.t3) >> 10; v[i + 56] = (o.x0 - o.t3) >> 10;
-            v[i + 8] = (o.x1 + o.t2) >> 10; v[i + 48] = (o.x1 - o.t2) >> 10;
-            v[i + 16] = (o.x2 + o.t1) >> 10; v[i + 40] = (o.x2 - o.t1) >> 10;
-            v[i + 24] = (o.x3 + o.t0) >> 10; v[i + 32] = (o.x3 - o.t0) >> 10;
+            v[i] = shifted(o.x0 + o.t3, 10); v[i + 56] = shifted(o.x0 - o.t3, 10);
+            v[i + 8] = shifted(o.x1 + o.t2, 10); v[i + 48] = shifted(o.x1 - o.t2, 10);
+            v[i + 16] = shifted(o.x2 + o.t1, 10); v[i + 40] = shifted(o.x2 - o.t1, 10);
+            v[i + 24] = shifted(o.x3 + o.t0, 10); v[i + 32] = shifted(o.x3 - o.t0, 10);
         }
         for(int r = 0; r < 8; r++) {
             const int *w = v + 8 * r;
             idct_terms o = idct_1d(w[0], w[1], w[2], w[3], w[4], w[5], w[6], w[7]);
-            const int bias = 65536 + (128 << 17);
+            const w32 bias = 65536 + (128 << 17);
             o.x0 += bias; o.x1 += bias; o.x2 += bias; o.x3 += bias;
             uint8_t *p = out + size_t(r) * size_t(stride);
-            p[0] = clamp255((o.x0 + o.t3) >> 17); p[7] = clamp255((o.x0 - o.t3) >> 17);
-            p[1] = clamp255((o.x1 + o.t2) >> 17); p[6] = clamp255((o.x1 - o.t2) >> 17);
-            p[2] = clamp255((o.x2 + o.t1) >> 17); p[5] = clamp255((o.x2 - o.t1) >> 17);
-            p[3] = clamp255((o.x3 + o.t0) >> 17); p[4] = clamp255((o.x3 - o.t0) >> 17);
+            p[0] = clamp255(shifted(o.x0 + o.t3, 17)); p[7] = clamp255(shifted(o.x0 - o.t3, 17));
+            p[1] = clamp255(shifted(o.x1 + o.t2, 17)); p[6] = clamp255(shifted(o.x1 - o.t2, 17));
+            p[2] = clamp255(shifted(o.x2 + o.t1, 17)); p[5] = clamp255(shifted(o.x2 - o.t1, 17));
+            p[3] = clamp255(shifted(o.x3 + o.t0, 17)); p[4] = clamp255(shifted(o.x3 - o.t0, 17));
         }
     }
 

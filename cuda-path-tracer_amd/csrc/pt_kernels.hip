@@ -133,13 +133,14 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
 // ---------------- hit record (flat_intersect.h:28-42,59-64; transport.h:39) ----------------
 struct hit_rec { float t, u, v; v3 p, normal; int32_t material_id, primitive_type, primitive_id; };
 
-// src/geometry/sphere_uv.h:7-13.  atan2/asin are evaluated in double and
-// rounded, so u,v can differ from a float libm's result in the last bit.
-__device__ __noinline__ void sphere_uv(v3 p, float& u, float& v) {
+// src/geometry/sphere_uv.h:7-13: {u, v}.  atan2/asin are evaluated in double
+// and rounded, so u,v can differ from a float libm's result in the last bit.
+// (Returned by value: out-parameters of an out-of-line call would keep the
+// caller's hit record in private memory.)
+__device__ __noinline__ float2 sphere_uv(v3 p) {
     float phi = float(atan2(double(p.z), double(p.x)));
     float theta = float(asin(double(p.y)));
-    u = float(1 - (double(phi) + M_PI) / (2 * M_PI));
-    v = float((double(theta) + M_PI / 2) / M_PI);
+    return make_float2(float(1 - (double(phi) + M_PI) / (2 * M_PI)), float((double(theta) + M_PI / 2) / M_PI));
 }
 
 __device__ hit_rec make_hit(const scene_dev& s, v3 o, v3 d, const best_hit& b) {
@@ -152,7 +153,8 @@ __device__ hit_rec make_hit(const scene_dev& s, v3 o, v3 d, const best_hit& b) {
         const pt_flat_sphere& sp = s.spheres[prim];
         v3 c = mk(sp.center);
         h.normal = divs(sub(h.p, c), sp.radius);
-        sphere_uv(h.normal, h.u, h.v);
+        const float2 uv = sphere_uv(h.normal);
+        h.u = uv.x; h.v = uv.y;
         h.material_id = sp.material_id;
         h.primitive_type = PT_SPHERE;
     } else {
@@ -277,7 +279,8 @@ __device__ light_sample sample_light(const scene_dev& s, const light_set& ls, rn
         res.normal = mk(radius * cs.x, z, radius * cs.y);
         res.point = add(mk(sp.center), scale(fabsf(sp.radius), res.normal));
         if(sp.radius < 0) res.normal = neg(res.normal);
-        sphere_uv(res.normal, res.u, res.v);
+        const float2 uv = sphere_uv(res.normal);
+        res.u = uv.x; res.v = uv.y;
         res.material = sp.material_id;
     }
     return res;

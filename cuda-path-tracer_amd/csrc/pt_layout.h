@@ -6,6 +6,7 @@
 // (pt_api.hip) and by the test suite's CPU model of the traversal.
 #pragma once
 
+#include <array>
 #include <cfloat>
 #include <cmath>
 #include <cstring>
@@ -312,6 +313,70 @@ inline bool build_wide(const pt_scene_desc& s, const std::vector<float4>& ti, wi
         double dx = double(b.maximum.x) - b.minimum.x, dy = double(b.maximum.y) - b.minimum.y, dz = double(b.maximum.z) - b.minimum.z;
         return dx * dy + dy * dz + dz * dx;
     };
+    // Which binary nodes become wide nodes: the cut that minimizes the
+    // surface-area cost (a slot's work weighted by the surface area of the box
+    // it is reached through), by dynamic programming over the binary tree,
+    // children before parents.  F[x][k] is the least cost of covering the
+    // subtree of x with at most k slots of one wide node; a slot is a single
+    // leaf (reached through its binary parent's box), a cluster of at most
+    // wide_leaf_max leaves, or a wide node of its own (slot_cost).
+    std::vector<int32_t> parent(static_cast<size_t>(n), -1), post;
+    post.reserve(size_t(n));
+    {
+        std::vector<std::pair<int32_t, bool>> walk{{s.root, false}};
+        while(!walk.empty()) {
+            auto [node, exit] = walk.back(); walk.pop_back();
+            const auto& nd = s.nodes[node];
+            if(exit || nd.type != PT_BRANCH) { post.push_back(node); continue; }
+            walk.push_back({node, true});
+            parent[size_t(nd.left)] = node;
+            walk.push_back({nd.left, false});
+            if(nd.right != nd.left) { parent[size_t(nd.right)] = node; walk.push_back({nd.right, false}); }
+        }
+    }
+    constexpr double node_cost = 1.0, leaf_base = 0.3, leaf_each = 0.5;
+    std::vector<std::array<double, 5>> F(static_cast<size_t>(n));
+    std::vector<std::array<uint8_t, 5>> split(static_cast<size_t>(n));  // (i << 4) | j, or 0 for one slot
+    std::vector<double> slot_cost(static_cast<size_t>(n));
+    auto children = [&](int32_t x, int32_t& l, int32_t& r) {
+        l = s.nodes[x].left; r = s.nodes[x].right;
+        return l != r ? 2 : 1;
+    };
+    for(int32_t x : post) {
+        const auto& nd = s.nodes[x];
+        const size_t xi = size_t(x);
+        if(nd.type != PT_BRANCH) {
+            const int32_t p = parent[xi] >= 0 ? parent[xi] : x;
+            slot_cost[xi] = area(p) * (leaf_base + leaf_each);
+            F[xi].fill(slot_cost[xi]);
+            split[xi].fill(0);
+            continue;
+        }
+        int32_t l, r;
+        const int m = children(x, l, r);
+        // best cover of x's subtree below x with at most k slots (k >= 2)
+        auto below = [&](int k, uint8_t& how) {
+            if(m == 1) { how = uint8_t((k << 4) | 0); return F[size_t(l)][size_t(k)]; }
+            double best = INFINITY;
+            for(int i = 1; i < k; i++)
+                for(int j = 1; i + j <= k; j++) {
+                    const double c = F[size_t(l)][size_t(i)] + F[size_t(r)][size_t(j)];
+                    if(c < best) { best = c; how = uint8_t((i << 4) | j); }
+                }
+            return best;
+        };
+        const int32_t leaves = count[xi];
+        uint8_t how = 0;
+        if(leaves <= wide_leaf_max) slot_cost[xi] = area(x) * (leaf_base + leaf_each * leaves);
+        else slot_cost[xi] = area(x) * node_cost + below(4, how);
+        F[xi][1] = slot_cost[xi];
+        split[xi][1] = 0;
+        for(int k = 2; k <= 4; k++) {
+            const double c = below(k, how);
+            if(c < slot_cost[xi]) { F[xi][size_t(k)] = c; split[xi][size_t(k)] = how; }
+            else { F[xi][size_t(k)] = slot_cost[xi]; split[xi][size_t(k)] = 0; }
+        }
+    }
     // A cluster of count <= wide_leaf_max leaves from depth-first position first; note that
     // the first leaf alone encodes as -1, a valid reference.
     auto cluster_ref = [&](int32_t node) { return ~((first[size_t(node)] << 3) | (count[size_t(node)] - 1)); };
@@ -349,27 +414,38 @@ inline bool build_wide(const pt_scene_desc& s, const std::vector<float4>& ti, wi
         dst[7] = make_float4(0, 0, 0, 0);
         return int32_t(w);
     };
+    // The slots covering the subtree of y with at most k of them (y's binary
+    // parent p), left to right.
+    std::function<void(int32_t, int32_t, int, std::vector<kid>&)> cover = [&](int32_t y, int32_t p, int k,
+                                                                             std::vector<kid>& kids) {
+        const uint8_t how = k >= 2 ? split[size_t(y)][size_t(k)] : 0;
+        if(how == 0) { kids.push_back({y, p}); return; }
+        int32_t l, r;
+        const int m = children(y, l, r);
+        if(m == 1) { cover(l, y, how >> 4, kids); return; }
+        cover(l, y, how >> 4, kids);
+        cover(r, y, how & 15, kids);
+    };
     make = [&](int32_t b, int depth) -> int32_t {
         if(!expandable(b)) return cluster_ref(b);
         std::vector<kid> kids;
-        const auto& nb = s.nodes[b];
-        kids.push_back({nb.left, b});
-        if(nb.left != nb.right) kids.push_back({nb.right, b});
-        while(kids.size() < 4) {
-            int best = -1; double best_area = -1;
-            for(size_t k = 0; k < kids.size(); k++)
-                if(expandable(kids[k].node) && area(kids[k].node) > best_area) { best = int(k); best_area = area(kids[k].node); }
-            if(best < 0) break;
-            const int32_t split = kids[size_t(best)].node;
-            const auto& nk = s.nodes[split];
-            std::vector<kid> repl{{nk.left, split}};
-            if(nk.left != nk.right) repl.push_back({nk.right, split});
-            kids.erase(kids.begin() + best);
-            kids.insert(kids.begin() + best, repl.begin(), repl.end());
-            if(kids.size() > 4) {  // cannot happen: one slot becomes at most two
-                failed = true;
-                return 0;
-            }
+        int32_t l, r;
+        if(children(b, l, r) == 1) cover(l, b, 4, kids);
+        else {
+            // the split of b's subtree into at most 4 slots that slot_cost[b] was computed with
+            double best = INFINITY;
+            int bi = 1, bj = 1;
+            for(int i = 1; i < 4; i++)
+                for(int j = 1; i + j <= 4; j++) {
+                    const double c = F[size_t(l)][size_t(i)] + F[size_t(r)][size_t(j)];
+                    if(c < best) { best = c; bi = i; bj = j; }
+                }
+            cover(l, b, bi, kids);
+            cover(r, b, bj, kids);
+        }
+        if(kids.empty() || kids.size() > 4) {  // cannot happen: at most 4 slots by construction
+            failed = true;
+            return 0;
         }
         return emit(kids, depth);
     };

@@ -215,6 +215,14 @@ struct seeded {
     std::mt19937_64 engine;
     explicit seeded(uint64_t s) : engine(s) {}
     float uniform(float lo, float hi) { return lo + (hi - lo) * float(double(engine() >> 11) * 0x1p-53); }
+    // Three draws as a vector: z first, then y, then x.  Every multi-draw
+    // expression of the synthetic scenes is written out in the order the
+    // scenes were first generated with (right to left, as GCC evaluates
+    // function arguments), since C++ leaves that order unspecified.
+    pt_vec3 uniform3(float lo, float hi) {
+        const float z = uniform(lo, hi), y = uniform(lo, hi);
+        return v3(uniform(lo, hi), y, z);
+    }
 };
 
 // Value noise on the integer lattice, hashed with the seed; smoothstep-trilinear.
@@ -324,9 +332,10 @@ pts_scene *pts_cornell(uint64_t seed, int32_t width, int32_t height, pt_flat_cam
         quad(tris, v3(-0.3f, ly, -0.3f), v3(-0.3f, ly, 0.3f), v3(0.3f, ly, 0.3f), v3(0.3f, ly, -0.3f), lamp);
         std::vector<pt_flat_sphere> sph;
         int32_t mats[3];
-        mats[0] = diffuse(*s, v3(rng.uniform(0.1f, 0.9f), rng.uniform(0.1f, 0.9f), rng.uniform(0.1f, 0.9f)));
-        pt_vec3 metal_albedo = v3(rng.uniform(0.5f, 1), rng.uniform(0.5f, 1), rng.uniform(0.5f, 1));
-        mats[1] = add_material(*s, PT_METAL, metal_albedo, rng.uniform(0, 0.3f), 1, -1);
+        mats[0] = diffuse(*s, rng.uniform3(0.1f, 0.9f));
+        const pt_vec3 metal_albedo = rng.uniform3(0.5f, 1);
+        const float metal_fuzz = rng.uniform(0, 0.3f);
+        mats[1] = add_material(*s, PT_METAL, metal_albedo, metal_fuzz, 1, -1);
         mats[2] = add_material(*s, PT_GLASS, v3(0, 0, 0), 0, 1.5f, -1);
         for(int k = 0; k < 3; k++) {
             for(int attempt = 0; attempt < 10000; attempt++) {
@@ -385,8 +394,11 @@ pts_scene *pts_icosphere(int32_t frequency, uint64_t seed, int32_t width, int32_
         std::vector<int32_t> cluster_material(static_cast<size_t>(20 * K * K));
         for(auto& m : cluster_material) {
             float pick = rng.uniform(0, 1);
-            if(pick < 0.60f) m = diffuse(*s, v3(rng.uniform(0.1f, 0.9f), rng.uniform(0.1f, 0.9f), rng.uniform(0.1f, 0.9f)));
-            else if(pick < 0.85f) m = add_material(*s, PT_METAL, v3(rng.uniform(0.5f, 1), rng.uniform(0.5f, 1), rng.uniform(0.5f, 1)), rng.uniform(0, 0.3f), 1, -1);
+            if(pick < 0.60f) m = diffuse(*s, rng.uniform3(0.1f, 0.9f));
+            else if(pick < 0.85f) {
+                const float fuzz = rng.uniform(0, 0.3f);
+                m = add_material(*s, PT_METAL, rng.uniform3(0.5f, 1), fuzz, 1, -1);
+            }
             else m = add_material(*s, PT_GLASS, v3(0, 0, 0), 0, 1.5f, -1);
         }
         struct face_ref { int32_t a, b, c, material; };
@@ -446,11 +458,15 @@ pts_scene *pts_soup(int64_t triangle_count, uint64_t seed, int32_t width, int32_
         std::vector<float> spread(static_cast<size_t>(clusters));
         std::vector<int32_t> mat(static_cast<size_t>(clusters));
         for(int c = 0; c < clusters; c++) {
-            centre[size_t(c)] = v3(rng.uniform(-2, 2), rng.uniform(-1, 1.5f), rng.uniform(-2, 2));
+            const float cz = rng.uniform(-2, 2), cy = rng.uniform(-1, 1.5f), cx = rng.uniform(-2, 2);
+            centre[size_t(c)] = v3(cx, cy, cz);
             spread[size_t(c)] = rng.uniform(0.1f, 0.5f);
             float pick = rng.uniform(0, 1);
-            if(pick < 0.6f) mat[size_t(c)] = diffuse(*s, v3(rng.uniform(0.1f, 0.9f), rng.uniform(0.1f, 0.9f), rng.uniform(0.1f, 0.9f)));
-            else if(pick < 0.85f) mat[size_t(c)] = add_material(*s, PT_METAL, v3(rng.uniform(0.5f, 1), rng.uniform(0.5f, 1), rng.uniform(0.5f, 1)), rng.uniform(0, 0.3f), 1, -1);
+            if(pick < 0.6f) mat[size_t(c)] = diffuse(*s, rng.uniform3(0.1f, 0.9f));
+            else if(pick < 0.85f) {
+                const float fuzz = rng.uniform(0, 0.3f);
+                mat[size_t(c)] = add_material(*s, PT_METAL, rng.uniform3(0.5f, 1), fuzz, 1, -1);
+            }
             else mat[size_t(c)] = add_material(*s, PT_GLASS, v3(0, 0, 0), 0, 1.5f, -1);
         }
         std::vector<pt_flat_triangle> tris;
@@ -459,9 +475,12 @@ pts_scene *pts_soup(int64_t triangle_count, uint64_t seed, int32_t width, int32_
         for(int64_t i = 0; i < triangle_count; i++) {
             int c = int(rng.engine() % uint64_t(clusters));
             float sp = spread[size_t(c)];
-            pt_vec3 p = centre[size_t(c)] + v3(sp * gauss(rng.engine), sp * gauss(rng.engine), sp * gauss(rng.engine));
-            pt_vec3 a = p + v3(edge * rng.uniform(-1, 1), edge * rng.uniform(-1, 1), edge * rng.uniform(-1, 1));
-            pt_vec3 b = p + v3(edge * rng.uniform(-1, 1), edge * rng.uniform(-1, 1), edge * rng.uniform(-1, 1));
+            const float gz = gauss(rng.engine), gy = gauss(rng.engine), gx = gauss(rng.engine);
+            const pt_vec3 p = centre[size_t(c)] + v3(sp * gx, sp * gy, sp * gz);
+            const pt_vec3 da = rng.uniform3(-1, 1);
+            const pt_vec3 a = p + v3(edge * da.x, edge * da.y, edge * da.z);
+            const pt_vec3 db = rng.uniform3(-1, 1);
+            const pt_vec3 b = p + v3(edge * db.x, edge * db.y, edge * db.z);
             tris.push_back(flat_tri(p, a, b, mat[size_t(c)]));
         }
         int32_t floor_m = diffuse(*s, v3(0.5f, 0.5f, 0.5f));

@@ -26,8 +26,8 @@ def build(sanitize=False):
     flags = ["--offload-arch=gfx950", "-O2" if sanitize else "-O3", "-std=c++17", "-fPIC", "-shared",
              "-ffp-contract=off", "-Wall", "-Wno-unused-function", "-I", INCLUDE, "-I", CSRC]
     if sanitize:
-        flags += ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
-                  "-Xarch_host", "-fno-sanitize-recover=undefined", "-g"]
+        # UndefinedBehaviorSanitizer halts at run time (UBSAN_OPTIONS=halt_on_error=1)
+        flags += ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined", "-g"]
     proc = subprocess.run([HIPCC, *flags, os.path.join(HERE, "wide_model.hip"), "-o", tmp],
                           stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     lines = [l for l in proc.stdout.splitlines() if "packed-fp32-ops" not in l and l.strip()]
