@@ -1,11 +1,16 @@
 """Batched rendering (pt_batch_*): many views of many scenes in one
 graph-captured bounce loop.  Every view must equal what a single renderer
 makes of the same scene, camera, size and seed, bit for bit."""
+import os
+
 import numpy as np
 import pytest
 
+import oracle
 import ptrace
 from conftest import golden
+
+THREADS = min(16, len(os.sched_getaffinity(0)))
 
 
 def _scenes():
@@ -34,6 +39,43 @@ def test_batch_views_match_single_renderers(depth, direct):
         total += r.rays_traced()
     st = batch.stats()
     assert st["samples"] == 5 and st["rays"] == total
+
+
+@pytest.mark.gpu
+def test_batch_of_benchmark_sized_scenes_matches_single_renderers_and_oracle():
+    # The batched configuration's range: clustered triangle soups of 10k to
+    # 2M triangles (the batch benchmark's generator), uneven resolutions from
+    # 256x256 to 1280x720, one scene seen twice, path depth 6 with MIS.
+    # Every view equals a single renderer bit for bit (and the batch's ray
+    # count their sum); every single renderer equals the oracle on the whole
+    # frame: per-pixel sums, PCG states and the exact ray count.
+    sizes = [10_000, 60_000, 300_000, 1_000_000, 2_000_000]
+    shapes = [(256, 256), (640, 480), (1024, 576), (800, 600), (1280, 720)]
+    built = [ptrace.soup(n, 100 + k, w, h) for k, (n, (w, h)) in enumerate(zip(sizes, shapes))]
+    scenes = [b[0] for b in built]
+    views = [(k, built[k][1], w, h, 31 + 7 * k) for k, (w, h) in enumerate(shapes)]
+    # the 2M-triangle scene again, from its generator's camera for 512x384
+    views.append((4, ptrace.soup(16, 104, 512, 384)[1], 512, 384, 73))
+    batch = ptrace.Batch(scenes, views, direct_light=True, max_depth=6)
+    batch.add_samples(2)
+    batch.add_samples(1)
+    total = 0
+    for k, (si, cam, w, h, seed) in enumerate(views):
+        print(f"view {k}: {len(scenes[si].triangles)} triangles, {w}x{h}", flush=True)
+        r = ptrace.Renderer(scenes[si], cam, w, h, seed, False, ptrace.DEBUG_SHADED, True, 6)
+        r.add_samples(2)
+        r.add_samples(1)
+        assert np.array_equal(batch.image(k), r.image()), k
+        total += r.rays_traced()
+        o = oracle.Renderer(scenes[si], cam, w, h, seed, False, 0, True, 6)
+        o.add_samples(2, threads=THREADS)
+        o.add_samples(1, threads=THREADS)
+        sums, rng = r.read_state()
+        assert np.array_equal(sums, o.sums()), k
+        assert np.array_equal(rng, o.rng_state()), k
+        assert r.rays_traced() == o.rays_traced(), k
+    st = batch.stats()
+    assert st["samples"] == 3 and st["rays"] == total
 
 
 @pytest.mark.gpu
