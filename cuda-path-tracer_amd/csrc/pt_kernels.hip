@@ -1182,6 +1182,12 @@ __global__ __launch_bounds__(path_lane_block) __attribute__((amdgpu_waves_per_eu
     const int lane = __lane_id();
     wide_stack_t<path_lane_block> wstk{reinterpret_cast<int2 *>(lds) + threadIdx.x, int(blockDim.x), p.spill,
                     size_t(gridDim.x) * blockDim.x, blockIdx.x * blockDim.x + threadIdx.x};
+    // The running sum of the lane's pixel, [3][path_lane_block] after the
+    // stack's LDS entries: read when the pixel is taken, written back when
+    // it is left, so that a sample's accumulation does not go through HBM
+    // (the additions and their order are those of sum += radiance).
+    float *acc = reinterpret_cast<float *>(reinterpret_cast<int2 *>(lds) + wide_lds_entries * path_lane_block) +
+                 threadIdx.x;
     int error = 0, boxes = 0, prims = 0, spheres = 0;
     uint32_t ext = 0, shadow = 0;   // queries this lane began, flushed to the totals before 2^31
     auto flush_rays = [&]() {
@@ -1211,11 +1217,14 @@ __global__ __launch_bounds__(path_lane_block) __attribute__((amdgpu_waves_per_eu
     // After a bounce's queries: the next bounce, the next sample, or done.
     auto next_bounce = [&](bool cont) {
         if(cont) { begin_ext(); return; }
-        float *sum = p.sum + 3 * size_t(pix);
-        sum[0] = sum[0] + st.radiance.x; sum[1] = sum[1] + st.radiance.y; sum[2] = sum[2] + st.radiance.z;
+        acc[0] = acc[0] + st.radiance.x;
+        acc[path_lane_block] = acc[path_lane_block] + st.radiance.y;
+        acc[2 * path_lane_block] = acc[2 * path_lane_block] + st.radiance.z;
         if(--st.left <= 0 || (MODE == PATHS_FIRST && st.left <= stop_left)) {
             // Done, or (first pass) stopped at a sample boundary: the next
             // sample's camera ray is drawn when the pixel is taken again.
+            float *sum = p.sum + 3 * size_t(pix);
+            sum[0] = acc[0]; sum[1] = acc[path_lane_block]; sum[2] = acc[2 * path_lane_block];
             p.rng[pix] = r.state;
             if constexpr(MODE != PATHS_FRESH) p.ctr[pix] = make_int2(0, st.left > 0 ? st.left : 0);
             if constexpr(MODE == PATHS_FIRST) costs[claim] = st.left > 0 ? cost : 0u;
@@ -1244,6 +1253,10 @@ __global__ __launch_bounds__(path_lane_block) __attribute__((amdgpu_waves_per_eu
                     }
                     r = pixel_rng(p.rng[pix], local_index<BATCH>(bd, pix));
                     root = query_root<BATCH>(s, bd, pix);
+                    {
+                        const float *sum = p.sum + 3 * size_t(pix);
+                        acc[0] = sum[0]; acc[path_lane_block] = sum[1]; acc[2 * path_lane_block] = sum[2];
+                    }
                     if constexpr(MODE == PATHS_REST || MODE == PATHS_FRESH) {
                         st.left = MODE == PATHS_FRESH ? first_samples : p.ctr[pix].y;
                         if(st.left > 0) {
@@ -1516,8 +1529,8 @@ static void path_lanes_mode(const scene_dev& s, const frame_params& f, const pat
 void launch_path_lanes(const scene_dev& s, const frame_params& f, const paths_dev& p, loop_ctl *ctl, int parity,
                        const int32_t *queue, uint32_t paths, int block, path_lanes lanes, path_pass pass,
                        hipStream_t stream, const batch_dev *batch) {
-    // The kernel keeps wide_lds_entries stack entries per lane in LDS; the
-    // size below covers them only for the wide traversal.
+    // The kernel keeps wide_lds_entries stack entries per lane in LDS (the
+    // binary stack of a retrace uses the same entries).
     if(!wide_traversal(s, f)) throw std::logic_error("path lanes need the 4-wide traversal");
     if(block != path_lane_block) throw std::logic_error("path lanes run blocks of path_lane_block threads");
     if(pass.mode == PATHS_FIRST && (!pass.costs || pass.first_samples <= 0))
@@ -1525,7 +1538,8 @@ void launch_path_lanes(const scene_dev& s, const frame_params& f, const paths_de
     if(pass.mode == PATHS_FRESH && (batch || pass.first_samples <= 0))
         throw std::logic_error("fresh path-lane pass without a sample count, or on a batch");
     if(!paths) return;
-    const size_t lds = trace_lds_bytes(s, f, block);
+    // wide_lds_entries stack entries and the pixel's running sum per lane
+    const size_t lds = size_t(block) * (wide_lds_entries * sizeof(int2) + 3 * sizeof(float));
     const unsigned grid = unsigned((paths + uint32_t(block) - 1) / uint32_t(block));
     const unsigned g = lanes.blocks > 0 ? std::min(grid, unsigned(lanes.blocks)) : grid;
     const int batch_lanes = std::max(1, std::min(64, lanes.shade_batch));
