@@ -80,6 +80,28 @@ __device__ inline bf16x8 gather8(const conv_src& s, int n, int y, int x, int ch)
     return load8(s.b + idx * size_t(s.cpb) + size_t(ch - s.cpa));
 }
 
+// The 32-channel input chunk kc of image img: its first channel at pixel 0,
+// its channel stride, whether it is read at half resolution.  Offsets within
+// an image fit 32 bits (the launcher checks it).
+struct chunk_src { const uint16_t *p; int cp, up; };
+__device__ inline chunk_src chunk_of(const conv_src& s, int img, int kc) {
+    const int ch = kc * 32;
+    if(ch < s.cpa) {
+        const size_t hh = size_t(s.a_up ? s.h >> 1 : s.h), ww = size_t(s.a_up ? s.w >> 1 : s.w);
+        return chunk_src{s.a + size_t(img) * hh * ww * size_t(s.cpa) + size_t(ch), s.cpa, s.a_up};
+    }
+    return chunk_src{s.b + size_t(img) * size_t(s.h) * size_t(s.w) * size_t(s.cpb) + size_t(ch - s.cpa), s.cpb, 0};
+}
+
+constexpr float LOG_CLAMP = 12.0f;  // log radiance above which the output is clamped
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+// Two floats to packed bf16, rounded to nearest even (one v_cvt_pk_bf16_f32;
+// equal to to_bf16 for every non-NaN value).
+__device__ inline uint32_t pack_bf16(float lo, float hi) {
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2{lo, hi}, bf16x2));
+}
+
 // Master channel of padded input channel pi (channels of a, then of b), or -1
 // for padding.
 __device__ inline int real_channel(int pi, int cin_a, int cpa, int cin_b) {
@@ -87,42 +109,70 @@ __device__ inline int real_channel(int pi, int cin_a, int cpa, int cin_b) {
     return pi - cpa < cin_b ? cin_a + (pi - cpa) : -1;
 }
 
-enum epilogue { EPI_BF16 = 0, EPI_BF16_RELU = 1, EPI_F32 = 2 };
+// EPI_RADIANCE (the output layer at inference): the denoised linear radiance
+// max(expm1(min(y + x, LOG_CLAMP)), 0) of channels 0..2 (y the layer's output,
+// x the network input, i.e. log1p of the noisy radiance) for the pixels
+// inside the unpadded frame, written as 3 floats per pixel.
+enum epilogue { EPI_BF16 = 0, EPI_BF16_RELU = 1, EPI_F32 = 2, EPI_RADIANCE = 3 };
 
-// One workgroup computes a 16 x 16 pixel tile of one image for MT
-// 16-channel output tiles; wk is [9][kchunks][cpo][32] bf16.  The tile's 18 x 18 input halo is
-// staged through LDS one 32-channel chunk at a time (double-buffered: the
-// next chunk's global loads are in flight while the current one is
-// multiplied), laid out [channel group of 8][halo pixel] so that the 16
-// pixels of an MFMA B fragment read 256 contiguous bytes.  Wave w owns tile
-// rows 4w..4w+3 (one 16-pixel N tile per row).
+// A workgroup computes 16 x 16 pixel tiles of the images for MT 16-channel
+// output tiles (blockIdx.y selects which); wk is [9][kchunks][cpo][32] bf16.
+// The grid is persistent: workgroup b takes tiles b, b + gridDim.x, ... and
+// walks the flattened sequence of (tile, 32-channel input chunk) items.  An
+// item's 18 x 18 input halo is staged through LDS (double-buffered: the next
+// item's global loads, of the same tile or of the next one, are in flight
+// while the current one is multiplied), laid out [channel group of 8][halo
+// pixel] so that the 16 pixels of an MFMA B fragment read 256 contiguous
+// bytes.  Wave w owns tile rows 4w..4w+3 (one 16-pixel N tile per row).  A
+// tile's outputs are written after its last chunk, while the next tile's
+// first chunk loads.  Bias is added to the channels below cout; EPI_F32
+// stores only the 4-channel groups that start below cout.
 constexpr int TILE = 16, HALO = TILE + 2, HALO_PIX = HALO * HALO;
 constexpr int STAGE_ITEMS = HALO_PIX * 4;                       // 16-byte items per chunk
 constexpr int STAGE_PER_THREAD = (STAGE_ITEMS + 255) / 256;
 
+// Resident workgroups per compute unit: three (the most the LDS allows, at
+// most 168 VGPRs and AGPRs together per lane) for up to two output tiles,
+// two for more (their accumulators need the registers).
+template<int MT>
+constexpr int conv_waves = MT <= 2 ? 3 : 2;
+
 template<int MT, int EPI>
-__global__ __launch_bounds__(256) void conv_tile_kernel(conv_src src, const uint16_t *__restrict__ wk, int kchunks,
-                                                        int cpo, const float *__restrict__ bias, int cout,
-                                                        void *__restrict__ out, int tiles_x, int tiles_y) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(conv_waves<MT>))) void conv_tile_kernel(
+        conv_src src, const uint16_t *__restrict__ wk, int kchunks, int cpo, const float *__restrict__ bias, int cout,
+        void *__restrict__ out, int tiles_x, int tiles_y, const uint16_t *__restrict__ xin, int frame_h, int frame_w,
+        uint16_t *__restrict__ pooled) {
     __shared__ u16x8 halo[2][4][HALO_PIX];
     const int tid = int(threadIdx.x), lane = tid & 63, wave = tid >> 6;
     const int kq = lane >> 4, col = lane & 15;
-    int tile = int(blockIdx.x);
-    const int tx = tile % tiles_x;
-    tile /= tiles_x;
-    const int ty = tile % tiles_y;
-    const int img = tile / tiles_y;
-    const int x0 = tx * TILE, y0 = ty * TILE;
+    const int tiles = src.n * tiles_x * tiles_y;
+    const int stride = int(gridDim.x);
     const int co0 = int(blockIdx.y) * MT * 16;
+    struct place { int img, x0, y0; };
+    auto where = [&](int t) {
+        place w;
+        w.img = t / (tiles_x * tiles_y);
+        const int r = t - w.img * tiles_x * tiles_y;
+        w.y0 = (r / tiles_x) * TILE;
+        w.x0 = (r - (r / tiles_x) * tiles_x) * TILE;
+        return w;
+    };
 
     u16x8 staged[STAGE_PER_THREAD];
-    auto fetch = [&](int kc) {
+    auto fetch = [&](const place& w, int kc) {
+        const chunk_src c = chunk_of(src, w.img, kc);
+        const int row = c.up ? src.w >> 1 : src.w;
+        // a tile whose halo lies inside the image needs no bounds tests
+        const bool inside = w.x0 > 0 && w.y0 > 0 && w.x0 + TILE < src.w && w.y0 + TILE < src.h;
         for(int i = 0; i < STAGE_PER_THREAD; i++) {
             const int item = tid + 256 * i;
             if(item < STAGE_ITEMS) {
                 const int hp = item >> 2, q = item & 3;
                 const int hy = hp / HALO, hx = hp - hy * HALO;
-                staged[i] = __builtin_bit_cast(u16x8, gather8(src, img, y0 - 1 + hy, x0 - 1 + hx, kc * 32 + 8 * q));
+                const int y = w.y0 - 1 + hy, x = w.x0 - 1 + hx;
+                const bool ok = inside || (y >= 0 && y < src.h && x >= 0 && x < src.w);
+                const int pix = c.up ? (y >> 1) * row + (x >> 1) : y * row + x;
+                staged[i] = ok ? *reinterpret_cast<const u16x8 *>(c.p + pix * c.cp + 8 * q) : u16x8{0, 0, 0, 0, 0, 0, 0, 0};
             }
         }
     };
@@ -136,13 +186,27 @@ __global__ __launch_bounds__(256) void conv_tile_kernel(conv_src src, const uint
     f32x4 acc[MT][4];
     for(int mt = 0; mt < MT; mt++)
         for(int nt = 0; nt < 4; nt++) acc[mt][nt] = f32x4{0, 0, 0, 0};
+    // this lane's four output channels per output tile, and their biases
+    float bv[MT][4];
+    for(int mt = 0; mt < MT; mt++)
+        for(int j = 0; j < 4; j++) {
+            const int co = co0 + mt * 16 + 4 * kq + j;
+            bv[mt][j] = (bias && co < cout) ? bias[co] : 0.0f;
+        }
 
-    fetch(0);
+    int tile = int(blockIdx.x), kc = 0;
+    if(tile >= tiles) return;  // whole workgroup
+    place cur = where(tile);
+    fetch(cur, 0);
     store(0);
     __syncthreads();
-    for(int kc = 0; kc < kchunks; kc++) {
-        const int buf = kc & 1;
-        if(kc + 1 < kchunks) fetch(kc + 1);
+    for(int buf = 0;; buf ^= 1) {
+        // the item after this one: the next chunk, or the next tile's first
+        int next_tile = tile, next_kc = kc + 1;
+        if(next_kc == kchunks) { next_kc = 0; next_tile = tile + stride; }
+        const bool more = next_tile < tiles;
+        const place nxt = more ? where(next_tile) : cur;
+        if(more) fetch(nxt, next_kc);
         const uint16_t *wchunk = wk + (size_t(kc) * size_t(cpo) + size_t(co0 + col)) * 32 + size_t(8 * kq);
 #pragma unroll
         for(int t = 0; t < 9; t++) {
@@ -157,32 +221,85 @@ __global__ __launch_bounds__(256) void conv_tile_kernel(conv_src src, const uint
                 for(int nt = 0; nt < 4; nt++)
                     acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afrag[mt], bfrag[nt], acc[mt][nt], 0, 0, 0);
         }
-        if(kc + 1 < kchunks) store(buf ^ 1);
-        __syncthreads();
-    }
-
-    const int x = x0 + col;
-    for(int nt = 0; nt < 4; nt++) {
-        const int y = y0 + 4 * wave + nt;
-        if(x >= src.w || y >= src.h) continue;
-        const size_t p = (size_t(img) * size_t(src.h) + size_t(y)) * size_t(src.w) + size_t(x);
-        for(int mt = 0; mt < MT; mt++) {
-            const int co = co0 + mt * 16 + 4 * kq;
-            float v[4];
-            for(int j = 0; j < 4; j++) {
-                v[j] = acc[mt][nt][j];
-                if(bias) v[j] += (co + j < cout) ? bias[co + j] : 0.0f;
-                if(EPI == EPI_BF16_RELU) v[j] = fmaxf(v[j], 0.0f);
+        if(kc == kchunks - 1) {  // the tile is complete: bias, activation, store
+            const int x = cur.x0 + col;
+            uint32_t packed[MT][4][2];  // bf16 outputs (EPI_BF16*), kept for the pooled output
+            for(int nt = 0; nt < 4; nt++) {
+                const int y = cur.y0 + 4 * wave + nt;
+                const bool inside = x < src.w && y < src.h;
+                const size_t p = (size_t(cur.img) * size_t(src.h) + size_t(y)) * size_t(src.w) + size_t(x);
+                for(int mt = 0; mt < MT; mt++) {
+                    const int co = co0 + mt * 16 + 4 * kq;
+                    float v[4];
+                    for(int j = 0; j < 4; j++) {
+                        v[j] = acc[mt][nt][j];
+                        if(bias) v[j] += bv[mt][j];
+                        if(EPI == EPI_BF16_RELU) v[j] = fmaxf(v[j], 0.0f);
+                    }
+                    if(EPI == EPI_RADIANCE) {
+                        // channels 0..2, held by the first 4-channel group (predicted_log)
+                        if(co == 0 && x < frame_w && y < frame_h) {
+                            float *rad = static_cast<float *>(out) +
+                                         ((size_t(cur.img) * size_t(frame_h) + size_t(y)) * size_t(frame_w) + size_t(x)) * 3;
+                            for(int c = 0; c < 3; c++) {
+                                const float pred = v[c] + from_bf16(xin[p * 32 + size_t(c)]);
+                                rad[c] = fmaxf(expm1f(fminf(pred, LOG_CLAMP)), 0.0f);
+                            }
+                        }
+                    } else if(EPI == EPI_F32) {
+                        if(co < cout && inside)
+                            *reinterpret_cast<float4 *>(static_cast<float *>(out) + p * size_t(cpo) + size_t(co)) =
+                                make_float4(v[0], v[1], v[2], v[3]);
+                    } else {
+                        packed[mt][nt][0] = pack_bf16(v[0], v[1]);
+                        packed[mt][nt][1] = pack_bf16(v[2], v[3]);
+                        if(inside)
+                            *reinterpret_cast<uint2 *>(static_cast<uint16_t *>(out) + p * size_t(cpo) + size_t(co)) =
+                                make_uint2(packed[mt][nt][0], packed[mt][nt][1]);
+                    }
+                }
             }
-            if(EPI == EPI_F32) {
-                *reinterpret_cast<float4 *>(static_cast<float *>(out) + p * size_t(cpo) + size_t(co)) =
-                    make_float4(v[0], v[1], v[2], v[3]);
-            } else {
-                const uint32_t lo = uint32_t(to_bf16(v[0])) | (uint32_t(to_bf16(v[1])) << 16);
-                const uint32_t hi = uint32_t(to_bf16(v[2])) | (uint32_t(to_bf16(v[3])) << 16);
-                *reinterpret_cast<uint2 *>(static_cast<uint16_t *>(out) + p * size_t(cpo) + size_t(co)) = make_uint2(lo, hi);
+            if constexpr(EPI == EPI_BF16 || EPI == EPI_BF16_RELU) {
+                // The 2x2 average pooling of the stored (bf16) values into the
+                // half-resolution tensor `pooled` (floor of the size): rows np,
+                // np + 1 of this lane and columns col, col ^ 1 (the neighbouring
+                // lane), summed left to right then top to bottom, times 0.25,
+                // rounded to bf16.
+                if(pooled) {
+                    const int ho = src.h >> 1, wo = src.w >> 1;
+                    const int X = x >> 1;
+                    for(int np = 0; np < 4; np += 2) {
+                        const int Y = (cur.y0 + 4 * wave + np) >> 1;
+                        for(int mt = 0; mt < MT; mt++) {
+                            uint32_t res[2];
+                            for(int h2 = 0; h2 < 2; h2++) {
+                                const uint32_t t0 = packed[mt][np][h2], t1 = packed[mt][np + 1][h2];
+                                const uint32_t u0 = uint32_t(__shfl_xor(int(t0), 1)), u1 = uint32_t(__shfl_xor(int(t1), 1));
+                                float avg[2];
+                                for(int e = 0; e < 2; e++) {
+                                    const int sh = 16 * e;
+                                    const float a00 = from_bf16(uint16_t(t0 >> sh)), a01 = from_bf16(uint16_t(u0 >> sh));
+                                    const float a10 = from_bf16(uint16_t(t1 >> sh)), a11 = from_bf16(uint16_t(u1 >> sh));
+                                    avg[e] = 0.25f * (a00 + a01 + a10 + a11);
+                                }
+                                res[h2] = pack_bf16(avg[0], avg[1]);
+                            }
+                            if(!(col & 1) && X < wo && Y < ho) {
+                                const size_t q = (size_t(cur.img) * size_t(ho) + size_t(Y)) * size_t(wo) + size_t(X);
+                                *reinterpret_cast<uint2 *>(pooled + q * size_t(cpo) + size_t(co0 + mt * 16 + 4 * kq)) =
+                                    make_uint2(res[0], res[1]);
+                            }
+                        }
+                    }
+                }
             }
+            for(int mt = 0; mt < MT; mt++)
+                for(int nt = 0; nt < 4; nt++) acc[mt][nt] = f32x4{0, 0, 0, 0};
         }
+        if(!more) break;
+        store(buf ^ 1);
+        __syncthreads();
+        tile = next_tile; kc = next_kc; cur = nxt;
     }
 }
 
@@ -332,22 +449,6 @@ __global__ __launch_bounds__(256) void bgrad_reduce_kernel(const float *partial,
 }
 
 // ---------------------------------------------------------------- elementwise stages
-__global__ void pool_kernel(const uint16_t *in, uint16_t *out, int n, int h, int w, int cp) {
-    // 2x2 average of an (n, h, w) tensor into (n, h/2, w/2).
-    const int ho = h >> 1, wo = w >> 1;
-    const int64_t count = int64_t(n) * ho * wo * cp;
-    for(int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += int64_t(gridDim.x) * blockDim.x) {
-        const int c = int(i % cp);
-        const int64_t q = i / cp;
-        const int x = int(q % wo), y = int((q / wo) % ho), b = int(q / (int64_t(wo) * ho));
-        const size_t r0 = (size_t(b) * size_t(h) + size_t(2 * y)) * size_t(w) + size_t(2 * x);
-        const size_t r1 = r0 + size_t(w);
-        float s = from_bf16(in[r0 * cp + c]) + from_bf16(in[(r0 + 1) * cp + c]) + from_bf16(in[r1 * cp + c]) +
-                  from_bf16(in[(r1 + 1) * cp + c]);
-        out[size_t(i)] = to_bf16(0.25f * s);
-    }
-}
-
 // g[p][c] += dx[p][c0 + c] for c < cp (dx has dxc channels per pixel).
 __global__ void route_same_kernel(float *g, const float *dx, int64_t pixels, int cp, int dxc, int c0) {
     const int64_t count = pixels * cp;
@@ -442,7 +543,6 @@ __global__ void gather_kernel(const float *feat, const float *target, const int3
 __device__ inline float predicted_log(const float *out, const uint16_t *x, int64_t p, int c) {
     return out[size_t(p) * 32 + size_t(c)] + from_bf16(x[size_t(p) * 32 + size_t(c)]);
 }
-constexpr float LOG_CLAMP = 12.0f;
 constexpr float RELMSE_EPS = 0.01f;
 
 // Loss = mean |y - log1p(t)| + mean (expm1(y) - t)^2 / (t^2 + 0.01) over
@@ -484,16 +584,6 @@ __global__ void sum_kernel(const float *v, int n, float *out) {
         float s = 0;
         for(int i = 0; i < n; i++) s += v[i];
         *out = s;
-    }
-}
-
-// Denoised linear radiance of a padded (n, hp, wp) output, cropped to (h, w).
-__global__ void output_kernel(const float *out, const uint16_t *x, int n, int h, int w, int hp, int wp, float *rad) {
-    const int64_t count = int64_t(n) * h * w;
-    for(int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += int64_t(gridDim.x) * blockDim.x) {
-        const int xx = int(i % w), yy = int((i / w) % h), b = int(i / (int64_t(w) * h));
-        const int64_t p = (int64_t(b) * hp + yy) * wp + xx;
-        for(int c = 0; c < 3; c++) rad[size_t(i) * 3 + size_t(c)] = fmaxf(expm1f(fminf(predicted_log(out, x, p, c), LOG_CLAMP)), 0.0f);
     }
 }
 
@@ -607,6 +697,7 @@ struct ptd_network {
     int64_t step = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int compute_units = 1;
     // shape-dependent buffers
     int n = 0, h = 0, w = 0;
     dbuf<uint16_t> tensors[T_COUNT];
@@ -630,6 +721,11 @@ struct ptd_network {
         tensor_cp[T_P1] = tensor_cp[t_y(1)]; tensor_cp[T_P2] = tensor_cp[t_y(2)];
         tensor_cp[T_P3] = tensor_cp[t_y(3)]; tensor_cp[T_P4] = tensor_cp[t_y(4)];
         HC(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        {
+            int device = 0;
+            HC(hipGetDevice(&device));
+            HC(hipDeviceGetAttribute(&compute_units, hipDeviceAttributeMultiprocessorCount, device));
+        }
         HC(hipEventCreate(&ev0));
         HC(hipEventCreate(&ev1));
         const int64_t count = offsets[NLAYERS];
@@ -746,41 +842,56 @@ struct ptd_network {
         return c;
     }
 
+    // The convolution's persistent grid: as many workgroups as stay resident
+    // (conv_waves per compute unit, shared by the output-channel groups of
+    // gridDim.y), fewer when there are fewer tiles, with the tiles dealt
+    // evenly.  An EPI_F32 output of at most 16 channels computes one
+    // 16-channel tile.
     template<int EPI>
     void launch_conv(const conv_src& src, const uint16_t *weights, int kchunks, int co_total, const float *bias, int cout,
-                     void *dst) {
+                     void *dst, int frame_h = 0, int frame_w = 0, uint16_t *pooled = nullptr) {
+        const uint16_t *xin = tensors[T_X].p;
         const int tiles = co_total / 16;
         const int tiles_x = (src.w + TILE - 1) / TILE, tiles_y = (src.h + TILE - 1) / TILE;
         const int64_t blocks = int64_t(src.n) * tiles_x * tiles_y;
-        if(blocks > INT32_MAX) throw std::invalid_argument("denoiser input too large");
-        const unsigned gx = unsigned(blocks);
-        if(tiles % 6 == 0)
-            hipLaunchKernelGGL((conv_tile_kernel<6, EPI>), dim3(gx, unsigned(tiles / 6)), dim3(256), 0, stream, src,
-                               weights, kchunks, co_total, bias, cout, dst, tiles_x, tiles_y);
+        if(blocks > INT32_MAX || int64_t(src.h) * src.w * std::max(src.cpa, src.cpb) >= INT32_MAX)
+            throw std::invalid_argument("denoiser input too large");
+        // workgroups per image-tile column: all resident at once, tiles dealt evenly
+        auto grid_x = [&](int mt) {
+            const int64_t resident = int64_t(compute_units) * (mt <= 2 ? conv_waves<2> : conv_waves<4>) /
+                                     std::max(1, tiles / mt);
+            const int64_t per_block = (blocks + std::max<int64_t>(1, resident) - 1) / std::max<int64_t>(1, resident);
+            return unsigned((blocks + per_block - 1) / per_block);
+        };
+        if((EPI == EPI_F32 || EPI == EPI_RADIANCE) && cout <= 16)
+            hipLaunchKernelGGL((conv_tile_kernel<1, EPI>), dim3(grid_x(1) , 1u), dim3(256), 0, stream, src, weights,
+                               kchunks, co_total, bias, cout, dst, tiles_x, tiles_y, xin, frame_h, frame_w, pooled);
+        else if(tiles % 6 == 0)
+            hipLaunchKernelGGL((conv_tile_kernel<6, EPI>), dim3(grid_x(6), unsigned(tiles / 6)), dim3(256), 0, stream,
+                               src, weights, kchunks, co_total, bias, cout, dst, tiles_x, tiles_y, xin, frame_h, frame_w, pooled);
         else if(tiles % 4 == 0)
-            hipLaunchKernelGGL((conv_tile_kernel<4, EPI>), dim3(gx, unsigned(tiles / 4)), dim3(256), 0, stream, src,
-                               weights, kchunks, co_total, bias, cout, dst, tiles_x, tiles_y);
+            hipLaunchKernelGGL((conv_tile_kernel<4, EPI>), dim3(grid_x(4), unsigned(tiles / 4)), dim3(256), 0, stream,
+                               src, weights, kchunks, co_total, bias, cout, dst, tiles_x, tiles_y, xin, frame_h, frame_w, pooled);
         else
-            hipLaunchKernelGGL((conv_tile_kernel<2, EPI>), dim3(gx, unsigned(tiles / 2)), dim3(256), 0, stream, src,
-                               weights, kchunks, co_total, bias, cout, dst, tiles_x, tiles_y);
+            hipLaunchKernelGGL((conv_tile_kernel<2, EPI>), dim3(grid_x(2), unsigned(tiles / 2)), dim3(256), 0, stream,
+                               src, weights, kchunks, co_total, bias, cout, dst, tiles_x, tiles_y, xin, frame_h, frame_w, pooled);
         HC(hipGetLastError());
     }
 
-    void forward() {
+    // The forward pass; with `radiance`, the output layer writes the denoised
+    // radiance of the frame_h x frame_w frame (inference) instead of `out`.
+    void forward(float *radiance = nullptr, int frame_h = 0, int frame_w = 0) {
         for(int l = 0; l < NLAYERS; l++) {
             const layer_spec& s = SPECS[l];
-            if(l >= 2 && l <= 5) {  // pool the previous encoder output first
-                const int t_in = t_y(l - 1), t_out = T_P1 + (l - 2);
-                const int lv = SPECS[l - 1].level;
-                const int64_t count = level_pixels(lv + 1) * tensor_cp[t_in];
-                hipLaunchKernelGGL(pool_kernel, dim3(grid_for(count)), dim3(256), 0, stream, tensors[t_in].p,
-                                   tensors[t_out].p, n, level_h(lv), level_w(lv), tensor_cp[t_in]);
-            }
             const conv_src src = source(l);
             const float *bias = params.p + offsets[l] + int64_t(s.cout) * 9 * (s.cin_a + s.cin_b);
             const int kchunks = (cpa[l] + cpb[l]) / 32;
-            if(l == NLAYERS - 1) launch_conv<EPI_F32>(src, wk[l].p, kchunks, cpo[l], bias, s.cout, out.p);
-            else launch_conv<EPI_BF16_RELU>(src, wk[l].p, kchunks, cpo[l], bias, s.cout, tensors[t_y(l)].p);
+            if(l == NLAYERS - 1 && radiance)
+                launch_conv<EPI_RADIANCE>(src, wk[l].p, kchunks, cpo[l], bias, s.cout, radiance, frame_h, frame_w);
+            else if(l == NLAYERS - 1) launch_conv<EPI_F32>(src, wk[l].p, kchunks, cpo[l], bias, s.cout, out.p);
+            else  // encoder layers 1..4 also write the 2x2 average that the next level reads
+                launch_conv<EPI_BF16_RELU>(src, wk[l].p, kchunks, cpo[l], bias, s.cout, tensors[t_y(l)].p, 0, 0,
+                                           l >= 1 && l <= 4 ? tensors[T_P1 + (l - 1)].p : nullptr);
         }
     }
 
@@ -838,7 +949,7 @@ struct ptd_network {
             conv_src g;
             g.a = dy.p; g.b = nullptr; g.cpa = cpo[l]; g.cpb = 0; g.a_up = 0;
             g.n = n; g.h = level_h(s.level); g.w = level_w(s.level);
-            launch_conv<EPI_F32>(g, wt[l].p, cpo[l] / 32, cpi, nullptr, 0, dx.p);
+            launch_conv<EPI_F32>(g, wt[l].p, cpo[l] / 32, cpi, nullptr, cpi, dx.p);  // every channel of dX stored
             const int64_t pixels = level_pixels(s.level);
             if(s.src_b >= 0) {  // concat: upsampled deeper output, then the skip tensor
                 const int ka = s.src_a - T_Y0, kb = s.src_b - T_Y0;
@@ -944,9 +1055,7 @@ pt_status ptd_forward(ptd_network *net, const float *features, int32_t n, int32_
         HC(hipMemcpyAsync(feat.p, features, size_t(pixels) * 9 * sizeof(float), hipMemcpyHostToDevice, net->stream));
         hipLaunchKernelGGL(features_kernel, dim3(grid_for(pixels)), dim3(256), 0, net->stream, feat.p, pixels,
                            net->tensors[T_X].p);
-        net->forward();
-        hipLaunchKernelGGL(output_kernel, dim3(grid_for(pixels)), dim3(256), 0, net->stream, net->out.p,
-                           net->tensors[T_X].p, n, h, w, h, w, net->radiance.p);
+        net->forward(net->radiance.p, h, w);
         HC(hipGetLastError());
         HC(hipMemcpyAsync(radiance, net->radiance.p, size_t(pixels) * 3 * sizeof(float), hipMemcpyDeviceToHost, net->stream));
         HC(hipStreamSynchronize(net->stream));
@@ -1038,9 +1147,7 @@ pt_status ptd_denoise_renderer(ptd_network *net, const pt_renderer *renderer, fl
         HC(hipEventRecord(net->ev0, net->stream));
         hipLaunchKernelGGL(frame_features_kernel, dim3(grid_for(int64_t(hp) * wp)), dim3(256), 0, net->stream, f.sum, f.aov,
                            float(1.0 / double(f.samples)), f.width, f.height, wp, hp, net->tensors[T_X].p);
-        net->forward();
-        hipLaunchKernelGGL(output_kernel, dim3(grid_for(pixels)), dim3(256), 0, net->stream, net->out.p,
-                           net->tensors[T_X].p, 1, f.height, f.width, hp, wp, net->radiance.p);
+        net->forward(net->radiance.p, f.height, f.width);
         HC(hipGetLastError());
         HC(hipEventRecord(net->ev1, net->stream));
         HC(hipMemcpyAsync(radiance, net->radiance.p, size_t(pixels) * 3 * sizeof(float), hipMemcpyDeviceToHost, net->stream));
