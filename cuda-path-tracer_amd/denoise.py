@@ -153,19 +153,31 @@ def relmse(x, ref, eps=0.01):
 
 
 # ---------------------------------------------------------------- training data
-def training_views(count, seed, size):
+ICOSPHERE_FREQUENCIES = (24, 40, 64, 128)
+
+
+def training_view_specs(count, seed):
     """Seeded variants of the small room and the displaced icosphere scenes:
-    (scene, camera) pairs at size x size pixels."""
+    ("cornell", scene seed) or ("icosphere", frequency, scene seed), alternating."""
     rng = np.random.default_rng(seed)
-    views = []
+    specs = []
     for i in range(count):
         if i % 2 == 0:
-            scene, cam = ptrace.cornell(int(rng.integers(1, 1 << 30)), size, size)
+            specs.append(("cornell", int(rng.integers(1, 1 << 30))))
         else:
-            freq = int(rng.choice([24, 40, 64]))
-            scene, cam = ptrace.icosphere(freq, int(rng.integers(1, 1 << 30)), size, size)
-        views.append((scene, cam))
-    return views
+            specs.append(("icosphere", int(rng.choice(ICOSPHERE_FREQUENCIES)), int(rng.integers(1, 1 << 30))))
+    return specs
+
+
+def build_view(spec, size):
+    if spec[0] == "cornell":
+        return ptrace.cornell(spec[1], size, size)
+    return ptrace.icosphere(spec[1], spec[2], size, size)
+
+
+def training_views(count, seed, size):
+    """(scene, camera) pairs at size x size pixels (see training_view_specs)."""
+    return [build_view(spec, size) for spec in training_view_specs(count, seed)]
 
 
 def render_features(scene, camera, width, height, seed, spp, depth=8):
@@ -177,38 +189,93 @@ def render_features(scene, camera, width, height, seed, spp, depth=8):
     return r.image(), albedo, normal
 
 
-def make_training_set(views=16, frame=512, crop=128, spp_in=(1, 4, 16), spp_target=1024, seed=11, log=print):
-    """Crops of rendered views: features (crops, crop, crop, 9) and targets
-    (crops, crop, crop, 3).  Each view gives one input per sample count in
-    spp_in (independent seeds), all against the same high-spp target."""
-    feats, targets = [], []
+def make_training_set(views=16, frame=512, crop=256, spp_in=(1, 4, 16), spp_target=1024, seed=11, group=64,
+                      log=print, first_view=0, view_count=None):
+    """Crops of rendered views: features (pairs, crop, crop, 9) and targets
+    (pairs, crop, crop, 3).  Each view is cut into (frame/crop)^2 crops; each
+    crop gives one pair per sample count in spp_in, all against the same
+    spp_target render of the view.
+
+    Targets: `group` views at a time in one batched renderer (many 512x512
+    frames fill the GPU where one does not), spp_target samples in calls of
+    at most 512.  Inputs: one renderer per view with first-hit feature
+    buffers, accumulated progressively through the sample counts of spp_in
+    (the 4-spp input holds the 1-spp one's sample), as a caller denoising a
+    progressive render would see them.  Only views [first_view, first_view +
+    view_count) of the seeded list of `views` are rendered."""
+    spp_in = tuple(sorted(spp_in))
+    per_side = frame // crop
+    per_view = per_side * per_side
+    if view_count is None:
+        view_count = views - first_view
+    n_pairs = view_count * per_view * len(spp_in)
+    feats = np.empty((n_pairs, crop, crop, 9), np.float32)
+    targets = np.empty((n_pairs, crop, crop, 3), np.float32)
+    specs = training_view_specs(views, seed)
     t0 = time.perf_counter()
-    for i, (scene, cam) in enumerate(training_views(views, seed, frame)):
-        ref, _, _ = render_features(scene, cam, frame, frame, 5000 + i, spp_target)
-        t = ref.reshape(frame, frame, 3)
-        for k, spp in enumerate(spp_in):
-            rad, alb, nrm = render_features(scene, cam, frame, frame, 1000 + 31 * i + k, spp)
-            f = np.concatenate([rad, alb, nrm], axis=1).reshape(frame, frame, 9)
-            for y in range(0, frame, crop):
-                for x in range(0, frame, crop):
-                    feats.append(f[y:y + crop, x:x + crop])
-                    targets.append(t[y:y + crop, x:x + crop])
-        log(f"view {i}: {time.perf_counter() - t0:.1f}s")
-    return np.stack(feats).astype(np.float32), np.stack(targets).astype(np.float32)
+    target_samples = 0
+    target_seconds = 0.0
+    k = 0
+    from concurrent.futures import ThreadPoolExecutor
+    workers = max(1, min(16, len(os.sched_getaffinity(0))))
+    end = first_view + view_count
+    for g in range(first_view, end, group):
+        ids = list(range(g, min(end, g + group)))
+        with ThreadPoolExecutor(workers) as ex:
+            built = list(ex.map(lambda i: build_view(specs[i], frame), ids))
+        batch = ptrace.Batch([sc for sc, _ in built],
+                             [(j, built[j][1], frame, frame, 5000 + ids[j]) for j in range(len(ids))],
+                             direct_light=True, max_depth=8)
+        tb = time.perf_counter()
+        done = 0
+        while done < spp_target:
+            step = min(512, spp_target - done)
+            batch.add_samples(step)
+            done += step
+        tref = [batch.image(j).reshape(frame, frame, 3) for j in range(len(ids))]
+        dt = time.perf_counter() - tb
+        del batch
+        target_samples += len(ids) * frame * frame * spp_target
+        target_seconds += dt
+        for j, i in enumerate(ids):
+            scene, cam = built[j]
+            r = ptrace.Renderer(scene, cam, frame, frame, 1000 + 31 * i, False, ptrace.DEBUG_SHADED, True, 8)
+            r.enable_aovs()
+            have = 0
+            for spp in spp_in:
+                r.add_samples(spp - have)
+                have = spp
+                albedo, normal = r.aovs()
+                f = np.concatenate([r.image(), albedo, normal], axis=1).reshape(frame, frame, 9)
+                for y in range(0, per_side * crop, crop):
+                    for x in range(0, per_side * crop, crop):
+                        feats[k] = f[y:y + crop, x:x + crop]
+                        targets[k] = tref[j][y:y + crop, x:x + crop]
+                        k += 1
+            del r
+        del built, tref
+        log(f"views {ids[-1] + 1 - first_view}/{view_count}: {time.perf_counter() - t0:.1f}s, targets "
+            f"{target_samples / max(target_seconds, 1e-9) / 1e9:.2f} G samples/s")
+    assert k == n_pairs
+    return feats, targets
 
 
-def train(net, feats, targets, steps=2000, batch=16, lr=3e-4, seed=3, chunk=100, log=print):
+def train(net, feats, targets, steps=2000, batch=16, lr=3e-4, seed=3, chunk=100, log=print, schedule=None):
+    """Adam steps of `batch` random crops; the learning rate decays
+    exponentially from lr to lr/10 over `schedule` = (first step, total steps)
+    of a schedule that may span several calls (default: this call's steps)."""
+    first, total = schedule if schedule else (0, steps)
     net.set_training_data(feats, targets)
-    rng = np.random.default_rng(seed)
+    rng = np.random.default_rng(seed + first)
     history, device_ms = [], 0.0
     for s in range(0, steps, chunk):
         n = min(chunk, steps - s)
         ids = rng.integers(0, len(feats), n * batch)
-        rate = lr * (0.1 ** (s / steps))  # exponential decay to lr/10
+        rate = lr * (0.1 ** ((first + s) / total))
         losses, ms = net.train(ids, batch, rate)
         device_ms += ms
         history += losses.tolist()
-        log(f"steps {s + n}: loss {np.mean(losses):.4f}, {ms / n:.2f} ms/step")
+        log(f"steps {first + s + n}/{total}: loss {np.mean(losses):.4f}, {ms / n:.2f} ms/step")
     return np.array(history), device_ms
 
 
@@ -274,12 +341,16 @@ def main():
     sub = ap.add_subparsers(dest="cmd", required=True)
     t = sub.add_parser("train")
     t.add_argument("--out", default=WEIGHTS_PATH)
-    t.add_argument("--views", type=int, default=48)
+    t.add_argument("--views", type=int, default=1024)
     t.add_argument("--frame", type=int, default=512)
     t.add_argument("--crop", type=int, default=256)
     t.add_argument("--spp-in", type=int, nargs="+", default=[1, 4, 16])
-    t.add_argument("--spp-target", type=int, default=2048)
-    t.add_argument("--steps", type=int, default=3000)
+    t.add_argument("--spp-target", type=int, default=4096)
+    t.add_argument("--steps", type=int, default=6000)
+    t.add_argument("--group", type=int, default=64, help="views per batched target render")
+    t.add_argument("--parts", type=int, default=1, help="split views and steps over this many runs")
+    t.add_argument("--part", type=int, default=0)
+    t.add_argument("--init", default=None, help="weights to start from (the previous part's)")
     t.add_argument("--batch", type=int, default=16)
     t.add_argument("--lr", type=float, default=3e-4)
     pr = sub.add_parser("profile", help="20 training steps of batch 16 on synthetic 256x256 crops (for rocprofv3)")
@@ -328,10 +399,26 @@ def main():
                 f"(median of {args.repeat})")
         return 0
     if args.cmd == "train":
-        feats, targets = make_training_set(args.views, args.frame, args.crop, tuple(args.spp_in), args.spp_target, log=log)
-        log(f"training set: {len(feats)} crops of {args.crop}x{args.crop}")
-        net = Denoiser(seed=1)
-        history, ms = train(net, feats, targets, args.steps, args.batch, args.lr, log=log)
+        # Part p of P renders views [p*V/P, (p+1)*V/P) of the same seeded list
+        # and runs steps [p*S/P, (p+1)*S/P) of one learning-rate schedule,
+        # starting from the previous part's weights (--init).
+        if not 0 <= args.part < args.parts or args.views % args.parts or args.steps % args.parts:
+            raise SystemExit("--part must lie in [0, --parts), and --parts must divide --views and --steps")
+        v0, nv = args.part * args.views // args.parts, args.views // args.parts
+        s0, ns = args.part * args.steps // args.parts, args.steps // args.parts
+        feats, targets = make_training_set(args.views, args.frame, args.crop, tuple(args.spp_in), args.spp_target,
+                                           group=args.group, log=log, first_view=v0, view_count=nv)
+        log(f"training set part {args.part + 1}/{args.parts}: views {v0}..{v0 + nv - 1}, {len(feats)} pairs of "
+            f"{args.crop}x{args.crop} ({len(feats) // len(args.spp_in)} targets at {args.spp_target} spp)")
+        net = Denoiser(seed=1, weights=args.init)
+        history, ms = train(net, feats, targets, ns, args.batch, args.lr, log=log, schedule=(s0, args.steps))
+        del feats, targets
+        if args.init:
+            prior = np.load(args.init)
+            if "loss" in prior.files:
+                history = np.concatenate([prior["loss"], history])
+            if "device_ms" in prior.files:
+                ms += float(prior["device_ms"])
         net.save(args.out, loss=history, device_ms=ms, config=json.dumps(vars(args)))
         log(f"saved {args.out}: final loss {history[-50:].mean():.4f}, {ms / 1e3:.1f}s on the device")
     else:

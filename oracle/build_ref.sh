@@ -8,6 +8,9 @@
 #                        (cuda-path-tracer_amd/binding/cuda_renderer_hip.cpp)
 #                        and libptrace.so, with the reference's test assets
 #                        read from tests/golden/assets.
+#   preview_session_check  this repository's tests/native/preview_session_check.cpp:
+#                        the reference's preview_session<cuda_renderer>
+#                        (src/display/preview_session.h) on the same binding.
 # Flags mirror the reference Release host build: -O3, no -march, no FMA
 # contraction (CMakeLists.txt:19-20 builds device code with --fmad=false).
 set -euo pipefail
@@ -37,5 +40,7 @@ if [ -f "$PKG/libptrace.so" ]; then
         -o "$HERE/_ref/cuda_tests"
     g++ "${COMMON[@]}" -DRAY_TRACER_ASSET_DIR="\"$ASSETS\"" -DRAY_TRACER_HAS_CUDA=1 "$REF/tests/lighting_tests.cpp" \
         "${LINK[@]}" -o "$HERE/_ref/cuda_lighting_tests"
-    echo "built $HERE/_ref/cuda_tests $HERE/_ref/cuda_lighting_tests"
+    g++ "${COMMON[@]}" -DRAY_TRACER_ASSET_DIR="\"$ASSETS\"" "$ROOT/tests/native/preview_session_check.cpp" \
+        "${LINK[@]}" -o "$HERE/_ref/preview_session_check"
+    echo "built $HERE/_ref/cuda_tests $HERE/_ref/cuda_lighting_tests $HERE/_ref/preview_session_check"
 fi

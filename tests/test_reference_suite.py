@@ -8,6 +8,13 @@ own bars apply: device images within 5% relative L2 of the portable CPU
 renderer, MIS within 1e-4 with identical ray counts, RGBA within +-1, exact
 tie and ownership checks.  Exit code 77 would mean "no device", which is a
 failure here.
+
+preview_session_check (tests/native/preview_session_check.cpp, built by the
+same script) instantiates the reference viewer's preview_session<cuda_renderer>
+(src/display/preview_session.h) on the binding: after idle frames, camera
+moves, focus loss, resizes and every debug mode, the session's RGBA equals a
+fresh renderer's byte for byte and the reference host renderer's within 1,
+and it reports rays and kernel time per frame.
 """
 import os
 import subprocess
@@ -20,7 +27,7 @@ REF = os.path.join(ROOT, "oracle", "_ref")
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("binary", ["cuda_tests", "cuda_lighting_tests"])
+@pytest.mark.parametrize("binary", ["cuda_tests", "cuda_lighting_tests", "preview_session_check"])
 def test_reference_device_suite_passes(binary):
     path = os.path.join(REF, binary)
     if not os.path.exists(path):
