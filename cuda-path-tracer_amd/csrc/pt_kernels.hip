@@ -224,10 +224,33 @@ __device__ __forceinline__ float mis_weight(float a, float b) {
 }
 // cosf/sinf of angle = 2*float(pi)*(k*2^-24) (light_sampling.h:20,47) as
 // the host libm computes them: double evaluation rounded to float, replaced
-// by the stored host value for the k where the two differ.
+// by the stored host value for the k where the two differ.  The double
+// evaluation: angle = j/128 + l with |l| <= 1/256 (exact in double), cos and
+// sin of j/128 from trig_base (805 angles cover [0, 2*pi)), cos l and sin l
+// by their Taylor polynomials to l^8 and l^9 (truncation below 2^-100), and
+// the angle-addition formulas.
+__device__ double2 trig_base[805];
+__global__ void trig_base_kernel() {
+    const int j = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if(j < 805) trig_base[j] = make_double2(cos(double(j) * 0x1p-7), sin(double(j) * 0x1p-7));
+}
 __device__ __noinline__ float2 device_cos_sin(uint32_t k) {
     const float angle = (2 * PI_F) * (float(k) * 0x1p-24f);
-    return make_float2(float(cos(double(angle))), float(sin(double(angle))));
+    const double a = double(angle);
+    const int j = int(a * 128.0 + 0.5);
+    const double l = a - double(j) * 0x1p-7;
+    const double l2 = l * l;
+    double c = fma(l2, 1.0 / 40320, -1.0 / 720);
+    c = fma(c, l2, 1.0 / 24);
+    c = fma(c, l2, -0.5);
+    c = fma(c, l2, 1.0);
+    double sn = fma(l2, 1.0 / 362880, -1.0 / 5040);
+    sn = fma(sn, l2, 1.0 / 120);
+    sn = fma(sn, l2, -1.0 / 6);
+    sn = fma(sn, l2, 1.0);
+    sn = sn * l;
+    const double2 b = trig_base[j];
+    return make_float2(float(fma(b.x, c, -(b.y * sn))), float(fma(b.y, c, b.x * sn)));
 }
 __device__ __forceinline__ float2 libm_cos_sin(const scene_dev& s, uint32_t k) {
     const unsigned long long word = s.trig_bits[k >> 6];
@@ -1398,6 +1421,7 @@ __global__ void trig_values_kernel(float2 *out) {
 
 // ---------------- launchers ----------------
 void launch_trig_values(float2 *out, hipStream_t stream) {
+    hipLaunchKernelGGL(trig_base_kernel, dim3(4), dim3(256), 0, stream);
     hipLaunchKernelGGL(trig_values_kernel, dim3((1u << 24) / 256), dim3(256), 0, stream, out);
 }
 void launch_query(const scene_dev& s, const pt_ray *rays, pt_trace_result *out, int32_t *boxes, int32_t *prims,
