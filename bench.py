@@ -26,10 +26,12 @@ Also reported:
                   Measured figures of the same kernel from rocprofv3 counters
                   (committed summary under profiles/): HBM bytes (traffic,
                   FETCH_SIZE + WRITE_SIZE scaled by a plain-copy calibration),
-                  the measured HBM fraction, the L2 hit rate and active lanes
-                  per VALU instruction.  The reference-order figure (the
-                  reference's own trace_stats: branch nodes and leaves popped
-                  by its binary traversal) is reported under reference_order.
+                  the measured HBM fraction, the L2 hit rate, active lanes
+                  per VALU instruction and the share of SIMD cycles issuing
+                  VALU instructions (valu_busy: the kernel's actual bound).
+                  The reference-order figure (the reference's own
+                  trace_stats: branch nodes and leaves popped by its binary
+                  traversal) is reported under reference_order.
   cpu_baseline -- the CPU oracle (a C restatement of the reference's host
                   transport path) on every core available to the process over
                   a bounded sample of the same frame; reference_host: the
@@ -340,7 +342,8 @@ def main():
                 "leaf_records_per_query": round((wide_counts["triangle_tests"] + wide_counts["sphere_tests"]) / queries, 2),
                 "retraced_queries_per_frame": retraced // max(1, args.steps),
                 "measured_hbm_frac": prof.get("measured_hbm_frac"), "l2_hit_rate": prof.get("l2_hit_rate"),
-                "valu_active_lanes": prof.get("valu_active_lanes"), "profile": prof.get("summary")}
+                "valu_active_lanes": prof.get("valu_active_lanes"), "valu_busy": prof.get("valu_busy"),
+                "profile": prof.get("summary")}
     ref_bytes = (32 * ref_counts["box_tests"] + 36 * ref_counts["triangle_tests"] + 16 * ref_counts["sphere_tests"]
                  + 48 * traced_ext + 36 * traced_sh)
     roofline["reference_order"] = {
