@@ -856,24 +856,24 @@ struct ptd_network {
         const int64_t blocks = int64_t(src.n) * tiles_x * tiles_y;
         if(blocks > INT32_MAX || int64_t(src.h) * src.w * std::max(src.cpa, src.cpb) >= INT32_MAX)
             throw std::invalid_argument("denoiser input too large");
-        // workgroups per image-tile column: all resident at once, tiles dealt evenly
-        auto grid_x = [&](int mt) {
+        // workgroups per image-tile column (gy columns): all resident at once, tiles dealt evenly
+        auto grid_x = [&](int mt, int gy) {
             const int64_t resident = int64_t(compute_units) * (mt <= 2 ? conv_waves<2> : conv_waves<4>) /
-                                     std::max(1, tiles / mt);
+                                     std::max(1, gy);
             const int64_t per_block = (blocks + std::max<int64_t>(1, resident) - 1) / std::max<int64_t>(1, resident);
             return unsigned((blocks + per_block - 1) / per_block);
         };
         if((EPI == EPI_F32 || EPI == EPI_RADIANCE) && cout <= 16)
-            hipLaunchKernelGGL((conv_tile_kernel<1, EPI>), dim3(grid_x(1) , 1u), dim3(256), 0, stream, src, weights,
+            hipLaunchKernelGGL((conv_tile_kernel<1, EPI>), dim3(grid_x(1, 1), 1u), dim3(256), 0, stream, src, weights,
                                kchunks, co_total, bias, cout, dst, tiles_x, tiles_y, xin, frame_h, frame_w, pooled);
         else if(tiles % 6 == 0)
-            hipLaunchKernelGGL((conv_tile_kernel<6, EPI>), dim3(grid_x(6), unsigned(tiles / 6)), dim3(256), 0, stream,
+            hipLaunchKernelGGL((conv_tile_kernel<6, EPI>), dim3(grid_x(6, tiles / 6), unsigned(tiles / 6)), dim3(256), 0, stream,
                                src, weights, kchunks, co_total, bias, cout, dst, tiles_x, tiles_y, xin, frame_h, frame_w, pooled);
         else if(tiles % 4 == 0)
-            hipLaunchKernelGGL((conv_tile_kernel<4, EPI>), dim3(grid_x(4), unsigned(tiles / 4)), dim3(256), 0, stream,
+            hipLaunchKernelGGL((conv_tile_kernel<4, EPI>), dim3(grid_x(4, tiles / 4), unsigned(tiles / 4)), dim3(256), 0, stream,
                                src, weights, kchunks, co_total, bias, cout, dst, tiles_x, tiles_y, xin, frame_h, frame_w, pooled);
         else
-            hipLaunchKernelGGL((conv_tile_kernel<2, EPI>), dim3(grid_x(2), unsigned(tiles / 2)), dim3(256), 0, stream,
+            hipLaunchKernelGGL((conv_tile_kernel<2, EPI>), dim3(grid_x(2, tiles / 2), unsigned(tiles / 2)), dim3(256), 0, stream,
                                src, weights, kchunks, co_total, bias, cout, dst, tiles_x, tiles_y, xin, frame_h, frame_w, pooled);
         HC(hipGetLastError());
     }
