@@ -31,12 +31,17 @@ Also reported:
                   VALU instructions (valu_busy: the kernel's actual bound).
                   The reference-order figure (the reference's own
                   trace_stats: branch nodes and leaves popped by its binary
-                  traversal) is reported under reference_order.
+                  traversal) is reported under reference_order, and the
+                  traversal's FP32 test arithmetic (12 FLOP per box test, 48
+                  per triangle, 30 per sphere) against the 157.3 TFLOP/s
+                  vector peak under fp32.
   cpu_baseline -- the CPU oracle (a C restatement of the reference's host
                   transport path) on every core available to the process over
                   a bounded sample of the same frame; reference_host: the
                   reference's own host path compiled from its sources
-                  (oracle/_ref), labelled, at its fixed path limit of 50.
+                  (oracle/_ref), labelled, at its fixed path limit of 50;
+                  single_thread: the oracle on one thread, 1 spp over the
+                  frame.
   denoised     -- time to relMSE <= 0.01 against a 16384-spp render of the
                   same 1080p view, for plain progressive accumulation and for a
                   low-spp render denoised by the shipped network.
@@ -92,6 +97,19 @@ def cpu_baseline(scene, camera, threads, spp):
     return {"value": rays / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port", "cpu": cpu_model(),
             "sample": f"{spp} spp (the timed GPU frame has {SPP}) over the full {WIDTH}x{HEIGHT} frame of the same "
                       f"scene, camera, seed and depth ({rays} rays, {dt:.1f} s, {threads} threads)"}
+
+
+def single_thread_baseline(scene, camera):
+    """The oracle on one thread over one sample of the full frame (about
+    5.4M rays): the per-core rate beside the all-core figure."""
+    import oracle
+    r = oracle.Renderer(scene, camera, WIDTH, HEIGHT, SEED, False, 0, True, DEPTH)
+    t0 = time.perf_counter()
+    r.add_samples(1, threads=1)
+    dt = time.perf_counter() - t0
+    rays = r.rays_traced()
+    return {"value": rays / dt / 1e6, "unit": "Mrays/s", "cores": 1, "kind": "port",
+            "sample": f"1 spp over the full {WIDTH}x{HEIGHT} frame ({rays} rays, {dt:.1f} s, 1 thread)"}
 
 
 def reference_host_baseline(scene, camera, threads, rows=216):
@@ -344,6 +362,13 @@ def main():
                 "measured_hbm_frac": prof.get("measured_hbm_frac"), "l2_hit_rate": prof.get("l2_hit_rate"),
                 "valu_active_lanes": prof.get("valu_active_lanes"), "valu_busy": prof.get("valu_busy"),
                 "profile": prof.get("summary")}
+    # Secondary figure: the traversal's FP32 arithmetic against the vector
+    # FP32 peak (157.3 TFLOP/s): 12 FLOP per box test (4 per child box of a
+    # wide node visited), 48 per triangle test and 30 per sphere test.
+    flop = (12 * 4 * wide_counts["box_tests"] + 48 * wide_counts["triangle_tests"]
+            + 30 * wide_counts["sphere_tests"])
+    roofline["fp32"] = {"flop_per_launch": int(flop), "achieved": round(flop / launch_s / 1e12, 2),
+                        "peak": 157.3, "unit": "TFLOP/s", "frac": round(flop / launch_s / 1e12 / 157.3, 4)}
     ref_bytes = (32 * ref_counts["box_tests"] + 36 * ref_counts["triangle_tests"] + 16 * ref_counts["sphere_tests"]
                  + 48 * traced_ext + 36 * traced_sh)
     roofline["reference_order"] = {
@@ -364,6 +389,9 @@ def main():
         log("reference host path:", ref_host)
         if cpu and ref_host:
             cpu["reference_host"] = ref_host
+        one = single_thread_baseline(scene, camera)
+        log("one thread:", one)
+        cpu["single_thread"] = one
 
     den = None
     if not args.no_denoise:

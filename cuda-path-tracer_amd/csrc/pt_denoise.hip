@@ -449,45 +449,82 @@ __global__ __launch_bounds__(256) void bgrad_reduce_kernel(const float *partial,
 }
 
 // ---------------------------------------------------------------- elementwise stages
+// Gradient routing and ReLU masks.  Every tensor has a multiple of 32
+// channels per pixel, so each thread moves one 4-channel group (16 bytes) or,
+// for the bf16 masks, 8 channels; consecutive threads take consecutive groups.
+// Per element the arithmetic is the plain one (g += dx, g += 0.25 * dx, the
+// 2x2 sum left to right then top to bottom).
+
+// i / d for a 64-bit i and a 32-bit d, in 32-bit arithmetic while i fits.
+__device__ inline uint64_t udiv(uint64_t i, uint32_t d) { return (i >> 32) ? i / d : uint64_t(uint32_t(i) / d); }
+
+__device__ inline float4 add4(float4 a, float4 b) { return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w); }
+
 // g[p][c] += dx[p][c0 + c] for c < cp (dx has dxc channels per pixel).
 __global__ void route_same_kernel(float *g, const float *dx, int64_t pixels, int cp, int dxc, int c0) {
-    const int64_t count = pixels * cp;
-    for(int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += int64_t(gridDim.x) * blockDim.x) {
-        const int64_t p = i / cp;
-        const int c = int(i - p * cp);
-        g[i] += dx[size_t(p) * size_t(dxc) + size_t(c0 + c)];
+    const uint32_t gp = uint32_t(cp) / 4u;
+    const uint64_t count = uint64_t(pixels) * gp;
+    float4 *g4 = reinterpret_cast<float4 *>(g);
+    for(uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += uint64_t(gridDim.x) * blockDim.x) {
+        const uint64_t p = udiv(i, gp);
+        const uint32_t c = uint32_t(i - p * gp) * 4u;
+        g4[i] = add4(g4[i], *reinterpret_cast<const float4 *>(dx + p * uint64_t(dxc) + uint64_t(c0) + c));
     }
 }
 // Gradient of a 2x nearest upsample: g[q][c] += sum of the 2x2 block of dx
 // (full resolution (n, h, w), dxc channels, channels [0, cp)).
 __global__ void route_up_kernel(float *g, const float *dx, int n, int h, int w, int cp, int dxc) {
-    const int ho = h >> 1, wo = w >> 1;
-    const int64_t count = int64_t(n) * ho * wo * cp;
-    for(int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += int64_t(gridDim.x) * blockDim.x) {
-        const int c = int(i % cp);
-        const int64_t q = i / cp;
-        const int x = int(q % wo), y = int((q / wo) % ho), b = int(q / (int64_t(wo) * ho));
-        const size_t r0 = (size_t(b) * size_t(h) + size_t(2 * y)) * size_t(w) + size_t(2 * x);
-        const size_t r1 = r0 + size_t(w);
-        g[i] += dx[r0 * dxc + c] + dx[(r0 + 1) * dxc + c] + dx[r1 * dxc + c] + dx[(r1 + 1) * dxc + c];
+    const uint32_t ho = uint32_t(h) >> 1, wo = uint32_t(w) >> 1, gp = uint32_t(cp) / 4u;
+    const uint64_t count = uint64_t(n) * ho * wo * gp;
+    float4 *g4 = reinterpret_cast<float4 *>(g);
+    for(uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += uint64_t(gridDim.x) * blockDim.x) {
+        const uint64_t q = udiv(i, gp);
+        const uint32_t c = uint32_t(i - q * gp) * 4u;
+        const uint64_t row = udiv(q, wo);
+        const uint32_t x = uint32_t(q - row * wo);
+        const uint64_t b = udiv(row, ho);
+        const uint32_t y = uint32_t(row - b * ho);
+        const uint64_t r0 = (b * uint64_t(h) + 2u * y) * uint64_t(w) + 2u * x, r1 = r0 + uint64_t(w);
+        const float4 d00 = *reinterpret_cast<const float4 *>(dx + r0 * uint64_t(dxc) + c);
+        const float4 d01 = *reinterpret_cast<const float4 *>(dx + (r0 + 1) * uint64_t(dxc) + c);
+        const float4 d10 = *reinterpret_cast<const float4 *>(dx + r1 * uint64_t(dxc) + c);
+        const float4 d11 = *reinterpret_cast<const float4 *>(dx + (r1 + 1) * uint64_t(dxc) + c);
+        g4[i] = add4(g4[i], add4(add4(add4(d00, d01), d10), d11));
     }
 }
-// Gradient of a 2x2 average pool: g[p][c] += dx[pool(p)][c] / 4 (g at full resolution).
+// Gradient of a 2x2 average pool: g[p][c] += dx[pool(p)][c] / 4 (g at full
+// resolution (n, h, w), dx at half, both cp channels); one thread per pooled
+// group, updating its four full-resolution pixels.
 __global__ void route_pool_kernel(float *g, const float *dx, int n, int h, int w, int cp) {
-    const int64_t count = int64_t(n) * h * w * cp;
-    const int ho = h >> 1, wo = w >> 1;
-    for(int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += int64_t(gridDim.x) * blockDim.x) {
-        const int c = int(i % cp);
-        const int64_t p = i / cp;
-        const int x = int(p % w), y = int((p / w) % h), b = int(p / (int64_t(w) * h));
-        const size_t q = (size_t(b) * size_t(ho) + size_t(y >> 1)) * size_t(wo) + size_t(x >> 1);
-        g[i] += 0.25f * dx[q * cp + c];
+    const uint32_t ho = uint32_t(h) >> 1, wo = uint32_t(w) >> 1, gp = uint32_t(cp) / 4u;
+    const uint64_t count = uint64_t(n) * ho * wo * gp;
+    float4 *g4 = reinterpret_cast<float4 *>(g);
+    for(uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += uint64_t(gridDim.x) * blockDim.x) {
+        const uint64_t q = udiv(i, gp);
+        const uint32_t k = uint32_t(i - q * gp);
+        const uint64_t row = udiv(q, wo);
+        const uint32_t x = uint32_t(q - row * wo);
+        const uint64_t b = udiv(row, ho);
+        const uint32_t y = uint32_t(row - b * ho);
+        const float4 d = reinterpret_cast<const float4 *>(dx)[i];
+        const float4 v = make_float4(0.25f * d.x, 0.25f * d.y, 0.25f * d.z, 0.25f * d.w);
+        const uint64_t p0 = (b * uint64_t(h) + 2u * y) * uint64_t(w) + 2u * x, p1 = p0 + uint64_t(w);
+        const uint64_t fine[4] = {p0, p0 + 1, p1, p1 + 1};
+        for(int j = 0; j < 4; j++) g4[fine[j] * gp + k] = add4(g4[fine[j] * gp + k], v);
     }
 }
-// dY (bf16) of the layer that produced y: g where y > 0 (ReLU), else 0.
+// dY (bf16) of the layer that produced y: g where y > 0 (ReLU), else 0;
+// count a multiple of 8.
 __global__ void relu_grad_kernel(uint16_t *dy, const float *g, const uint16_t *y, int64_t count) {
-    for(int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += int64_t(gridDim.x) * blockDim.x)
-        dy[i] = from_bf16(y[i]) > 0.0f ? to_bf16(g[i]) : uint16_t(0);
+    const uint64_t n8 = uint64_t(count) / 8u;
+    for(uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n8; i += uint64_t(gridDim.x) * blockDim.x) {
+        const u16x8 yv = reinterpret_cast<const u16x8 *>(y)[i];
+        const float4 g0 = reinterpret_cast<const float4 *>(g)[2 * i], g1 = reinterpret_cast<const float4 *>(g)[2 * i + 1];
+        const float gv[8] = {g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.z, g1.w};
+        u16x8 out;
+        for(int j = 0; j < 8; j++) out[j] = from_bf16(yv[j]) > 0.0f ? to_bf16(gv[j]) : uint16_t(0);
+        reinterpret_cast<u16x8 *>(dy)[i] = out;
+    }
 }
 
 // Network input: log1p(radiance), albedo, normal (9 channels of 32).
@@ -939,7 +976,7 @@ struct ptd_network {
             const layer_spec& s = SPECS[l];
             if(l < NLAYERS - 1) {  // dy of this layer from its output's gradient and ReLU
                 const int64_t count = level_pixels(s.level) * tensor_cp[t_y(l)];
-                hipLaunchKernelGGL(relu_grad_kernel, dim3(grid_for(count)), dim3(256), 0, stream, dy.p, grad_y[l].p,
+                hipLaunchKernelGGL(relu_grad_kernel, dim3(grid_for(count / 8)), dim3(256), 0, stream, dy.p, grad_y[l].p,
                                    tensors[t_y(l)].p, count);
             }
             weight_gradient(l);
@@ -953,21 +990,21 @@ struct ptd_network {
             const int64_t pixels = level_pixels(s.level);
             if(s.src_b >= 0) {  // concat: upsampled deeper output, then the skip tensor
                 const int ka = s.src_a - T_Y0, kb = s.src_b - T_Y0;
-                const int64_t ca = level_pixels(s.level + 1) * cpa[l];
+                const int64_t ca = level_pixels(s.level + 1) * cpa[l] / 4;
                 hipLaunchKernelGGL(route_up_kernel, dim3(grid_for(ca)), dim3(256), 0, stream, grad_y[ka].p, dx.p, n,
                                    level_h(s.level), level_w(s.level), cpa[l], cpi);
-                const int64_t cb = pixels * cpb[l];
+                const int64_t cb = pixels * cpb[l] / 4;
                 hipLaunchKernelGGL(route_same_kernel, dim3(grid_for(cb)), dim3(256), 0, stream, grad_y[kb].p, dx.p, pixels,
                                    cpb[l], cpi, cpa[l]);
             } else if(s.src_a >= T_P1) {  // pooled encoder output
                 const int k = l - 1;
                 const int lv = SPECS[k].level;
-                const int64_t count = level_pixels(lv) * tensor_cp[t_y(k)];
+                const int64_t count = level_pixels(lv + 1) * tensor_cp[t_y(k)] / 4;
                 hipLaunchKernelGGL(route_pool_kernel, dim3(grid_for(count)), dim3(256), 0, stream, grad_y[k].p, dx.p, n,
                                    level_h(lv), level_w(lv), tensor_cp[t_y(k)]);
             } else {  // plain: the previous layer's output
                 const int k = s.src_a - T_Y0;
-                const int64_t count = pixels * cpi;
+                const int64_t count = pixels * cpi / 4;
                 hipLaunchKernelGGL(route_same_kernel, dim3(grid_for(count)), dim3(256), 0, stream, grad_y[k].p, dx.p, pixels,
                                    cpi, cpi, 0);
             }
