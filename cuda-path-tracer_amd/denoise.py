@@ -296,7 +296,13 @@ def time_to_relmse(scene, camera, width, height, net, target=0.01, reference_spp
     reference = ref.image()
     del ref
     plain = []
+    # Each timed renderer first renders one untimed sample and is reset: its
+    # first add_samples call sizes the schedule's scratch buffers (set-up, like
+    # the denoiser's below), and the reset restores the seeded initial state,
+    # so the timed samples are those of a fresh renderer.
     r = ptrace.Renderer(scene, camera, width, height, seed, False, ptrace.DEBUG_SHADED, True, 8)
+    r.add_samples(1)
+    r.reset(camera)
     have, spent = 0, 0.0
     for spp in sorted(plain_spp):
         t0 = time.perf_counter()
@@ -320,6 +326,8 @@ def time_to_relmse(scene, camera, width, height, net, target=0.01, reference_spp
     for spp in sorted(denoised_spp):
         r = ptrace.Renderer(scene, camera, width, height, seed, False, ptrace.DEBUG_SHADED, True, 8)
         r.enable_aovs()
+        r.add_samples(1)
+        r.reset(camera)
         t0 = time.perf_counter()
         r.add_samples(spp)
         out, ms = net.denoise_renderer(r)
