@@ -309,12 +309,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(conv_waves<
 // 64-pixel sub-block of dY is staged once.  Both operands are transposed
 // through LDS so that K (pixels) is contiguous per lane.  NCO = cpo / 16
 // output-channel tiles; 9 x 2 x NCO MFMA tiles spread over the 4 waves.
+// The chunk-0 blocks also sum dY over the split's pixels into
+// bpartial[s][co] (the bias gradient's partials): each thread accumulates
+// the 8-channel groups it stages, in pixel order, and the threads' sums of a
+// channel are added in thread order at the end.
 constexpr int WG_PIX = 64;
 constexpr int LDS_ROW = WG_PIX + 8;  // bf16 per LDS row (16-byte aligned rows, fewer bank conflicts)
 
 template<int NCO>
 __global__ __launch_bounds__(256) void wgrad_kernel(conv_src src, const uint16_t *__restrict__ dy, int cpo,
-                                                    int64_t blocks_per_split, int cpi, float *__restrict__ partial) {
+                                                    int64_t blocks_per_split, int cpi, float *__restrict__ partial,
+                                                    float *__restrict__ bpartial) {
     __shared__ __attribute__((aligned(16))) uint16_t xt[9 * 32 * LDS_ROW];
     __shared__ __attribute__((aligned(16))) uint16_t dt[NCO * 16 * LDS_ROW];
     const int tid = int(threadIdx.x), lane = tid & 63, wave = tid >> 6;
@@ -331,6 +336,11 @@ __global__ __launch_bounds__(256) void wgrad_kernel(conv_src src, const uint16_t
     for(int i = 0; i < TPW; i++) acc[i] = f32x4{0, 0, 0, 0};
     const int groups = cpo / 8;
     const int pxl = tid >> 2, g = tid & 3;  // this thread's pixel and 8-channel group of the X tile
+    // bias sums of the dY groups this thread stages: item tid + 256 j has group (tid + 256 j) % groups
+    constexpr int BJ = (WG_PIX * 2 * NCO + 255) / 256;
+    float bsum[BJ][8];
+    for(int j = 0; j < BJ; j++)
+        for(int e = 0; e < 8; e++) bsum[j][e] = 0.0f;
     for(int64_t sb = sb0; sb < sb1; sb++) {
         const int64_t p0 = sb * WG_PIX;
         {   // X at the nine tap offsets, transposed into xt[t][ci][px].
@@ -349,12 +359,16 @@ __global__ __launch_bounds__(256) void wgrad_kernel(conv_src src, const uint16_t
                 for(int j = 0; j < 8; j++) xt[(t * 32 + 8 * g + j) * LDS_ROW + pxl] = u[j];
             }
         }
-        for(int i = tid; i < WG_PIX * groups; i += 256) {  // dY tile, transposed into dt[co][px]
+        for(int j = 0; j < BJ; j++) {  // dY tile, transposed into dt[co][px]
+            const int i = tid + 256 * j;
+            if(i >= WG_PIX * groups) break;
             const int q = i / groups, gg = i - q * groups;
             const int64_t p = p0 + q;
             u16x8 u = u16x8{0, 0, 0, 0, 0, 0, 0, 0};
             if(p < total) u = *reinterpret_cast<const u16x8 *>(dy + size_t(p) * size_t(cpo) + size_t(8 * gg));
-            for(int j = 0; j < 8; j++) dt[(8 * gg + j) * LDS_ROW + q] = u[j];
+            for(int e = 0; e < 8; e++) dt[(8 * gg + e) * LDS_ROW + q] = u[e];
+            if(kc == 0)
+                for(int e = 0; e < 8; e++) bsum[j][e] += from_bf16(u[e]);
         }
         __syncthreads();
         for(int i = 0; i < TPW; i++) {
@@ -380,6 +394,22 @@ __global__ __launch_bounds__(256) void wgrad_kernel(conv_src src, const uint16_t
         for(int v = 0; v < 4; v++) {
             const int ci = kc * 32 + ti * 16 + 4 * kq + v, co = tj * 16 + col;
             partial[((size_t(blockIdx.x) * 9 + size_t(t)) * size_t(cpi) + size_t(ci)) * size_t(cpo) + size_t(co)] = acc[i][v];
+        }
+    }
+    if(kc == 0) {
+        // the threads' bias sums through LDS (xt is free after the loop's last barrier)
+        float *red = reinterpret_cast<float *>(xt);
+        static_assert(BJ * 256 * 8 * sizeof(float) <= sizeof(xt), "bias sums fit in the X tile");
+        for(int j = 0; j < BJ; j++)
+            for(int e = 0; e < 8; e++) red[(j * 256 + tid) * 8 + e] = bsum[j][e];
+        __syncthreads();
+        if(tid < cpo) {
+            // item q * groups + gg (sub-block pixel q, group gg) was staged by
+            // thread (q * groups + gg) % 256, iteration (q * groups + gg) / 256
+            const int gg = tid >> 3, e = tid & 7;
+            float b = 0.0f;
+            for(int q = 0; q < WG_PIX; q++) b += red[(q * groups + gg) * 8 + e];
+            bpartial[size_t(blockIdx.x) * size_t(cpo) + size_t(tid)] = b;
         }
     }
 }
@@ -408,31 +438,6 @@ __global__ __launch_bounds__(256) void wgrad_reduce_kernel(const float *partial,
     grad[(size_t(co) * 9 + size_t(t)) * size_t(cin_a + cin_b) + size_t(ci)] = total;
 }
 
-// Bias gradient partials: partial[block][co] = sum over the block's pixels of
-// dY[p][co].  Each thread reads 16-byte groups of 8 channels; consecutive
-// threads cover consecutive groups of consecutive pixels.
-__global__ __launch_bounds__(256) void bgrad_kernel(const uint16_t *dy, int64_t total, int cpo, int64_t per_block, float *partial) {
-    __shared__ float red[256][9];
-    const int tid = int(threadIdx.x);
-    const int groups = cpo / 8, rows = 256 / groups;
-    const int g = tid % groups, r = tid / groups;
-    float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    if(r < rows) {
-        const int64_t p0 = int64_t(blockIdx.x) * per_block, p1 = std::min(total, p0 + per_block);
-        for(int64_t p = p0 + r; p < p1; p += rows) {
-            const u16x8 v = *reinterpret_cast<const u16x8 *>(dy + size_t(p) * size_t(cpo) + size_t(8 * g));
-            for(int j = 0; j < 8; j++) s[j] += from_bf16(v[j]);
-        }
-    }
-    for(int j = 0; j < 8; j++) red[tid][j] = s[j];
-    __syncthreads();
-    if(tid < cpo) {
-        const int gg = tid / 8, j = tid % 8;
-        float t = 0;
-        for(int k = 0; k < rows; k++) t += red[k * groups + gg][j];
-        partial[size_t(blockIdx.x) * size_t(cpo) + size_t(tid)] = t;
-    }
-}
 // One block per output channel: a fixed-order tree sum of the block partials.
 __global__ __launch_bounds__(256) void bgrad_reduce_kernel(const float *partial, int blocks, int cpo, int cout, float *grad) {
     __shared__ float red[256];
@@ -453,27 +458,33 @@ __global__ __launch_bounds__(256) void bgrad_reduce_kernel(const float *partial,
 // channels per pixel, so each thread moves one 4-channel group (16 bytes) or,
 // for the bf16 masks, 8 channels; consecutive threads take consecutive groups.
 // Per element the arithmetic is the plain one (g += dx, g += 0.25 * dx, the
-// 2x2 sum left to right then top to bottom).
+// 2x2 sum left to right then top to bottom).  With `first` set the kernel is
+// the gradient's first contribution and g is read as zero (0 + v, so that a
+// -0 contribution stores +0 as an add into a zeroed buffer would): the
+// buffers need no clearing.
 
 // i / d for a 64-bit i and a 32-bit d, in 32-bit arithmetic while i fits.
 __device__ inline uint64_t udiv(uint64_t i, uint32_t d) { return (i >> 32) ? i / d : uint64_t(uint32_t(i) / d); }
 
 __device__ inline float4 add4(float4 a, float4 b) { return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w); }
+__device__ inline float4 accumulate(const float4 *g, uint64_t i, bool first, float4 v) {
+    return add4(first ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : g[i], v);
+}
 
 // g[p][c] += dx[p][c0 + c] for c < cp (dx has dxc channels per pixel).
-__global__ void route_same_kernel(float *g, const float *dx, int64_t pixels, int cp, int dxc, int c0) {
+__global__ void route_same_kernel(float *g, const float *dx, int64_t pixels, int cp, int dxc, int c0, int first) {
     const uint32_t gp = uint32_t(cp) / 4u;
     const uint64_t count = uint64_t(pixels) * gp;
     float4 *g4 = reinterpret_cast<float4 *>(g);
     for(uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += uint64_t(gridDim.x) * blockDim.x) {
         const uint64_t p = udiv(i, gp);
         const uint32_t c = uint32_t(i - p * gp) * 4u;
-        g4[i] = add4(g4[i], *reinterpret_cast<const float4 *>(dx + p * uint64_t(dxc) + uint64_t(c0) + c));
+        g4[i] = accumulate(g4, i, first, *reinterpret_cast<const float4 *>(dx + p * uint64_t(dxc) + uint64_t(c0) + c));
     }
 }
 // Gradient of a 2x nearest upsample: g[q][c] += sum of the 2x2 block of dx
 // (full resolution (n, h, w), dxc channels, channels [0, cp)).
-__global__ void route_up_kernel(float *g, const float *dx, int n, int h, int w, int cp, int dxc) {
+__global__ void route_up_kernel(float *g, const float *dx, int n, int h, int w, int cp, int dxc, int first) {
     const uint32_t ho = uint32_t(h) >> 1, wo = uint32_t(w) >> 1, gp = uint32_t(cp) / 4u;
     const uint64_t count = uint64_t(n) * ho * wo * gp;
     float4 *g4 = reinterpret_cast<float4 *>(g);
@@ -489,13 +500,13 @@ __global__ void route_up_kernel(float *g, const float *dx, int n, int h, int w, 
         const float4 d01 = *reinterpret_cast<const float4 *>(dx + (r0 + 1) * uint64_t(dxc) + c);
         const float4 d10 = *reinterpret_cast<const float4 *>(dx + r1 * uint64_t(dxc) + c);
         const float4 d11 = *reinterpret_cast<const float4 *>(dx + (r1 + 1) * uint64_t(dxc) + c);
-        g4[i] = add4(g4[i], add4(add4(add4(d00, d01), d10), d11));
+        g4[i] = accumulate(g4, i, first, add4(add4(add4(d00, d01), d10), d11));
     }
 }
 // Gradient of a 2x2 average pool: g[p][c] += dx[pool(p)][c] / 4 (g at full
 // resolution (n, h, w), dx at half, both cp channels); one thread per pooled
 // group, updating its four full-resolution pixels.
-__global__ void route_pool_kernel(float *g, const float *dx, int n, int h, int w, int cp) {
+__global__ void route_pool_kernel(float *g, const float *dx, int n, int h, int w, int cp, int first) {
     const uint32_t ho = uint32_t(h) >> 1, wo = uint32_t(w) >> 1, gp = uint32_t(cp) / 4u;
     const uint64_t count = uint64_t(n) * ho * wo * gp;
     float4 *g4 = reinterpret_cast<float4 *>(g);
@@ -510,7 +521,7 @@ __global__ void route_pool_kernel(float *g, const float *dx, int n, int h, int w
         const float4 v = make_float4(0.25f * d.x, 0.25f * d.y, 0.25f * d.z, 0.25f * d.w);
         const uint64_t p0 = (b * uint64_t(h) + 2u * y) * uint64_t(w) + 2u * x, p1 = p0 + uint64_t(w);
         const uint64_t fine[4] = {p0, p0 + 1, p1, p1 + 1};
-        for(int j = 0; j < 4; j++) g4[fine[j] * gp + k] = add4(g4[fine[j] * gp + k], v);
+        for(int j = 0; j < 4; j++) g4[fine[j] * gp + k] = accumulate(g4, fine[j] * gp + k, first, v);
     }
 }
 // dY (bf16) of the layer that produced y: g where y > 0 (ReLU), else 0;
@@ -851,8 +862,7 @@ struct ptd_network {
             int64_t splits, per_split;
             wgrad_plan(l, splits, per_split);
             wp_max = std::max(wp_max, size_t(splits) * 9 * size_t(cpa[l] + cpb[l]) * size_t(cpo[l]));
-            const int64_t bblocks = (level_pixels(SPECS[l].level) + BIAS_PIXELS - 1) / BIAS_PIXELS;
-            bp_max = std::max(bp_max, size_t(bblocks) * size_t(cpo[l]));
+            bp_max = std::max(bp_max, size_t(splits) * size_t(cpo[l]));
         }
         wpartial.ensure(wp_max);
         bpartial.ensure(bp_max);
@@ -867,7 +877,6 @@ struct ptd_network {
         per_split = (nsub + std::min(nsub, want) - 1) / std::min(nsub, want);
         splits = (nsub + per_split - 1) / per_split;
     }
-    static constexpr int64_t BIAS_PIXELS = 2048;  // pixels per block of the bias-gradient reduction
 
     conv_src source(int l) const {
         const layer_spec& s = SPECS[l];
@@ -946,33 +955,33 @@ struct ptd_network {
         const layer_spec& s = SPECS[l];
         const conv_src src = source(l);
         const int cpi = cpa[l] + cpb[l];
-        const int64_t pixels = level_pixels(s.level);
         int64_t used, bps;
         wgrad_plan(l, used, bps);
         const dim3 grid(unsigned(used), unsigned(cpi / 32));
         switch(cpo[l] / 16) {
-            case 2: hipLaunchKernelGGL(wgrad_kernel<2>, grid, dim3(256), 0, stream, src, dy.p, cpo[l], bps, cpi, wpartial.p); break;
-            case 4: hipLaunchKernelGGL(wgrad_kernel<4>, grid, dim3(256), 0, stream, src, dy.p, cpo[l], bps, cpi, wpartial.p); break;
-            case 6: hipLaunchKernelGGL(wgrad_kernel<6>, grid, dim3(256), 0, stream, src, dy.p, cpo[l], bps, cpi, wpartial.p); break;
+            case 2: hipLaunchKernelGGL(wgrad_kernel<2>, grid, dim3(256), 0, stream, src, dy.p, cpo[l], bps, cpi, wpartial.p, bpartial.p); break;
+            case 4: hipLaunchKernelGGL(wgrad_kernel<4>, grid, dim3(256), 0, stream, src, dy.p, cpo[l], bps, cpi, wpartial.p, bpartial.p); break;
+            case 6: hipLaunchKernelGGL(wgrad_kernel<6>, grid, dim3(256), 0, stream, src, dy.p, cpo[l], bps, cpi, wpartial.p, bpartial.p); break;
             default: throw std::logic_error("unsupported output width");
         }
         const int64_t nw = int64_t(s.cout) * 9 * (s.cin_a + s.cin_b);
         const int64_t per_split = int64_t(9) * cpi * cpo[l];
         hipLaunchKernelGGL(wgrad_reduce_kernel, dim3(unsigned((per_split + 63) / 64)), dim3(64, 4), 0, stream, wpartial.p,
                            int(used), cpi, cpo[l], s.cin_a, cpa[l], s.cin_b, s.cout, grads.p + offsets[l]);
-        const int64_t bblocks = (pixels + BIAS_PIXELS - 1) / BIAS_PIXELS;
-        hipLaunchKernelGGL(bgrad_kernel, dim3(unsigned(bblocks)), dim3(256), 0, stream, dy.p, pixels, cpo[l], BIAS_PIXELS,
-                           bpartial.p);
-        hipLaunchKernelGGL(bgrad_reduce_kernel, dim3(unsigned(s.cout)), dim3(256), 0, stream, bpartial.p, int(bblocks),
+        hipLaunchKernelGGL(bgrad_reduce_kernel, dim3(unsigned(s.cout)), dim3(256), 0, stream, bpartial.p, int(used),
                            cpo[l], s.cout, grads.p + offsets[l] + nw);
         HC(hipGetLastError());
     }
 
     // Backward through all layers from dy = dL/d(out); fills grads.
     void backward() {
-        for(int k = 0; k < 10; k++)
-            HC(hipMemsetAsync(grad_y[k].p, 0, size_t(level_pixels(SPECS[k].level)) * size_t(tensor_cp[t_y(k)]) * sizeof(float), stream));
+        // grad_y[k] is written by its first contribution (consumers come
+        // later in the layer order, so backward reaches them first) and added
+        // to by the others; every hidden output has at least one consumer.
+        bool written[10] = {};
+        auto first = [&](int k) { const bool f = !written[k]; written[k] = true; return f ? 1 : 0; };
         for(int l = NLAYERS - 1; l >= 0; l--) {
+            if(l < NLAYERS - 1 && !written[l]) throw std::logic_error("hidden output without a gradient");
             const layer_spec& s = SPECS[l];
             if(l < NLAYERS - 1) {  // dy of this layer from its output's gradient and ReLU
                 const int64_t count = level_pixels(s.level) * tensor_cp[t_y(l)];
@@ -992,21 +1001,21 @@ struct ptd_network {
                 const int ka = s.src_a - T_Y0, kb = s.src_b - T_Y0;
                 const int64_t ca = level_pixels(s.level + 1) * cpa[l] / 4;
                 hipLaunchKernelGGL(route_up_kernel, dim3(grid_for(ca)), dim3(256), 0, stream, grad_y[ka].p, dx.p, n,
-                                   level_h(s.level), level_w(s.level), cpa[l], cpi);
+                                   level_h(s.level), level_w(s.level), cpa[l], cpi, first(ka));
                 const int64_t cb = pixels * cpb[l] / 4;
                 hipLaunchKernelGGL(route_same_kernel, dim3(grid_for(cb)), dim3(256), 0, stream, grad_y[kb].p, dx.p, pixels,
-                                   cpb[l], cpi, cpa[l]);
+                                   cpb[l], cpi, cpa[l], first(kb));
             } else if(s.src_a >= T_P1) {  // pooled encoder output
                 const int k = l - 1;
                 const int lv = SPECS[k].level;
                 const int64_t count = level_pixels(lv + 1) * tensor_cp[t_y(k)] / 4;
                 hipLaunchKernelGGL(route_pool_kernel, dim3(grid_for(count)), dim3(256), 0, stream, grad_y[k].p, dx.p, n,
-                                   level_h(lv), level_w(lv), tensor_cp[t_y(k)]);
+                                   level_h(lv), level_w(lv), tensor_cp[t_y(k)], first(k));
             } else {  // plain: the previous layer's output
                 const int k = s.src_a - T_Y0;
                 const int64_t count = pixels * cpi / 4;
                 hipLaunchKernelGGL(route_same_kernel, dim3(grid_for(count)), dim3(256), 0, stream, grad_y[k].p, dx.p, pixels,
-                                   cpi, cpi, 0);
+                                   cpi, cpi, 0, first(k));
             }
             HC(hipGetLastError());
         }
