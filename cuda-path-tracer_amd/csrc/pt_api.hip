@@ -422,6 +422,7 @@ struct pt_renderer {
     dev_buf<float> sum;
     dev_buf<uint64_t> rng;
     dev_buf<uint8_t> rgba;
+    mutable dev_buf<float> image_buf;    // image(): the scaled sums, allocated at the first read
     dev_buf<pt::loop_ctl> ctl;           // debug views' counters
     pt::loop_ctl *ctl_host = nullptr;
     // wavefront state (shaded mode only), indexed by pixel and shared by the partitions
@@ -1232,6 +1233,19 @@ pt_status pt_device_available(char *description, size_t capacity) {
     return st;
 }
 
+pt_status pt_host_alloc(size_t bytes, void **out) {
+    return guarded([&] {
+        if(!out) fail(PT_EINVAL, "null output pointer");
+        *out = nullptr;
+        if(bytes == 0) fail(PT_EINVAL, "zero-byte host buffer");
+        HIP_CHECK(hipHostMalloc(out, bytes, hipHostMallocDefault));
+    });
+}
+
+pt_status pt_host_free(void *p) {
+    return guarded([&] { if(p) HIP_CHECK(hipHostFree(p)); });
+}
+
 pt_status pt_renderer_create(const pt_scene_desc *scene, const pt_flat_camera *camera, const pt_settings *settings,
                              pt_renderer **out) {
     return guarded([&] {
@@ -1307,12 +1321,18 @@ pt_status pt_renderer_read_image(const pt_renderer *renderer, float *rgb, size_t
     return guarded([&] {
         if(!renderer || !rgb) fail(PT_EINVAL, "null argument");
         if(float_count < 3 * renderer->pixels) fail(PT_EINVAL, "image buffer is too small");
-        HIP_CHECK(hipMemcpy(rgb, renderer->sum.get(), 3 * renderer->pixels * sizeof(float), hipMemcpyDeviceToHost));
+        const size_t count = 3 * renderer->pixels;
+        const float *src = renderer->sum.get();
         if(renderer->samples) {
-            // value /= float(samples): multiply by float(1.0/samples) (vec3.h:152-158)
-            float k = float(1.0 / double(float(renderer->samples)));
-            for(size_t i = 0; i < 3 * renderer->pixels; i++) rgb[i] = rgb[i] * k;
+            // value /= float(samples): multiply by float(1.0/samples) (vec3.h:152-158), on the device
+            const float k = float(1.0 / double(float(renderer->samples)));
+            if(renderer->image_buf.size() < count) renderer->image_buf.alloc(count);
+            pt::launch_scale(src, renderer->image_buf.get(), int64_t(count), k, renderer->stream);
+            HIP_CHECK(hipGetLastError());
+            src = renderer->image_buf.get();
         }
+        HIP_CHECK(hipMemcpyAsync(rgb, src, count * sizeof(float), hipMemcpyDeviceToHost, renderer->stream));
+        HIP_CHECK(hipStreamSynchronize(renderer->stream));
     });
 }
 

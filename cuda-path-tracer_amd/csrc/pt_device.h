@@ -245,6 +245,7 @@ void launch_zero_work(loop_ctl *ctl, hipStream_t stream);
 // Zero a control block and set n_ext[0] = n (the item count of a launch).
 void launch_fresh_ctl(loop_ctl *ctl, uint32_t n, hipStream_t stream);
 void launch_pack(const float *sum, uint8_t *rgba, int64_t count, int32_t samples, hipStream_t stream);
+void launch_scale(const float *sum, float *out, int64_t count, float k, hipStream_t stream);
 void launch_add_rays(loop_ctl *ctl, unsigned long long rays, hipStream_t stream);
 
 // Dynamic LDS bytes the traversal kernels need per block.

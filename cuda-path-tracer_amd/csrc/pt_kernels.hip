@@ -1412,6 +1412,12 @@ __global__ void pack_kernel(const float *sum, uint8_t *rgba, int64_t count, int3
     rgba[4 * i + 3] = 255;
 }
 
+// image(): sum * float(1.0 / samples) per component (vec3.h:152-158).
+__global__ void scale_kernel(const float *sum, float *out, int64_t count, float k) {
+    for(int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += int64_t(gridDim.x) * blockDim.x)
+        out[i] = sum[i] * k;
+}
+
 } // namespace
 
 __global__ void trig_values_kernel(float2 *out) {
@@ -1622,6 +1628,10 @@ void launch_zero_work(loop_ctl *ctl, hipStream_t stream) {
 void launch_pack(const float *sum, uint8_t *rgba, int64_t count, int32_t samples, hipStream_t stream) {
     unsigned grid = unsigned((count + 255) / 256);
     hipLaunchKernelGGL(pack_kernel, dim3(grid ? grid : 1), dim3(256), 0, stream, sum, rgba, count, samples);
+}
+void launch_scale(const float *sum, float *out, int64_t count, float k, hipStream_t stream) {
+    const int64_t blocks = std::min<int64_t>(std::max<int64_t>(1, (count + 255) / 256), 65536);
+    hipLaunchKernelGGL(scale_kernel, dim3(unsigned(blocks)), dim3(256), 0, stream, sum, out, count, k);
 }
 void launch_add_rays(loop_ctl *ctl, unsigned long long rays, hipStream_t stream) {
     hipLaunchKernelGGL(add_rays_kernel, dim3(1), dim3(1), 0, stream, ctl, rays);

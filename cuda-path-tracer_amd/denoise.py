@@ -137,9 +137,13 @@ class Denoiser:
                                ctypes.byref(ms)))
         return losses, float(ms.value)
 
-    def denoise_renderer(self, renderer):
-        """(W*H, 3) denoised radiance of a renderer's accumulation and the device milliseconds."""
-        out = np.zeros((renderer.pixels, 3), np.float32)
+    def denoise_renderer(self, renderer, out=None):
+        """(W*H, 3) denoised radiance of a renderer's accumulation (into `out`
+        when given, e.g. a ptrace.pinned() array) and the device milliseconds."""
+        if out is None:
+            out = np.zeros((renderer.pixels, 3), np.float32)
+        elif out.dtype != np.float32 or out.size < 3 * renderer.pixels or not out.flags.c_contiguous:
+            raise ValueError("out must be a C-contiguous float32 array of at least 3 * pixels values")
         ms = ctypes.c_float()
         _check(lib().ptd_denoise_renderer(self._h, renderer._h, out.ctypes.data, ctypes.byref(ms)))
         return out, float(ms.value)
@@ -284,7 +288,10 @@ def time_to_relmse(scene, camera, width, height, net, target=0.01, reference_spp
                    plain_spp=(4, 8, 16, 24, 32, 40, 48, 56, 64, 96, 128, 256, 512, 1024, 2048, 4096),
                    denoised_spp=(1, 4, 16), seed=7, log=print):
     """Wall time for plain progressive accumulation and for render + denoise to
-    reach relMSE <= target against a high-spp render of the same view."""
+    reach relMSE <= target against a high-spp render of the same view.  Both
+    forms deliver their image into the same page-locked host buffer
+    (ptrace.pinned), allocated once beforehand like the device buffers."""
+    result = ptrace.pinned((width * height, 3))
     ref = ptrace.Renderer(scene, camera, width, height, 99991, False, ptrace.DEBUG_SHADED, True, 8)
     t0 = time.perf_counter()
     done = 0
@@ -312,7 +319,7 @@ def time_to_relmse(scene, camera, width, height, net, target=0.01, reference_spp
         # The image read back to the host is part of the result, as it is for
         # the denoised form (denoise_renderer returns host memory).
         t0 = time.perf_counter()
-        image = r.image()
+        image = r.image(out=result)
         readback = time.perf_counter() - t0
         plain.append({"spp": spp, "seconds": spent + readback, "relmse": relmse(image, reference)})
         log("plain", plain[-1])
@@ -321,7 +328,7 @@ def time_to_relmse(scene, camera, width, height, net, target=0.01, reference_spp
     with ptrace.Renderer(scene, camera, width, height, seed + 1, False, ptrace.DEBUG_SHADED, True, 8) as warm:
         warm.enable_aovs()
         warm.add_samples(1)
-        net.denoise_renderer(warm)
+        net.denoise_renderer(warm, out=result)
     den = []
     for spp in sorted(denoised_spp):
         r = ptrace.Renderer(scene, camera, width, height, seed, False, ptrace.DEBUG_SHADED, True, 8)
@@ -330,7 +337,7 @@ def time_to_relmse(scene, camera, width, height, net, target=0.01, reference_spp
         r.reset(camera)
         t0 = time.perf_counter()
         r.add_samples(spp)
-        out, ms = net.denoise_renderer(r)
+        out, ms = net.denoise_renderer(r, out=result)
         secs = time.perf_counter() - t0
         den.append({"spp": spp, "seconds": secs, "denoise_ms": ms, "relmse": relmse(out, reference),
                     "noisy_relmse": relmse(r.image(), reference)})

@@ -139,6 +139,8 @@ SCENE_LIBRARY_PATH = os.path.join(HERE, "libptscene.so")
 PT_FUNCTIONS = {
     "pt_last_error": (ctypes.c_char_p, []),
     "pt_device_available": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t]),
+    "pt_host_alloc": (ctypes.c_int, [ctypes.c_size_t, ctypes.c_void_p]),
+    "pt_host_free": (ctypes.c_int, [ctypes.c_void_p]),
     "pt_renderer_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "pt_renderer_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "pt_renderer_reset": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
@@ -243,6 +245,23 @@ def device_available():
     return ok, buf.value.decode()
 
 
+def pinned(shape, dtype=np.float32):
+    """A zeroed numpy array in page-locked host memory (pt_host_alloc), freed
+    with the array: device-to-host reads into it (Renderer.image(out=...),
+    the denoiser's output) run at the link's rate."""
+    dtype = np.dtype(dtype)
+    count = int(np.prod(shape))
+    nbytes = max(1, count * dtype.itemsize)
+    ptr = ctypes.c_void_p()
+    _check(lib().pt_host_alloc(nbytes, ctypes.byref(ptr)))
+    buf = (ctypes.c_uint8 * nbytes).from_address(ptr.value)
+    arr = np.frombuffer(buf, dtype=dtype, count=count).reshape(shape)
+    arr[...] = 0
+    import weakref
+    weakref.finalize(buf, lib().pt_host_free, ctypes.c_void_p(ptr.value))
+    return arr
+
+
 # ---------------------------------------------------------------- renderer
 class Renderer:
     """cuda_renderer (src/cuda/cuda_renderer.h:10-29) over the HIP core.
@@ -295,9 +314,13 @@ class Renderer:
     def set_block_shape(self, width, height):
         _check(lib().pt_renderer_set_block_shape(self._h, int(width), int(height)))
 
-    def image(self):
-        """Averaged linear RGB, (width*height, 3) float32, row 0 at the bottom."""
-        out = np.empty((self.pixels, 3), np.float32)
+    def image(self, out=None):
+        """Averaged linear RGB, (width*height, 3) float32, row 0 at the bottom;
+        into `out` when given (e.g. a pinned() array)."""
+        if out is None:
+            out = np.empty((self.pixels, 3), np.float32)
+        elif out.dtype != np.float32 or out.size < 3 * self.pixels or not out.flags.c_contiguous:
+            raise ValueError("out must be a C-contiguous float32 array of at least 3 * pixels values")
         _check(lib().pt_renderer_read_image(self._h, out.ctypes.data, out.size))
         return out
 

@@ -107,6 +107,11 @@ const char *pt_last_error(void);
  * device name, or PT_EDEVICE and the reason, written NUL-terminated into
  * description[0..capacity). */
 pt_status pt_device_available(char *description, size_t capacity);
+/* Page-locked host memory for read-backs (no counterpart in the reference):
+ * device-to-host copies into it run at the link's rate instead of through a
+ * staging copy.  Free with pt_host_free. */
+pt_status pt_host_alloc(size_t bytes, void **out);
+pt_status pt_host_free(void *p);
 
 /* cuda_renderer::cuda_renderer (cuda_renderer.h:12-13, .cu:177-183):
  * validates scene, camera, mode and size, uploads, then reset(camera). */
@@ -135,7 +140,9 @@ pt_status pt_renderer_add_samples_noalloc(pt_renderer *renderer, int32_t count);
  * 8x8, 16x8, 16x16, 32x8; otherwise PT_EINVAL. */
 pt_status pt_renderer_set_block_shape(pt_renderer *renderer, int32_t width, int32_t height);
 /* cuda_renderer::image (cuda_renderer.h:20, .cu:230-234): width*height*3
- * floats of sum /= float(samples), row 0 at the bottom. */
+ * floats of sum /= float(samples), row 0 at the bottom.  The scaling runs on
+ * the device (the same float products); the copy to rgb is fastest when rgb
+ * is page-locked (pt_host_alloc). */
 pt_status pt_renderer_read_image(const pt_renderer *renderer, float *rgb, size_t float_count);
 /* cuda_renderer::rgba_image (cuda_renderer.h:21, .cu:241-244) */
 pt_status pt_renderer_read_rgba(const pt_renderer *renderer, uint8_t *rgba, size_t bytes);

@@ -156,6 +156,10 @@ def test_trained_network_beats_the_plain_render():
     plain = denoise.relmse(r.image(), ref)
     den = denoise.relmse(out, ref)
     assert den < 0.5 * plain, (den, plain)
+    # the same result delivered into a page-locked buffer
+    buf = ptrace.pinned((128 * 128, 3))
+    got, _ = net.denoise_renderer(r, out=buf)
+    assert got is buf and np.array_equal(buf, out)
 
 
 @pytest.mark.gpu

@@ -179,3 +179,23 @@ def test_write_rgba_into_caller_device_buffer_and_kernel_time():
         hip.hipFree(ptr)
     r.reset(cam)
     assert r.kernel_milliseconds() == 0
+
+
+def test_image_into_page_locked_buffer():
+    # image() scales the sums on the device (the same float products as the
+    # reference's host loop, cuda_renderer.cu:230-234) and copies into any
+    # host buffer; a pinned() one takes the direct path.
+    scene, cam = ptrace.cornell(7, 64, 40)
+    r = ptrace.Renderer(scene, cam, 64, 40, 7, False, 0, True, 8)
+    buf = ptrace.pinned((64 * 40, 3))
+    assert not r.image(out=buf).any()  # no samples yet: the zero sums
+    r.add_samples(3)
+    got = r.image(out=buf)
+    assert got is buf
+    o = oracle.Renderer(scene, cam, 64, 40, 7, False, 0, True, 8)
+    o.add_samples(3)
+    assert np.array_equal(buf, o.image()) and np.array_equal(r.image(), o.image())
+    with pytest.raises(ValueError):
+        r.image(out=np.zeros((10, 3), np.float32))
+    r.reset(cam)
+    assert not r.image(out=buf).any()
