@@ -68,6 +68,15 @@ def test_boundary_validation_runs_before_the_device():
         ptrace.query_rays(broken, np.zeros(1, ptrace.RAY))
 
 
+def test_host_buffer_arguments_are_checked_before_the_device():
+    import ctypes
+    L = ptrace.lib()
+    out = ctypes.c_void_p()
+    assert L.pt_host_alloc(0, ctypes.byref(out)) == ptrace.PT_EINVAL and out.value is None
+    assert L.pt_host_alloc(16, None) == ptrace.PT_EINVAL
+    assert L.pt_host_free(None) == ptrace.PT_OK
+
+
 @pytest.mark.parametrize("fixture", ["asset_cube.npz", "asset_sphere.npz", "asset_triangle.npz", "cornell.npz",
                                      "pruning.npz", "spheres.npz"])
 def test_c_abi_validation_agrees_with_oracle(fixture):
