@@ -174,6 +174,46 @@ def batch_workload(ptrace, count, seed=SEED):
     return scenes, views, int(tris.sum())
 
 
+def bench_build(args, ptrace):
+    """The device linear BVH builder (pt_build_bvh) on the frame's scene:
+    device time per build over the timed steps (HIP events around the build's
+    kernels), the call's wall time (host checks, copies in and out, the depth
+    walk), the host binned-SAH build of the same primitives, and the 16-spp
+    1080p render rate on each tree."""
+    scene, camera = ptrace.icosphere(FREQ, SEED, WIDTH, HEIGHT)
+    arrays = (scene.triangles, scene.spheres, scene.materials, scene.textures, scene.pixels)
+    n = len(scene.triangles) + len(scene.spheres)
+    for _ in range(args.warmup):
+        ptrace.build_bvh_device(*arrays)
+    device_ms, call_ms = [], []
+    for _ in range(args.steps):
+        t = time.perf_counter()
+        built, ms = ptrace.build_bvh_device(*arrays)
+        call_ms.append((time.perf_counter() - t) * 1e3)
+        device_ms.append(ms)
+        log(f"device build: {ms:.3f} ms on the device, {call_ms[-1]:.1f} ms for the call")
+    t = time.perf_counter()
+    sah = ptrace.build_scene(*arrays, split="sah")
+    host_ms = (time.perf_counter() - t) * 1e3
+    rates = {}
+    for name, tree in (("sah", sah), ("device", built)):
+        r = ptrace.Renderer(tree, camera, WIDTH, HEIGHT, SEED, False, ptrace.DEBUG_SHADED, True, DEPTH)
+        r.add_samples(1)
+        r.reset(camera)
+        r.add_samples(16)
+        rates[name] = round(r.rays_traced() / r.kernel_milliseconds() / 1e3, 1)
+        r.close()
+    mean = sum(device_ms) / len(device_ms)
+    print(json.dumps({
+        "metric": "bvh_build_primitives_per_s", "value": round(n / (mean / 1e3), 1), "unit": "primitives/s",
+        "higher_is_better": True, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "device_ms": round(mean, 4), "call_ms": round(sum(call_ms) / len(call_ms), 2),
+        "host_sah_ms": round(host_ms, 1), "host_sah_depth": sah.max_depth, "device_depth": built.max_depth,
+        "render_mrays_16spp": rates, "data": "synthetic",
+        "config": {"workload": f"device linear BVH of the {n}-triangle 1080p icosphere frame", "primitives": n}}))
+    return 0
+
+
 def bench_batch(args, ptrace):
     t0 = time.perf_counter()
     scenes, views, total_tris = batch_workload(ptrace, args.scenes)
@@ -252,8 +292,9 @@ def main():
     ap.add_argument("--spp", type=int, default=SPP)
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a plain 1 GiB device copy (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
-    ap.add_argument("--workload", choices=["frame", "batch"], default="frame",
-                    help="frame: the 1080p 1M-triangle frame (default); batch: 64 scenes in one batched loop")
+    ap.add_argument("--workload", choices=["frame", "batch", "build"], default="frame",
+                    help="frame: the 1080p 1M-triangle frame (default); batch: 64 scenes in one batched loop; "
+                         "build: the device BVH builder on the frame's scene")
     ap.add_argument("--scenes", type=int, default=64, help="batch workload: number of scenes")
     ap.add_argument("--autotune", action="store_true", help="batch workload: run the autotuner before timing")
     args = ap.parse_args()
@@ -279,6 +320,8 @@ def main():
     log("device:", device)
     if args.workload == "batch":
         return bench_batch(args, ptrace)
+    if args.workload == "build":
+        return bench_build(args, ptrace)
     t0 = time.perf_counter()
     scene, camera = ptrace.icosphere(FREQ, SEED, WIDTH, HEIGHT)
     log(f"scene: {len(scene.triangles)} triangles, {len(scene.nodes)} nodes, depth {scene.max_depth}, "

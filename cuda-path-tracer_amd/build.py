@@ -44,7 +44,7 @@ def _run(cmd, cwd=None):
 def build_ptrace():
     out = os.path.join(HERE, "libptrace.so")
     objs = []
-    for name in ("pt_kernels.hip", "pt_api.hip", "pt_denoise.hip"):
+    for name in ("pt_kernels.hip", "pt_api.hip", "pt_denoise.hip", "pt_lbvh.hip"):
         obj = os.path.join(HERE, "_build", name + ".o")
         os.makedirs(os.path.dirname(obj), exist_ok=True)
         _run([HIPCC, *HIP_FLAGS, "-I", INCLUDE, "-I", CSRC, "-c", os.path.join(CSRC, name), "-o", obj])

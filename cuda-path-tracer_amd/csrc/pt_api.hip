@@ -1246,6 +1246,56 @@ pt_status pt_host_free(void *p) {
     return guarded([&] { if(p) HIP_CHECK(hipHostFree(p)); });
 }
 
+pt_status pt_build_bvh(const pt_flat_triangle *triangles, int64_t triangle_count, const pt_flat_sphere *spheres,
+                       int64_t sphere_count, pt_flat_node *nodes, int64_t node_capacity, int32_t *root,
+                       int32_t *max_depth, float *milliseconds) {
+    return guarded([&] {
+        if(!root || !max_depth) fail(PT_EINVAL, "null output pointer");
+        *root = -1;
+        *max_depth = 0;
+        if(milliseconds) *milliseconds = 0;
+        if(triangle_count < 0 || sphere_count < 0 || (triangle_count && !triangles) || (sphere_count && !spheres))
+            fail(PT_EINVAL, "invalid primitive arrays");
+        const int64_t n = triangle_count + sphere_count;
+        if(n > int64_t(INT32_MAX / 2)) fail(PT_EINVAL, "too many primitives for 32-bit node indices");
+        // The leaf-box rules of flat_validate.h:66-77 need finite vertices and
+        // finite, positive radii.
+        for(int64_t i = 0; i < triangle_count; i++) {
+            const pt_flat_triangle& t = triangles[i];
+            if(!finite3(t.v0) || !finite3(t.v1) || !finite3(t.v2)) fail(PT_ESCENE, "non-finite triangle vertex");
+        }
+        for(int64_t i = 0; i < sphere_count; i++) {
+            const pt_flat_sphere& s = spheres[i];
+            if(!finite3(s.center) || !std::isfinite(s.radius) || !(s.radius > 0)) fail(PT_ESCENE, "invalid sphere");
+        }
+        if(n == 0) return;
+        if(!nodes || node_capacity < 2 * n - 1) fail(PT_EINVAL, "node array holds fewer than 2n - 1 nodes");
+        require_device();
+        pt::build_lbvh(triangles, triangle_count, spheres, sphere_count, nodes, milliseconds);
+        // Depth of the hierarchy (the traversal stack bound, flat_data.h:52).
+        const int64_t count = 2 * n - 1;
+        std::vector<std::pair<int32_t, int>> stack{{0, 1}};
+        int depth = 0;
+        int64_t visited = 0;
+        while(!stack.empty()) {
+            const auto [id, d] = stack.back();
+            stack.pop_back();
+            if(id < 0 || id >= count || ++visited > count) fail(PT_EDEVICE, "linear BVH build produced a broken tree");
+            depth = std::max(depth, d);
+            if(nodes[id].type == PT_BRANCH) {
+                stack.push_back({nodes[id].right, d + 1});
+                stack.push_back({nodes[id].left, d + 1});
+            }
+        }
+        if(visited != count) fail(PT_EDEVICE, "linear BVH build produced a broken tree");
+        if(depth > pt::stack_limit)
+            fail(PT_ESCENE, "linear BVH deeper than " + std::to_string(pt::stack_limit) +
+                                " levels; build this scene with the host builder (pts_build)");
+        *root = 0;
+        *max_depth = depth;
+    });
+}
+
 pt_status pt_renderer_create(const pt_scene_desc *scene, const pt_flat_camera *camera, const pt_settings *settings,
                              pt_renderer **out) {
     return guarded([&] {

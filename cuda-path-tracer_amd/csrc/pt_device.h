@@ -277,4 +277,11 @@ struct frame_view {
 };
 bool renderer_frame(const pt_renderer *renderer, frame_view *out);
 
+// Linear BVH of nt triangles and ns spheres (nt + ns >= 1, validated by the
+// caller) built on the current device: 2 (nt + ns) - 1 flat nodes into
+// nodes_out (host memory), root 0; the device time of the build in
+// *milliseconds (may be null).  pt_lbvh.hip.
+void build_lbvh(const pt_flat_triangle *triangles, int64_t nt, const pt_flat_sphere *spheres, int64_t ns,
+                pt_flat_node *nodes_out, float *milliseconds);
+
 } // namespace pt

@@ -141,6 +141,8 @@ PT_FUNCTIONS = {
     "pt_device_available": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t]),
     "pt_host_alloc": (ctypes.c_int, [ctypes.c_size_t, ctypes.c_void_p]),
     "pt_host_free": (ctypes.c_int, [ctypes.c_void_p]),
+    "pt_build_bvh": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
+                                    ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "pt_renderer_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "pt_renderer_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "pt_renderer_reset": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
@@ -558,6 +560,25 @@ def build_scene(triangles=None, spheres=None, materials=None, textures=None, pix
     h = scene_lib().pts_build_seeded(_ptr(t), len(t), _ptr(s), len(s), _ptr(m), len(m), _ptr(x), len(x), _ptr(p),
                                      len(p), SPLITS[split], int(seed))
     return _take_scene(h)
+
+
+def build_bvh_device(triangles=None, spheres=None, materials=None, textures=None, pixels=None):
+    """Linear BVH built on the device (pt_build_bvh): a FlatScene over the
+    given arrays, primitives in their given order, and the device time of
+    the build in milliseconds."""
+    t = np.ascontiguousarray(triangles if triangles is not None else np.zeros(0, TRIANGLE), TRIANGLE)
+    s = np.ascontiguousarray(spheres if spheres is not None else np.zeros(0, SPHERE), SPHERE)
+    n = len(t) + len(s)
+    nodes = np.zeros(max(2 * n - 1, 0), NODE)
+    root, depth, ms = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_float()
+    _check(lib().pt_build_bvh(_ptr(t), len(t), _ptr(s), len(s), _ptr(nodes), len(nodes), ctypes.byref(root),
+                              ctypes.byref(depth), ctypes.byref(ms)))
+    scene = FlatScene(t, s, nodes,
+                      np.ascontiguousarray(materials if materials is not None else np.zeros(0, MATERIAL), MATERIAL),
+                      np.ascontiguousarray(textures if textures is not None else np.zeros(0, TEXTURE), TEXTURE),
+                      np.ascontiguousarray(pixels if pixels is not None else np.zeros(0, np.uint8), np.uint8),
+                      root.value, depth.value)
+    return scene, ms.value
 
 
 def load_obj(path, aspect, scale=(1, 1, 1), rotate=(0, 0, 0), translate=(0, 0, 0), split="reference", seed=7,

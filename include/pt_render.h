@@ -113,6 +113,25 @@ pt_status pt_device_available(char *description, size_t capacity);
 pt_status pt_host_alloc(size_t bytes, void **out);
 pt_status pt_host_free(void *p);
 
+/* Linear BVH built on the device (no counterpart in the reference, whose
+ * hierarchy is built on the host, src/bvh/bvh.h:132-171; for callers that
+ * rebuild large or changing scenes): the flat hierarchy of the given
+ * triangles and spheres (primitive p < triangle_count is triangle p, the
+ * rest spheres), one primitive per leaf with the reference's leaf boxes
+ * (triangle.h:96-109, sphere.h:69-75) and branch boxes the union of their
+ * children (aabb.h:40-56).  Karras 2012: 63-bit Morton codes of the box
+ * centroids, radix sort, independent splits, bottom-up boxes.  Writes
+ * 2n - 1 nodes (n = triangle_count + sphere_count) into nodes (host memory,
+ * node_capacity entries), *root = 0 and *max_depth; primitives keep their
+ * indices.  n == 0 gives *root = -1, *max_depth = 0.  PT_EINVAL for bad
+ * arrays or a short node array, PT_ESCENE for a non-finite vertex, a bad
+ * sphere, or a tree deeper than the 64-level traversal stack (then use
+ * pts_build).  *milliseconds (optional) is the device time of the build.
+ * The output is identical for identical input. */
+pt_status pt_build_bvh(const pt_flat_triangle *triangles, int64_t triangle_count, const pt_flat_sphere *spheres,
+                       int64_t sphere_count, pt_flat_node *nodes, int64_t node_capacity, int32_t *root,
+                       int32_t *max_depth, float *milliseconds);
+
 /* cuda_renderer::cuda_renderer (cuda_renderer.h:12-13, .cu:177-183):
  * validates scene, camera, mode and size, uploads, then reset(camera). */
 pt_status pt_renderer_create(const pt_scene_desc *scene, const pt_flat_camera *camera,

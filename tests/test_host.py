@@ -77,6 +77,30 @@ def test_host_buffer_arguments_are_checked_before_the_device():
     assert L.pt_host_free(None) == ptrace.PT_OK
 
 
+def test_device_bvh_arguments_are_checked_before_the_device():
+    import ctypes
+    L = ptrace.lib()
+    tri = np.zeros(2, ptrace.TRIANGLE)
+    tri["v1"], tri["v2"] = (1, 0, 0), (0, 1, 0)
+    nodes = np.zeros(2, ptrace.NODE)  # 2n - 1 = 3 needed
+    root, depth, ms = ctypes.c_int32(7), ctypes.c_int32(7), ctypes.c_float(7)
+    out = (ctypes.byref(root), ctypes.byref(depth), ctypes.byref(ms))
+    p = lambda a: ctypes.c_void_p(a.ctypes.data)
+    assert L.pt_build_bvh(p(tri), 2, None, 0, p(nodes), 2, *out) == ptrace.PT_EINVAL
+    assert "2n - 1" in L.pt_last_error().decode() and root.value == -1 and depth.value == 0 and ms.value == 0
+    assert L.pt_build_bvh(None, 2, None, 0, p(nodes), 2, *out) == ptrace.PT_EINVAL
+    assert L.pt_build_bvh(p(tri), -1, None, 0, p(nodes), 2, *out) == ptrace.PT_EINVAL
+    assert L.pt_build_bvh(p(tri), 2, None, 0, p(nodes), 3, None, *out[1:]) == ptrace.PT_EINVAL
+    bad = tri.copy()
+    bad["v2"][1, 0] = np.inf
+    assert L.pt_build_bvh(p(bad), 2, None, 0, p(nodes), 3, *out) == ptrace.PT_ESCENE
+    sph = np.zeros(1, ptrace.SPHERE)  # radius 0
+    assert L.pt_build_bvh(None, 0, p(sph), 1, p(nodes), 3, *out) == ptrace.PT_ESCENE
+    # No primitives: the empty hierarchy, without touching the device.
+    assert L.pt_build_bvh(None, 0, None, 0, None, 0, *out) == ptrace.PT_OK
+    assert root.value == -1 and depth.value == 0
+
+
 @pytest.mark.parametrize("fixture", ["asset_cube.npz", "asset_sphere.npz", "asset_triangle.npz", "cornell.npz",
                                      "pruning.npz", "spheres.npz"])
 def test_c_abi_validation_agrees_with_oracle(fixture):
