@@ -1103,6 +1103,7 @@ pt_batch *create_batch(const pt_scene_desc *scenes, int32_t scene_count, const p
             if(!pt::build_wide(s, ti, w)) fail(PT_EINVAL, "batch scenes need tree hierarchies (no shared subtrees)");
             const int32_t node_offset = int32_t(wn.size() / 8), leaf_offset = int32_t(lv.size() / 3);
             if(lv.size() / 3 + w.leaves.size() / 3 >= (size_t(1) << 27)) fail(PT_EINVAL, "batch exceeds 2^27 leaves");
+            if(wn.size() / 8 + w.nodes.size() / 8 > pt::wide_node_limit) fail(PT_EINVAL, "batch exceeds 2^25 wide nodes");
             for(size_t i = 0; i < w.nodes.size() / 8; i++) {
                 float4 *r = &w.nodes[8 * i + 6];
                 int32_t ref[4];

@@ -77,6 +77,9 @@ constexpr int wide_lds_entries = 12;  // per-lane stack entries kept in LDS
 constexpr int path_lane_block = 256;  // threads per path-lane block (the kernel's stack layout assumes it)
 
 constexpr int32_t empty_root = INT32_MIN;
+// Wide node records are addressed by 32-bit byte offsets (node * 128): a
+// layout with more nodes is not built (the binary traversal runs instead).
+constexpr size_t wide_node_limit = size_t(1) << 25;
 
 // Per-pixel path state, structure of arrays in HBM.
 // Traversal state of queries that a trace launch hands on to the next one

@@ -524,7 +524,7 @@ inline scene_layout build_layout(const pt_scene_desc& s, bool with_wide) {
     L.lights = scene_lights(s);
     L.root_ref = s.root < 0 ? empty_root : L.ref_of(s, s.root);
     L.stack_capacity = std::max(2, std::min(stack_limit, int(s.max_depth)));
-    if(with_wide && build_wide(s, L.tri_isect, L.wide)) {
+    if(with_wide && build_wide(s, L.tri_isect, L.wide) && L.wide.nodes.size() / 8 <= wide_node_limit) {
         L.has_wide = true;
         L.wide_stack_capacity = std::max(3 * L.wide.depth + 1, L.stack_capacity);
     }

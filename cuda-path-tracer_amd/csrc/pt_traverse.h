@@ -204,30 +204,30 @@ PT_HD bool traverse_brute(const scene_dev& s, const ray_in& r, float low, float 
 
 // ---------------- 4-wide nearest-first traversal (shaded path) ----------------
 // The four child boxes of a wide node (SoA: child k in component k), with
-// trace_box's arithmetic (transport.h:14-31) and the reference's near/far
-// choice by the sign of 1/d: per axis a = (near - o) * (1/d), b likewise for
-// the far plane, low = a > low ? a : low and high = b < high ? b : high in x,
-// y, z order, the box hit when high > low; returns the entry distance low.
-// fmaxf/fminf give what those ternaries give, since both skip a NaN plane
-// distance, and they let the compiler use 3-input min/max.  One form serves
-// every ray, d == 0 included, where trace_box instead rejects the box if
-// o < min or o > max and leaves low and high alone: 1/d is then +-inf, so
-// with o outside the slab one plane distance is +inf as the near one or
-// -inf as the far one (a miss), with o strictly inside they are -inf and
-// +inf (no bound), and with o on a plane that distance is 0 * inf = NaN
-// (skipped).  (min - o is never 0 for min != o: denormals are kept.)  The
-// sign of a zero distance cannot matter: low starts at 0.001 and a zero
-// high misses either way.
-PT_HD void slab_entry4(const float4& LX, const float4& LY, const float4& LZ, const float4& HX, const float4& HY,
-                       const float4& HZ, const ray_in& r, float low, float high, bool h[4], float e[4]) {
-    const bool sx = r.ix < 0, sy = r.iy < 0, sz = r.iz < 0;
-    const float lx[4] = {LX.x, LX.y, LX.z, LX.w}, ly[4] = {LY.x, LY.y, LY.z, LY.w}, lz[4] = {LZ.x, LZ.y, LZ.z, LZ.w};
-    const float hx[4] = {HX.x, HX.y, HX.z, HX.w}, hy[4] = {HY.x, HY.y, HY.z, HY.w}, hz[4] = {HZ.x, HZ.y, HZ.z, HZ.w};
+// trace_box's arithmetic (transport.h:14-31): per axis a = (near - o) * (1/d)
+// and b = (far - o) * (1/d), where the near plane is the lower one for
+// 1/d >= 0 and the upper one otherwise (the caller loads the planes in that
+// order, so no selects are needed here); low = a > low ? a : low and
+// high = b < high ? b : high in x, y, z order, the box hit when high > low;
+// returns the entry distance low.  fmaxf/fminf give what those ternaries
+// give, since both skip a NaN plane distance, and they let the compiler use
+// 3-input min/max.  One form serves every ray, d == 0 included, where
+// trace_box instead rejects the box if o < min or o > max and leaves low and
+// high alone: 1/d is then +-inf, so with o outside the slab one plane
+// distance is +inf as the near one or -inf as the far one (a miss), with o
+// strictly inside they are -inf and +inf (no bound), and with o on a plane
+// that distance is 0 * inf = NaN (skipped).  (min - o is never 0 for
+// min != o: denormals are kept.)  The sign of a zero distance cannot
+// matter: low starts at 0.001 and a zero high misses either way.
+PT_HD void slab_entry4(const float4& NX, const float4& NY, const float4& NZ, const float4& FX, const float4& FY,
+                       const float4& FZ, const ray_in& r, float low, float high, bool h[4], float e[4]) {
+    const float nx[4] = {NX.x, NX.y, NX.z, NX.w}, ny[4] = {NY.x, NY.y, NY.z, NY.w}, nz[4] = {NZ.x, NZ.y, NZ.z, NZ.w};
+    const float fx[4] = {FX.x, FX.y, FX.z, FX.w}, fy[4] = {FY.x, FY.y, FY.z, FY.w}, fz[4] = {FZ.x, FZ.y, FZ.z, FZ.w};
 #pragma unroll
     for(int k = 0; k < 4; k++) {
-        const float ax = ((sx ? hx[k] : lx[k]) - r.ox) * r.ix, bx = ((sx ? lx[k] : hx[k]) - r.ox) * r.ix;
-        const float ay = ((sy ? hy[k] : ly[k]) - r.oy) * r.iy, by = ((sy ? ly[k] : hy[k]) - r.oy) * r.iy;
-        const float az = ((sz ? hz[k] : lz[k]) - r.oz) * r.iz, bz = ((sz ? lz[k] : hz[k]) - r.oz) * r.iz;
+        const float ax = (nx[k] - r.ox) * r.ix, bx = (fx[k] - r.ox) * r.ix;
+        const float ay = (ny[k] - r.oy) * r.iy, by = (fy[k] - r.oy) * r.iy;
+        const float az = (nz[k] - r.oz) * r.iz, bz = (fz[k] - r.oz) * r.iz;
         const float lo = fmaxf(fmaxf(fmaxf(low, ax), ay), az);
         const float hi = fminf(fminf(fminf(high, bx), by), bz);
         e[k] = lo;
@@ -313,13 +313,22 @@ PT_HD bool wide_step(const scene_dev& s, wide_query& q, STACK& stk, int& boxes, 
     const ray_in& r = q.r;
     float limit = q.limit;
     if(q.cur >= 0) {
-        const float4 *nd = s.wnodes + 8 * size_t(q.cur);
-        const float4 LX = nd[0], LY = nd[1], LZ = nd[2], HX = nd[3], HY = nd[4], HZ = nd[5], RF = nd[6];
+        // The node's record at byte offset q.cur * 128 (32 bits: at most
+        // wide_node_limit nodes): lo.x lo.y lo.z hi.x hi.y hi.z ref, 16 bytes
+        // each.  Per axis the near plane is lo, or hi (48 bytes on) where
+        // 1/d < 0, and the far plane the other one.
+        const char *nb = reinterpret_cast<const char *>(s.wnodes);
+        const uint32_t at = uint32_t(q.cur) << 7;
+        const uint32_t sx = r.ix < 0 ? 48u : 0u, sy = r.iy < 0 ? 48u : 0u, sz = r.iz < 0 ? 48u : 0u;
+        auto plane = [nb](uint32_t offset) { return *reinterpret_cast<const float4 *>(nb + offset); };
+        const float4 NX = plane(at + sx), NY = plane(at + 16u + sy), NZ = plane(at + 32u + sz);
+        const float4 FX = plane(at + (sx ^ 48u)), FY = plane(at + 16u + (sy ^ 48u)), FZ = plane(at + 32u + (sz ^ 48u));
+        const float4 RF = plane(at + 96u);
         const int32_t r0 = f2i(RF.x), r1 = f2i(RF.y), r2 = f2i(RF.z), r3 = f2i(RF.w);
         float e[4];
         int32_t c[4];
         bool hh[4];
-        slab_entry4(LX, LY, LZ, HX, HY, HZ, r, low, widened<WINDOW>(q, limit), hh, e);
+        slab_entry4(NX, NY, NZ, FX, FY, FZ, r, low, widened<WINDOW>(q, limit), hh, e);
         const bool h0 = (r0 != wide_empty) & hh[0], h1 = (r1 != wide_empty) & hh[1];
         const bool h2 = (r2 != wide_empty) & hh[2], h3 = (r3 != wide_empty) & hh[3];
         if(STATS) boxes++;  // wide nodes visited (each a 128-byte record)
