@@ -308,7 +308,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(conv_waves<
 // X(p + offset(t))[ci] * dY[p][co], all nine taps t at once so that each
 // 64-pixel sub-block of dY is staged once.  Both operands are transposed
 // through LDS so that K (pixels) is contiguous per lane.  NCO = cpo / 16
-// output-channel tiles; 9 x 2 x NCO MFMA tiles spread over the 4 waves.
+// output-channel tiles; the 18 (tap, 16-channel input half) pairs are dealt
+// to the 4 waves, each wave computing all NCO output tiles of its pairs: the
+// dY fragments stay in registers for the whole sub-block and each X fragment
+// serves NCO MFMAs, so that the LDS reads per MFMA fall from 2 to about
+// 0.5-0.7.  Every tile accumulates its MFMAs in the same order (sub-block,
+// then the two 32-pixel steps).
 // The chunk-0 blocks also sum dY over the split's pixels into
 // bpartial[s][co] (the bias gradient's partials): each thread accumulates
 // the 8-channel groups it stages, in pixel order, and the threads' sums of a
@@ -330,10 +335,11 @@ __global__ __launch_bounds__(256) void wgrad_kernel(conv_src src, const uint16_t
     const int64_t nsub = (total + WG_PIX - 1) / WG_PIX;
     const int64_t sb0 = int64_t(blockIdx.x) * blocks_per_split;
     const int64_t sb1 = std::min(nsub, sb0 + blocks_per_split);
-    constexpr int TILES = 9 * 2 * NCO;
-    constexpr int TPW = (TILES + 3) / 4;
-    f32x4 acc[TPW];
-    for(int i = 0; i < TPW; i++) acc[i] = f32x4{0, 0, 0, 0};
+    constexpr int PAIRS = 9 * 2;
+    constexpr int PPW = (PAIRS + 3) / 4;
+    f32x4 acc[PPW][NCO];
+    for(int i = 0; i < PPW; i++)
+        for(int tj = 0; tj < NCO; tj++) acc[i][tj] = f32x4{0, 0, 0, 0};
     const int groups = cpo / 8;
     const int pxl = tid >> 2, g = tid & 3;  // this thread's pixel and 8-channel group of the X tile
     // bias sums of the dY groups this thread stages: item tid + 256 j has group (tid + 256 j) % groups
@@ -371,30 +377,35 @@ __global__ __launch_bounds__(256) void wgrad_kernel(conv_src src, const uint16_t
                 for(int e = 0; e < 8; e++) bsum[j][e] += from_bf16(u[e]);
         }
         __syncthreads();
-        for(int i = 0; i < TPW; i++) {
-            const int tile = wave + 4 * i;
-            if(tile >= TILES) break;
-            const int t = tile / (2 * NCO), r = tile - t * (2 * NCO);
-            const int ti = r & 1, tj = r >> 1;
-            for(int ks = 0; ks < WG_PIX / 32; ks++) {
+        constexpr int KS = WG_PIX / 32;
+        bf16x8 b[NCO][KS];
+        for(int tj = 0; tj < NCO; tj++)
+            for(int ks = 0; ks < KS; ks++)
+                b[tj][ks] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(
+                    &dt[(tj * 16 + col) * LDS_ROW + ks * 32 + 8 * kq]));
+        for(int i = 0; i < PPW; i++) {
+            const int pair = wave + 4 * i;
+            if(pair >= PAIRS) break;
+            const int t = pair >> 1, ti = pair & 1;
+            for(int ks = 0; ks < KS; ks++) {
                 const bf16x8 a = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(
                     &xt[(t * 32 + ti * 16 + col) * LDS_ROW + ks * 32 + 8 * kq]));
-                const bf16x8 b = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(
-                    &dt[(tj * 16 + col) * LDS_ROW + ks * 32 + 8 * kq]));
-                acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[i], 0, 0, 0);
+                for(int tj = 0; tj < NCO; tj++)
+                    acc[i][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[tj][ks], acc[i][tj], 0, 0, 0);
             }
         }
         __syncthreads();
     }
-    for(int i = 0; i < TPW; i++) {
-        const int tile = wave + 4 * i;
-        if(tile >= TILES) break;
-        const int t = tile / (2 * NCO), r = tile - t * (2 * NCO);
-        const int ti = r & 1, tj = r >> 1;
-        for(int v = 0; v < 4; v++) {
-            const int ci = kc * 32 + ti * 16 + 4 * kq + v, co = tj * 16 + col;
-            partial[((size_t(blockIdx.x) * 9 + size_t(t)) * size_t(cpi) + size_t(ci)) * size_t(cpo) + size_t(co)] = acc[i][v];
-        }
+    for(int i = 0; i < PPW; i++) {
+        const int pair = wave + 4 * i;
+        if(pair >= PAIRS) break;
+        const int t = pair >> 1, ti = pair & 1;
+        for(int tj = 0; tj < NCO; tj++)
+            for(int v = 0; v < 4; v++) {
+                const int ci = kc * 32 + ti * 16 + 4 * kq + v, co = tj * 16 + col;
+                partial[((size_t(blockIdx.x) * 9 + size_t(t)) * size_t(cpi) + size_t(ci)) * size_t(cpo) + size_t(co)] =
+                    acc[i][tj][v];
+            }
     }
     if(kc == 0) {
         // the threads' bias sums through LDS (xt is free after the loop's last barrier)
