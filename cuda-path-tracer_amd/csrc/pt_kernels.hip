@@ -775,6 +775,7 @@ __global__ __launch_bounds__(256) void trace_lanes_kernel(scene_dev s, frame_par
                         q.best.t = __int_as_float(bb.x); q.best.u = __int_as_float(bb.y);
                         q.best.v = __int_as_float(bb.z); q.best.code = bb.w;
                         q.limit = next_toward(q.best.t, high);
+                        q.wlimit = widened<wide_window_ulps>(q, q.limit);
                     }
                     const int2 *src = in.stack + size_t(item) * size_t(p.save_stride);
                     for(int k = 0; k < q.count; k++) { const int2 e = src[k]; wstk.store(k, e.x, __int_as_float(e.y)); }

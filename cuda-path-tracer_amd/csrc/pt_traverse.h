@@ -275,7 +275,8 @@ struct wide_query {
     // widened limit (widened()).  entry: the entry distance of the box of
     // the node or cluster being visited (-inf for a root without a box).
     // best.t is the closest hit so far, the upper bound before the first.
-    float limit, entry;
+    // wlimit: widened(limit), kept beside it (it changes only with a hit).
+    float limit, wlimit, entry;
     int32_t cur, count, best_rank;
     best_hit best;
     bool found, any;
@@ -289,7 +290,7 @@ constexpr int32_t wide_retrace = INT32_MIN + 2;
 PT_HD bool needs_retrace(const wide_query& q) { return q.cur == wide_retrace; }
 
 PT_HD void wide_begin(int32_t root, wide_query& q, bool any, float high) {
-    q.limit = high; q.entry = -INFINITY; q.cur = root; q.count = 0; q.best_rank = -1;
+    q.limit = high; q.wlimit = high; q.entry = -INFINITY; q.cur = root; q.count = 0; q.best_rank = -1;
     q.best.t = high; q.best.code = -1;
     q.found = false; q.any = any;
 }
@@ -328,7 +329,7 @@ PT_HD bool wide_step(const scene_dev& s, wide_query& q, STACK& stk, int& boxes, 
         float e[4];
         int32_t c[4];
         bool hh[4];
-        slab_entry4(NX, NY, NZ, FX, FY, FZ, r, low, widened<WINDOW>(q, limit), hh, e);
+        slab_entry4(NX, NY, NZ, FX, FY, FZ, r, low, q.wlimit, hh, e);
         const bool h0 = (r0 != wide_empty) & hh[0], h1 = (r1 != wide_empty) & hh[1];
         const bool h2 = (r2 != wide_empty) & hh[2], h3 = (r3 != wide_empty) & hh[3];
         if(STATS) boxes++;  // wide nodes visited (each a 128-byte record)
@@ -396,6 +397,7 @@ PT_HD bool wide_step(const scene_dev& s, wide_query& q, STACK& stk, int& boxes, 
                 // next_toward(t, upper bound): a hit has 0.001 < t < limit
                 limit = u2f(f2u(t) + 1u);
                 q.limit = limit;
+                q.wlimit = widened<WINDOW>(q, limit);
             }
         }
     }
@@ -413,7 +415,7 @@ PT_HD bool wide_step(const scene_dev& s, wide_query& q, STACK& stk, int& boxes, 
             q.entry = entry;
             return false;
         }
-        if(entry < widened<WINDOW>(q, limit)) {
+        if(entry < q.wlimit) {
             q.cur = wide_retrace;
             return true;
         }
