@@ -178,8 +178,9 @@ def bench_build(args, ptrace):
     """The device linear BVH builder (pt_build_bvh) on the frame's scene:
     device time per build over the timed steps (HIP events around the build's
     kernels), the call's wall time (host checks, copies in and out, the depth
-    walk), the host binned-SAH build of the same primitives, and the 16-spp
-    1080p render rate on each tree."""
+    walk), the host binned-SAH build of the same primitives, and per tree
+    the renderer's creation (host 4-wide layout and upload) and its 16-spp
+    1080p render rate."""
     scene, camera = ptrace.icosphere(FREQ, SEED, WIDTH, HEIGHT)
     arrays = (scene.triangles, scene.spheres, scene.materials, scene.textures, scene.pixels)
     n = len(scene.triangles) + len(scene.spheres)
@@ -195,9 +196,11 @@ def bench_build(args, ptrace):
     t = time.perf_counter()
     sah = ptrace.build_scene(*arrays, split="sah")
     host_ms = (time.perf_counter() - t) * 1e3
-    rates = {}
+    rates, create_ms = {}, {}
     for name, tree in (("sah", sah), ("device", built)):
+        t = time.perf_counter()
         r = ptrace.Renderer(tree, camera, WIDTH, HEIGHT, SEED, False, ptrace.DEBUG_SHADED, True, DEPTH)
+        create_ms[name] = round((time.perf_counter() - t) * 1e3, 1)
         r.add_samples(1)
         r.reset(camera)
         r.add_samples(16)
@@ -209,7 +212,7 @@ def bench_build(args, ptrace):
         "higher_is_better": True, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "device_ms": round(mean, 4), "call_ms": round(sum(call_ms) / len(call_ms), 2),
         "host_sah_ms": round(host_ms, 1), "host_sah_depth": sah.max_depth, "device_depth": built.max_depth,
-        "render_mrays_16spp": rates, "data": "synthetic",
+        "render_mrays_16spp": rates, "renderer_create_ms": create_ms, "data": "synthetic",
         "config": {"workload": f"device linear BVH of the {n}-triangle 1080p icosphere frame", "primitives": n}}))
     return 0
 

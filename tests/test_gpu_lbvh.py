@@ -1,7 +1,7 @@
 """Device linear BVH builder (pt_build_bvh) on the MI355X.
 
 The reference builds its hierarchy on the host (src/bvh/bvh.h:132-171); the
-device builder is an addition for scenes rebuilt often, so it is held to the
+device builder is an addition (a build in milliseconds), so it is held to the
 flat format's own rules and to the renderer's exactness on its trees.  Bars:
   * the tree passes the oracle's restatement of flat_validate.h:10-83 with
     the depth pt_build_bvh reports;
