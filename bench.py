@@ -312,8 +312,8 @@ def main():
         print(json.dumps({"copy_probe_bytes": 4 * n, "copies": 3}))
         return 0
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (pixels would be sharded by rows "
-              "across GPUs with one final gather; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path (pixels would be sharded by rows "
+              "across GPUs with one final gather). Run with --gpus 1.", file=sys.stderr)
         return 2
 
     import ptrace

@@ -1,7 +1,7 @@
 /*
  * pt_denoise.h -- learned denoiser for the progressive render (no
  * counterpart in the reference; the plain progressive render stays the
- * fallback and the judge of its error).
+ * fallback and the measure of its error).
  *
  * A 5-level convolutional encoder-decoder with skip connections (3x3
  * kernels, 32-48-64-80-96 channels, 2x2 average pooling, nearest

@@ -206,7 +206,7 @@ def test_config1_room_matches_oracle_state():
 
 def test_direct_lighting_only_per_pixel():
     # max_depth = 1: NEE at the first hit plus the BSDF-sampled light hit.
-    # The bar of the north star is 1e-5 relative per pixel (and exactly 0
+    # The direct-lighting bar is 1e-5 relative per pixel (and exactly 0
     # where the oracle is 0); the renderer meets it with equality.
     scene, cam = ptrace.cornell(11, 160, 100)
     r = _render(scene, cam, 160, 100, 3, [8], 0, True, max_depth=1)
