@@ -10,7 +10,8 @@ flat format's own rules and to the renderer's exactness on its trees.  Bars:
     triangle benchmark scene;
   * two builds of one input are byte-identical;
   * renders over device-built trees equal the oracle's renders over the same
-    arrays (accumulation sums, PCG states, ray counts) bit for bit;
+    arrays (accumulation sums, PCG states, ray counts) bit for bit, and a
+    batch over such trees equals single renderers;
   * depth 64 builds and depth 67 is refused (the 64-entry traversal stack,
     flat_data.h:52).
 """
@@ -139,3 +140,21 @@ def test_depth_limit():
     assert scene.max_depth == 64
     with pytest.raises(RuntimeError, match="deeper than 64"):
         ptrace.build_bvh_device(None, _chain(8))
+
+
+def test_batch_over_device_built_trees_matches_single_renderers():
+    # Device-built hierarchies in one batched launch: every view equals a
+    # single renderer over the same tree (and so the oracle, by the tests
+    # above), and the batch's ray count their sum.
+    built = [ptrace.cornell(3, 40, 30), ptrace.soup(6000, 2, 48, 32), ptrace.icosphere(20, 5, 36, 36)]
+    scenes = [_check_build(s.triangles, s.spheres, s.materials, s.textures, s.pixels)[0] for s, _ in built]
+    views = [(0, built[0][1], 40, 30, 7), (1, built[1][1], 48, 32, 9), (2, built[2][1], 36, 36, 3)]
+    batch = ptrace.Batch(scenes, views, direct_light=True, max_depth=6)
+    batch.add_samples(3)
+    total = 0
+    for k, (si, cam, w, h, seed) in enumerate(views):
+        r = ptrace.Renderer(scenes[si], cam, w, h, seed, False, ptrace.DEBUG_SHADED, True, 6)
+        r.add_samples(3)
+        assert np.array_equal(batch.image(k), r.image()), k
+        total += r.rays_traced()
+    assert batch.stats()["rays"] == total
