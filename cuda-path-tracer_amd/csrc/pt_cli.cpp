@@ -5,8 +5,11 @@
 // failure, 77 no device), same scene staging (srand48(seed); obj_scene;
 // flatten_scene) and the same ASCII PPM on stdout.  Additions:
 //   --backend hip      ("cuda" is accepted as the reference's name for it)
-//   --bvh reference|sah  hierarchy of the imported mesh (default: reference,
-//                        which reproduces the reference's output exactly)
+//   --bvh reference|sah|device  hierarchy of the imported mesh (default:
+//                        reference, which reproduces the reference's output
+//                        exactly; device: the linear BVH built on the GPU,
+//                        pt_build_bvh, over the SAH import's arrays, and for
+//                        the built-in scenes too)
 //   --max-depth N      path length limit (default 50, the reference's)
 //   --output FILE      write the PPM to FILE instead of stdout
 //   --scene obj|cornell|icosphere  built-in synthetic scenes
@@ -107,7 +110,8 @@ options parse_options(int argc, char **argv) {
         else if(a == "--seed") o.seed = parse_integer(value(), 0);
         else if(a == "--bvh") {
             o.bvh = value();
-            if(o.bvh != "reference" && o.bvh != "sah") throw std::invalid_argument("bvh must be reference or sah");
+            if(o.bvh != "reference" && o.bvh != "sah" && o.bvh != "device")
+                throw std::invalid_argument("bvh must be reference, sah or device");
         } else if(a == "--max-depth") o.max_depth = parse_integer(value(), 1);
         else if(a == "--output") o.output = value();
         else if(a == "--scene") {
@@ -139,7 +143,7 @@ const char *usage =
     "  [--obj-scale X Y Z] [--obj-rotate X Y Z] [--obj-translate X Y Z]\n"
     "  [--debug shaded|normals|triangle-id|material-id|bvh|intersections] [--normals]\n"
     "  [--lighting classic|mis] (default: mis)\n"
-    "  [--backend hip] [--bvh reference|sah] [--max-depth N] [--output FILE]\n"
+    "  [--backend hip] [--bvh reference|sah|device] [--max-depth N] [--output FILE]\n"
     "  [--scene obj|cornell|icosphere] [--frequency N]\n"
     "Defaults: HIP, Blender sphere, 500x500, 100 samples/pixel, seed 7.\n"
     "PPM output goes to stdout. OBJ transforms apply scale, XYZ rotation in degrees, then translation.\n";
@@ -183,6 +187,22 @@ int main(int argc, char **argv) {
         if(!scene) throw std::runtime_error(pt_scene_last_error());
         pt_scene_desc desc;
         pts_desc(scene, &desc);
+        std::vector<pt_flat_node> device_nodes;  // the device-built hierarchy, while desc points at it
+        if(o.bvh == "device" && desc.triangle_count + desc.sphere_count > 0) {
+            device_nodes.resize(size_t(2 * (desc.triangle_count + desc.sphere_count) - 1));
+            int32_t root = -1, depth = 0;
+            const pt_status built = pt_build_bvh(desc.triangles, desc.triangle_count, desc.spheres, desc.sphere_count,
+                                                 device_nodes.data(), int64_t(device_nodes.size()), &root, &depth,
+                                                 nullptr);
+            if(built != PT_OK) {
+                pts_free(scene);
+                check(built);
+            }
+            desc.nodes = device_nodes.data();
+            desc.node_count = int64_t(device_nodes.size());
+            desc.root = root;
+            desc.max_depth = depth;
+        }
         pt_settings settings{o.width, o.height, uint64_t(o.seed), 0, o.debug, o.direct_light ? 1 : 0, o.max_depth};
         pt_renderer *renderer = nullptr;
         pt_status status = pt_renderer_create(&desc, &camera, &settings, &renderer);

@@ -96,6 +96,28 @@ def test_cli_render_matches_oracle(debug, name):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("debug,name", [(4, "bvh"), (2, "triangle-id"), (0, "shaded")])
+def test_cli_device_hierarchy_matches_oracle(debug, name):
+    # --bvh device: the SAH import's arrays with the hierarchy built on the
+    # device (pt_build_bvh); against the oracle over the same arrays and tree
+    # (the bvh view counts that tree's box tests).
+    obj = os.path.join(ASSETS, "gallery", "gallery.obj")
+    w, h, samples, seed = 48, 32, 4 if debug == 0 else 1, 5
+    p = run("--obj", obj, "--width", w, "--height", h, "--samples", samples, "--seed", seed, "--debug", name,
+            "--bvh", "device", check=True)
+    scene, cam, _ = ptrace.load_obj(obj, w / h, split="sah", seed=seed)
+    scene, _ = ptrace.build_bvh_device(scene.triangles, scene.spheres, scene.materials, scene.textures, scene.pixels)
+    n = len(scene.triangles) + len(scene.spheres)
+    assert f"Loaded 472 triangles, {2 * n - 1} BVH nodes" in p.stderr
+    _, _, got = _ppm(p.stdout)
+    want = _oracle_ppm(scene, cam, w, h, seed, samples, debug, True)
+    if debug != 0:
+        assert np.array_equal(got, want)
+    else:
+        assert (got == want).all(axis=2).mean() >= 0.99
+
+
+@pytest.mark.gpu
 def test_cli_batch_split_is_byte_identical(tmp_path):
     # The reference's tests/render_tests.py:24 property: the PPM does not
     # depend on how the samples are split into batches.
