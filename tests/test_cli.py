@@ -89,10 +89,7 @@ def test_cli_render_matches_oracle(debug, name):
     _, _, got = _ppm(p.stdout)
     scene, cam, _ = ptrace.load_obj(obj, w / h, seed=seed)
     want = _oracle_ppm(scene, cam, w, h, seed, samples, debug, True)
-    if debug != 0:
-        assert np.array_equal(got, want)
-    else:
-        assert (got == want).all(axis=2).mean() >= 0.99
+    assert np.array_equal(got, want)
 
 
 @pytest.mark.gpu
@@ -111,10 +108,7 @@ def test_cli_device_hierarchy_matches_oracle(debug, name):
     assert f"Loaded 472 triangles, {2 * n - 1} BVH nodes" in p.stderr
     _, _, got = _ppm(p.stdout)
     want = _oracle_ppm(scene, cam, w, h, seed, samples, debug, True)
-    if debug != 0:
-        assert np.array_equal(got, want)
-    else:
-        assert (got == want).all(axis=2).mean() >= 0.99
+    assert np.array_equal(got, want)
 
 
 @pytest.mark.gpu
