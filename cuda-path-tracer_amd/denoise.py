@@ -286,11 +286,14 @@ def train(net, feats, targets, steps=2000, batch=16, lr=3e-4, seed=3, chunk=100,
 # ---------------------------------------------------------------- evaluation
 def time_to_relmse(scene, camera, width, height, net, target=0.01, reference_spp=16384,
                    plain_spp=(4, 8, 16, 24, 32, 40, 48, 56, 64, 96, 128, 256, 512, 1024, 2048, 4096),
-                   denoised_spp=(1, 4, 16), seed=7, log=print):
+                   denoised_spp=(1, 4, 16), seed=7, log=print, repeats=3):
     """Wall time for plain progressive accumulation and for render + denoise to
     reach relMSE <= target against a high-spp render of the same view.  Both
     forms deliver their image into the same page-locked host buffer
-    (ptrace.pinned), allocated once beforehand like the device buffers."""
+    (ptrace.pinned), allocated once beforehand like the device buffers.
+    Each form is timed `repeats` times from a reset renderer (the same
+    samples each time) and every reported time is the median, so that one
+    slow wall-clock sample of a few milliseconds does not decide the ratio."""
     result = ptrace.pinned((width * height, 3))
     ref = ptrace.Renderer(scene, camera, width, height, 99991, False, ptrace.DEBUG_SHADED, True, 8)
     t0 = time.perf_counter()
@@ -309,19 +312,26 @@ def time_to_relmse(scene, camera, width, height, net, target=0.01, reference_spp
     # so the timed samples are those of a fresh renderer.
     r = ptrace.Renderer(scene, camera, width, height, seed, False, ptrace.DEBUG_SHADED, True, 8)
     r.add_samples(1)
-    r.reset(camera)
-    have, spent = 0, 0.0
+    times = {spp: [] for spp in plain_spp}
+    errors = {}
+    for rep in range(repeats):
+        r.reset(camera)
+        have, spent = 0, 0.0
+        for spp in sorted(plain_spp):
+            t0 = time.perf_counter()
+            r.add_samples(spp - have)
+            spent += time.perf_counter() - t0
+            have = spp
+            # The image read back to the host is part of the result, as it is
+            # for the denoised form (denoise_renderer returns host memory).
+            t0 = time.perf_counter()
+            image = r.image(out=result)
+            times[spp].append(spent + time.perf_counter() - t0)
+            if rep == 0:
+                errors[spp] = relmse(image, reference)
     for spp in sorted(plain_spp):
-        t0 = time.perf_counter()
-        r.add_samples(spp - have)
-        spent += time.perf_counter() - t0
-        have = spp
-        # The image read back to the host is part of the result, as it is for
-        # the denoised form (denoise_renderer returns host memory).
-        t0 = time.perf_counter()
-        image = r.image(out=result)
-        readback = time.perf_counter() - t0
-        plain.append({"spp": spp, "seconds": spent + readback, "relmse": relmse(image, reference)})
+        plain.append({"spp": spp, "seconds": float(np.median(times[spp])), "relmse": errors[spp],
+                      "seconds_each": [round(t, 5) for t in times[spp]]})
         log("plain", plain[-1])
     # Size the network's buffers for this frame once, untimed (set-up, like
     # the renderers' own allocations).
@@ -334,13 +344,17 @@ def time_to_relmse(scene, camera, width, height, net, target=0.01, reference_spp
         r = ptrace.Renderer(scene, camera, width, height, seed, False, ptrace.DEBUG_SHADED, True, 8)
         r.enable_aovs()
         r.add_samples(1)
-        r.reset(camera)
-        t0 = time.perf_counter()
-        r.add_samples(spp)
-        out, ms = net.denoise_renderer(r, out=result)
-        secs = time.perf_counter() - t0
-        den.append({"spp": spp, "seconds": secs, "denoise_ms": ms, "relmse": relmse(out, reference),
-                    "noisy_relmse": relmse(r.image(), reference)})
+        secs, device_ms = [], []
+        for rep in range(repeats):
+            r.reset(camera)
+            t0 = time.perf_counter()
+            r.add_samples(spp)
+            out, ms = net.denoise_renderer(r, out=result)
+            secs.append(time.perf_counter() - t0)
+            device_ms.append(ms)
+        den.append({"spp": spp, "seconds": float(np.median(secs)), "denoise_ms": float(np.median(device_ms)),
+                    "relmse": relmse(out, reference), "noisy_relmse": relmse(r.image(), reference),
+                    "seconds_each": [round(t, 5) for t in secs]})
         log("denoised", den[-1])
 
     def first(rows):
