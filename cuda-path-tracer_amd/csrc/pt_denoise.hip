@@ -321,6 +321,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(conv_waves<
 constexpr int WG_PIX = 64;
 constexpr int LDS_ROW = WG_PIX + 8;  // bf16 per LDS row (16-byte aligned rows, fewer bank conflicts)
 
+// Column swizzles of the transposed tiles.  A row holds one channel's 64
+// pixels; 8 rows of 36 dwords wrap the LDS banks exactly, so the 8-channel
+// groups a wave writes at once (one row each, the same pixels) would land
+// on the same banks.  Each row's pixel column is XORed with a multiple of 8
+// pixels that depends on its channel group, which spreads those writes over
+// the banks and keeps every aligned 8-pixel block (an MFMA operand read)
+// contiguous.  X tile: a wave writes 16 pixels of 4 groups; dY tile: 64 / G
+// pixels of G = cpo / 8 groups.
+__device__ inline int x_swizzle(int row) { return 16 * ((row >> 3) & 3); }
+template<int G> __device__ inline int dy_swizzle(int row) {
+    return G == 4 ? 16 * ((row >> 3) & 3) : 8 * ((row >> 3) & 7);
+}
+
 template<int NCO>
 __global__ __launch_bounds__(256) void wgrad_kernel(conv_src src, const uint16_t *__restrict__ dy, int cpo,
                                                     int64_t blocks_per_split, int cpi, float *__restrict__ partial,
@@ -362,7 +375,7 @@ __global__ __launch_bounds__(256) void wgrad_kernel(conv_src src, const uint16_t
                 bf16x8 v = zero8();
                 if(p < total) v = gather8(src, n, y + t / 3 - 1, x + t % 3 - 1, kc * 32 + 8 * g);
                 const u16x8 u = __builtin_bit_cast(u16x8, v);
-                for(int j = 0; j < 8; j++) xt[(t * 32 + 8 * g + j) * LDS_ROW + pxl] = u[j];
+                for(int j = 0; j < 8; j++) xt[(t * 32 + 8 * g + j) * LDS_ROW + (pxl ^ (16 * g))] = u[j];
             }
         }
         for(int j = 0; j < BJ; j++) {  // dY tile, transposed into dt[co][px]
@@ -372,7 +385,7 @@ __global__ __launch_bounds__(256) void wgrad_kernel(conv_src src, const uint16_t
             const int64_t p = p0 + q;
             u16x8 u = u16x8{0, 0, 0, 0, 0, 0, 0, 0};
             if(p < total) u = *reinterpret_cast<const u16x8 *>(dy + size_t(p) * size_t(cpo) + size_t(8 * gg));
-            for(int e = 0; e < 8; e++) dt[(8 * gg + e) * LDS_ROW + q] = u[e];
+            for(int e = 0; e < 8; e++) dt[(8 * gg + e) * LDS_ROW + (q ^ dy_swizzle<2 * NCO>(8 * gg))] = u[e];
             if(kc == 0)
                 for(int e = 0; e < 8; e++) bsum[j][e] += from_bf16(u[e]);
         }
@@ -382,14 +395,14 @@ __global__ __launch_bounds__(256) void wgrad_kernel(conv_src src, const uint16_t
         for(int tj = 0; tj < NCO; tj++)
             for(int ks = 0; ks < KS; ks++)
                 b[tj][ks] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(
-                    &dt[(tj * 16 + col) * LDS_ROW + ks * 32 + 8 * kq]));
+                    &dt[(tj * 16 + col) * LDS_ROW + ((ks * 32 + 8 * kq) ^ dy_swizzle<2 * NCO>(tj * 16 + col))]));
         for(int i = 0; i < PPW; i++) {
             const int pair = wave + 4 * i;
             if(pair >= PAIRS) break;
             const int t = pair >> 1, ti = pair & 1;
             for(int ks = 0; ks < KS; ks++) {
                 const bf16x8 a = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(
-                    &xt[(t * 32 + ti * 16 + col) * LDS_ROW + ks * 32 + 8 * kq]));
+                    &xt[(t * 32 + ti * 16 + col) * LDS_ROW + ((ks * 32 + 8 * kq) ^ x_swizzle(ti * 16 + col))]));
                 for(int tj = 0; tj < NCO; tj++)
                     acc[i][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[tj][ks], acc[i][tj], 0, 0, 0);
             }
