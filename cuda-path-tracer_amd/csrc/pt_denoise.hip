@@ -651,12 +651,19 @@ __global__ __launch_bounds__(256) void loss_kernel(const float *out, const uint1
     }
     if(threadIdx.x == 0) block_loss[blockIdx.x] = red[0] * scale;
 }
-__global__ void sum_kernel(const float *v, int n, float *out) {
-    if(threadIdx.x == 0 && blockIdx.x == 0) {
-        float s = 0;
-        for(int i = 0; i < n; i++) s += v[i];
-        *out = s;
+// The sum of n values in a fixed order (one block of 64 threads): thread t
+// adds values t, t + 64, ... in turn, then a fixed tree over the threads.
+__global__ __launch_bounds__(64) void sum_kernel(const float *v, int n, float *out) {
+    __shared__ float part[64];
+    float s = 0;
+    for(int i = int(threadIdx.x); i < n; i += 64) s += v[i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for(int k = 32; k > 0; k >>= 1) {
+        if(int(threadIdx.x) < k) part[threadIdx.x] += part[threadIdx.x + k];
+        __syncthreads();
     }
+    if(threadIdx.x == 0) *out = part[0];
 }
 
 // Adam on the FP32 master parameters.
