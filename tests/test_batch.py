@@ -88,7 +88,7 @@ def test_batch_split_path_lanes_match_single_renderers():
     scenes, cams = _scenes()
     views = [(0, cams[0], 200, 150, 7), (2, cams[2], 180, 180, 3), (1, cams[1], 96, 64, 9)]
     batch = ptrace.Batch(scenes[:3], views, direct_light=True, max_depth=8)
-    batch.set_tuning(path_blocks_per_cu=1, path_sort_group=50)
+    batch.set_tuning(path_blocks_per_cu=1, path_sort_group=50, path_first_samples=4)
     batch.add_samples(3)
     batch.add_samples(6)
     assert batch.last_launch_stats()["split_launches"] == 1
