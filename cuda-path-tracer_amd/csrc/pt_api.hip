@@ -915,7 +915,9 @@ struct pt_batch {
     // pixels stay together, and so do the hierarchies they read, which keeps
     // the L2 useful for a batch of large scenes (64 soups of 10k-2M
     // triangles: 1520 against 1430 Mrays/s with two cost-ordered passes).
-    pt_tuning tuning{128, 256, 16, 8, 0, 0, 0, all_paths, 48, 1, 16, 32, 12, -1, 4, 8};
+    // Shading batches of 24 lanes: the batch's waves wait less for shading
+    // partners, which its memory-bound traversal prefers.
+    pt_tuning tuning{128, 256, 16, 8, 0, 0, 0, all_paths, 48, 1, 16, 24, 12, -1, 4, 8};
     int cus = 1;
     int32_t samples = 0;
     uint64_t rays = 0;
